@@ -1,0 +1,166 @@
+/*
+ * srfx.h — C ABI of the MI355X-native SRF projector library (libsrfx.so).
+ *
+ * Every entry point takes device pointers, plain sizes and a HIP stream
+ * (passed as void*, NULL = the default stream), enqueues its work on that
+ * stream without synchronising, and returns an int status (SRFX_OK = 0).
+ * Geometry is passed by value; nothing is copied back to the host.  The
+ * library holds no state between calls.
+ *
+ * Reference counterparts are TensorFlow custom ops of Hong-Xiang/SRF; paths
+ * below are relative to the reference repository root.
+ *
+ * Frames and layouts
+ *   LOR arrays are structure-of-arrays: column c of event e is lors[c*ld + e]
+ *   (the [ncols, N] tensor the reference ops slice row by row, tor.cc:140-160,
+ *   siddon.cc:108-122).  ld >= n.
+ *   An op works in "its" frame: x, y are the in-slice axes and z the slice axis
+ *   for TOR (the reference passes grid/center/size reversed and permutes the
+ *   image per axis group, physics/split_lors.py:54-89).  srfx_grid.stride gives
+ *   the element stride of one step along the op's x, y and z; all-zero strides
+ *   mean the reference layout, a contiguous [gz][gy][gx] array.  Non-zero
+ *   strides let a caller index an image stored in another axis order (for
+ *   example the unpermuted (nx, ny, nz) image) without transposing it.
+ *
+ * Outputs
+ *   By default an op first zeroes its output, as the reference ops do with
+ *   cudaMemset (tor.cc:132, 222, 310; siddon.cc:106, 171, 242).  With
+ *   SRFX_ACCUMULATE in flags the op adds into the output instead, which is how
+ *   the three TOR axis groups are summed into one image.
+ *
+ * Floating point
+ *   Scatter ops use FP32 atomics: the order of additions, and so the last bits
+ *   of the result, vary from run to run, as in the reference.
+ */
+#ifndef SRFX_H
+#define SRFX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum srfx_status {
+    SRFX_OK = 0,
+    SRFX_EINVAL = 1,      /* bad argument: null pointer, n < 0, ld < n, grid <= 0, size <= 0 */
+    SRFX_ELAUNCH = 2,     /* a HIP call or kernel launch failed */
+    SRFX_EUNSUPPORTED = 3 /* valid but unsupported (e.g. TOR patch wider than 16 voxels) */
+};
+
+#define SRFX_ACCUMULATE 1u
+
+typedef struct srfx_grid {
+    int32_t shape[3];  /* gx, gy, gz: voxels along the op's x, y, z */
+    float center[3];   /* centre of the image box, mm */
+    float size[3];     /* full extent of the box, mm (voxel pitch = size/shape) */
+    int64_t stride[3]; /* elements per step along x, y, z; {0,0,0} = contiguous [gz][gy][gx] */
+} srfx_grid;
+
+typedef struct srfx_tor_model {
+    float kernel_width;      /* attr kernel_width: sigma_tor = kernel_width / 6 */
+    int32_t geo_sigma2_flag; /* attr geo_sigma2_flag: scale sigma_tor^2 by the per-LOR factor */
+    float tof_bin;           /* attr tof_bin, mm */
+    float tof_sigma2;        /* attr tof_sigma2, mm^2; >= 405758 switches TOF off */
+} srfx_tor_model;
+
+/* ---------------------------------------------------------------- TOR --- */
+
+/* Replaces op "Projection" of tor.so: tor.cc:14-28 (registration), 106-181
+ * (Compute), tor.cu.cc:348-392 (host loop over slices), 244-278 (kernel).
+ * lors: 10 columns x1,y1,z1,x2,y2,z2,xc,yc,zc,sigma2_factor in the op frame.
+ * proj[e] = sum over strictly crossed slices and the PxP patch of
+ *           w_TOR * w_TOF * image[voxel]. */
+int srfx_tor_project(const float *lors, int64_t n, int64_t ld, const float *image,
+                     srfx_grid grid, srfx_tor_model model, float *proj, uint32_t flags,
+                     void *stream);
+
+/* Replaces op "Backprojection" of tor.so: tor.cc:30-47, 203-271;
+ * tor.cu.cc:394-433, 283-315, 169-209.
+ * image[voxel] += w / lors_value[e] for every event with lors_value[e] > 1e-7
+ * (the list-mode MLEM ratio folded into the scatter, A^T(1/v)). */
+int srfx_tor_backproject(const float *lors, int64_t n, int64_t ld, const float *lors_value,
+                         srfx_grid grid, srfx_tor_model model, float *image, uint32_t flags,
+                         void *stream);
+
+/* Plain adjoint image[voxel] += w * g[e], signed g, no threshold.  No
+ * counterpart in the reference (its backprojection divides by v); it is the
+ * backward pass of the differentiable projector. */
+int srfx_tor_backproject_adjoint(const float *lors, int64_t n, int64_t ld, const float *g,
+                                 srfx_grid grid, srfx_tor_model model, float *image,
+                                 uint32_t flags, void *stream);
+
+/* Replaces op "Maplors" of tor.so: tor.cc:49-64, 290-348; tor.cu.cc:435-481,
+ * 316-346, 212-242, 94-122.  lors: 7 columns x1..z2, sigma2_factor.
+ * image[voxel] += w_TOR / lors_value[e] (no TOF, no threshold). */
+int srfx_tor_maplors(const float *lors, int64_t n, int64_t ld, const float *lors_value,
+                     srfx_grid grid, float kernel_width, int32_t geo_sigma2_flag, float *image,
+                     uint32_t flags, void *stream);
+
+/* Fused list-mode OSEM subset pass for one axis group: for every event, the
+ * projection p_e is formed in registers and, when p_e > 1e-7, w / p_e is
+ * scattered into bp.  Equals srfx_tor_project followed by
+ * srfx_tor_backproject(lors_value = proj) (model/recon_step.py:33-41), without
+ * writing or re-reading p or re-reading the LORs. */
+int srfx_tor_osem_subset(const float *lors, int64_t n, int64_t ld, const float *image,
+                         srfx_grid image_grid, srfx_grid bp_grid, srfx_tor_model model,
+                         float *bp, uint32_t flags, void *stream);
+
+/* Number of non-zero (event, voxel) pairs of the TOR model: strictly crossed
+ * slice, voxel inside the grid and the patch, d^2 < 9 sigma_tor^2.
+ * nnz_per_event (nullable, n int32) and nnz_total (nullable, one device
+ * uint64, added to) receive the counts. */
+int srfx_tor_count_nnz(const float *lors, int64_t n, int64_t ld, srfx_grid grid,
+                       srfx_tor_model model, int32_t *nnz_per_event, uint64_t *nnz_total,
+                       void *stream);
+
+/* ------------------------------------------------------------- Siddon --- */
+
+/* Replaces op "Projection" of siddon.so: siddon.cc:14-26, 88-136;
+ * siddon.cu.cc:583-606, 424-471, 293-331.  lors: 7 columns x1..z2, tof.
+ * proj[e] = sum over traversed voxels of image * l / L^2 [* TOF weight]. */
+int srfx_siddon_project(const float *lors, int64_t n, int64_t ld, const float *image,
+                        srfx_grid grid, float tof_bin, float tof_sigma2, float *proj,
+                        uint32_t flags, void *stream);
+
+/* Replaces op "Backprojection" of siddon.so: siddon.cc:28-43, 152-205;
+ * siddon.cu.cc:614-641, 480-526, 339-381.  image += a / v for v > 0. */
+int srfx_siddon_backproject(const float *lors, int64_t n, int64_t ld, const float *lors_value,
+                            srfx_grid grid, float tof_bin, float tof_sigma2, float *image,
+                            uint32_t flags, void *stream);
+
+/* Plain adjoint image += a * g (no reference counterpart; autograd backward). */
+int srfx_siddon_backproject_adjoint(const float *lors, int64_t n, int64_t ld, const float *g,
+                                    srfx_grid grid, float tof_bin, float tof_sigma2,
+                                    float *image, uint32_t flags, void *stream);
+
+/* Replaces op "Maplors" of siddon.so: siddon.cc:45-57, 221-268;
+ * siddon.cu.cc:650-675, 535-575, 389-415.  Non-TOF, weight 1/(L*L),
+ * image += a / v for v > 0.  lors: at least 6 columns. */
+int srfx_siddon_maplors(const float *lors, int64_t n, int64_t ld, const float *lors_value,
+                        srfx_grid grid, float *image, uint32_t flags, void *stream);
+
+/* Fused Siddon OSEM subset pass: project, then scatter a / p for p > 0. */
+int srfx_siddon_osem_subset(const float *lors, int64_t n, int64_t ld, const float *image,
+                            srfx_grid grid, float tof_bin, float tof_sigma2, float *bp,
+                            uint32_t flags, void *stream);
+
+/* Non-zero Siddon segments (l > 0, and t^2/sigma^2 < 9 when TOF is on). */
+int srfx_siddon_count_nnz(const float *lors, int64_t n, int64_t ld, srfx_grid grid,
+                          float tof_bin, float tof_sigma2, int32_t *nnz_per_event,
+                          uint64_t *nnz_total, void *stream);
+
+/* ------------------------------------------------------------- Update --- */
+
+/* mlem_update, model/recon_step.py:48-49: x <- x * emap * bp (elementwise,
+ * n voxels, evaluated as (x * emap) * bp). */
+int srfx_mlem_update(float *x, const float *emap, const float *bp, int64_t n, void *stream);
+
+/* Status text for a return code. */
+const char *srfx_strerror(int status);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SRFX_H */

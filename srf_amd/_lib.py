@@ -1,0 +1,113 @@
+"""ctypes binding of libsrfx.so (the HIP kernels behind include/srfx.h).
+
+The library is built in-tree by ``python -m srf_amd.build`` (or
+``__graft_entry__.build()``).  Loading fails loudly when it is missing: there is
+no CPU fallback for any op of this package.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libsrfx.so")
+
+SRFX_OK = 0
+SRFX_EINVAL = 1
+SRFX_ELAUNCH = 2
+SRFX_EUNSUPPORTED = 3
+SRFX_ACCUMULATE = 1
+
+
+class SrfxGrid(ctypes.Structure):
+    """srfx_grid: shape (gx, gy, gz), center, size (full extent, mm) and the
+    element strides of one step along the op's x, y, z (all 0 = contiguous
+    [gz][gy][gx])."""
+    _fields_ = [("shape", ctypes.c_int32 * 3), ("center", ctypes.c_float * 3),
+                ("size", ctypes.c_float * 3), ("stride", ctypes.c_int64 * 3)]
+
+
+class SrfxTorModel(ctypes.Structure):
+    _fields_ = [("kernel_width", ctypes.c_float), ("geo_sigma2_flag", ctypes.c_int32),
+                ("tof_bin", ctypes.c_float), ("tof_sigma2", ctypes.c_float)]
+
+
+class SrfxError(RuntimeError):
+    def __init__(self, name, status, detail):
+        super().__init__(f"{name} failed with status {status} ({detail})")
+        self.status = status
+
+
+_P = ctypes.c_void_p
+_I64 = ctypes.c_int64
+_U32 = ctypes.c_uint32
+_F = ctypes.c_float
+_I32 = ctypes.c_int32
+
+# name -> argtypes (all return int)
+_PROTOS = {
+    "srfx_tor_project": [_P, _I64, _I64, _P, SrfxGrid, SrfxTorModel, _P, _U32, _P],
+    "srfx_tor_backproject": [_P, _I64, _I64, _P, SrfxGrid, SrfxTorModel, _P, _U32, _P],
+    "srfx_tor_backproject_adjoint": [_P, _I64, _I64, _P, SrfxGrid, SrfxTorModel, _P, _U32, _P],
+    "srfx_tor_maplors": [_P, _I64, _I64, _P, SrfxGrid, _F, _I32, _P, _U32, _P],
+    "srfx_tor_osem_subset": [_P, _I64, _I64, _P, SrfxGrid, SrfxGrid, SrfxTorModel, _P, _U32, _P],
+    "srfx_tor_count_nnz": [_P, _I64, _I64, SrfxGrid, SrfxTorModel, _P, _P, _P],
+    "srfx_siddon_project": [_P, _I64, _I64, _P, SrfxGrid, _F, _F, _P, _U32, _P],
+    "srfx_siddon_backproject": [_P, _I64, _I64, _P, SrfxGrid, _F, _F, _P, _U32, _P],
+    "srfx_siddon_backproject_adjoint": [_P, _I64, _I64, _P, SrfxGrid, _F, _F, _P, _U32, _P],
+    "srfx_siddon_maplors": [_P, _I64, _I64, _P, SrfxGrid, _P, _U32, _P],
+    "srfx_siddon_osem_subset": [_P, _I64, _I64, _P, SrfxGrid, _F, _F, _P, _U32, _P],
+    "srfx_siddon_count_nnz": [_P, _I64, _I64, SrfxGrid, _F, _F, _P, _P, _P],
+    "srfx_mlem_update": [_P, _P, _P, _I64, _P],
+}
+
+_lib = None
+
+
+def load():
+    """Load libsrfx.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"{LIB_PATH} is missing: build it with `python -m srf_amd.build` "
+            "(this package has no CPU fallback)")
+    lib = ctypes.CDLL(LIB_PATH)
+    for name, args in _PROTOS.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = ctypes.c_int
+    lib.srfx_strerror.argtypes = [ctypes.c_int]
+    lib.srfx_strerror.restype = ctypes.c_char_p
+    _lib = lib
+    return lib
+
+
+def exported_symbols():
+    return list(_PROTOS) + ["srfx_strerror"]
+
+
+def call(name, *args):
+    lib = load()
+    status = getattr(lib, name)(*args)
+    if status != SRFX_OK:
+        raise SrfxError(name, status, lib.srfx_strerror(status).decode())
+    return status
+
+
+def make_grid(shape, center, size, stride=(0, 0, 0)):
+    g = SrfxGrid()
+    for k in range(3):
+        g.shape[k] = int(shape[k])
+        g.center[k] = float(center[k])
+        g.size[k] = float(size[k])
+        g.stride[k] = int(stride[k])
+    return g
+
+
+def make_tor_model(kernel_width, geo_sigma2_flag, tof_bin, tof_sigma2):
+    m = SrfxTorModel()
+    m.kernel_width = float(kernel_width)
+    m.geo_sigma2_flag = 1 if geo_sigma2_flag else 0
+    m.tof_bin = float(tof_bin)
+    m.tof_sigma2 = float(tof_sigma2)
+    return m

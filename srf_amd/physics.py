@@ -1,0 +1,197 @@
+"""Projection models: the reference's physics layer on the HIP ops.
+
+SplitLoRsModel    TOR model on axis-split LORs (src/python/srf/physics/split_lors.py)
+CompleteLoRsModel Siddon model on complete LORs (src/python/srf/physics/complete_lors.py)
+
+The reference transposes the image into each group's frame before every op
+and back afterwards, then sums the three groups (split_lors.py:77-113).  Here
+each group's op indexes the world image (nx, ny, nz) directly through element
+strides, and the three backprojections accumulate into one output image; the
+grid, center and size seen by the kernels are the reference's permuted ones,
+so the results are the same.
+"""
+import torch
+
+from . import _lib
+from .data import Image, ListModeData, ListModeDataSplit
+from .ops import _check_dev, _ptr, _soa, _stream, _vec
+
+# Axis permutation that brings the world image into a group's frame
+# (split_lors.py:54-61); the op frame is the permuted image's axes reversed.
+PERM = {"x": [0, 2, 1], "y": [1, 2, 0], "z": [2, 1, 0]}
+
+
+def group_frame(image_shape, center, size, strides, axis):
+    """Grid/center/size/strides of the op frame of group ``axis`` for a world
+    image of shape (nx, ny, nz) with element ``strides``."""
+    p = PERM[axis][::-1]  # op axes x, y, z -> world axes
+    grid = [int(image_shape[k]) for k in p]
+    cen = [float(center[k]) for k in p]
+    siz = [float(size[k]) for k in p]
+    st = [int(strides[k]) for k in p]
+    return grid, cen, siz, st
+
+
+def _world(image):
+    data = image.data
+    _check_dev(data, "image")
+    if data.dim() != 3:
+        raise ValueError("image data must be (nx, ny, nz)")
+    return data
+
+
+class SplitLoRsModel:
+    """TOR projection model (split_lors.py:28-52).  tof_bin and tof_sigma2
+    default to 1e4 as in the reference (TOF stays on: 1e4 < 405758)."""
+
+    AXIS = ("x", "y", "z")
+
+    def __init__(self, kernel_width, geo_sigma2_flag=False, tof_sigma2=None, tof_bin=None):
+        self.kernel_width = float(kernel_width)
+        self.geo_sigma2_flag = bool(geo_sigma2_flag)
+        self.tof_sigma2 = 1.0e4 if tof_sigma2 is None else float(tof_sigma2)
+        self.tof_bin = 1.0e4 if tof_bin is None else float(tof_bin)
+
+    def _model(self):
+        return _lib.make_tor_model(self.kernel_width, self.geo_sigma2_flag, self.tof_bin,
+                                   self.tof_sigma2)
+
+    def _grid(self, image, axis):
+        data = _world(image)
+        g, c, s, st = group_frame(data.shape, image.center, image.size, data.stride(), axis)
+        return _lib.make_grid(g, c, s, st)
+
+    def projection(self, image, data):
+        """split_lors.py:73-90: per group p = Projection(...)."""
+        x = _world(image)
+        out = {}
+        for a in self.AXIS:
+            lm = data[a]
+            lors, n, ld = _soa(lm.soa, 10)
+            p = torch.empty(n, dtype=torch.float32, device=x.device)
+            if n:
+                _lib.call("srfx_tor_project", _ptr(lors), n, ld, _ptr(x), self._grid(image, a),
+                          self._model(), _ptr(p), 0, _stream())
+            out[a] = lm.replace_values(p)
+        return ListModeDataSplit(out["x"], out["y"], out["z"])
+
+    def backprojection(self, data, image, out=None):
+        """split_lors.py:93-113: sum over groups of Backprojection(lors_value)."""
+        x = _world(image)
+        if out is None:
+            out = torch.zeros_like(x)
+        for a in self.AXIS:
+            lm = data[a]
+            lors, n, ld = _soa(lm.soa, 10)
+            if n == 0:
+                continue
+            v = _vec(lm.values, n, "lors_value")
+            _lib.call("srfx_tor_backproject", _ptr(lors), n, ld, _ptr(v),
+                      self._grid(Image(out, image.center, image.size), a), self._model(),
+                      _ptr(out), _lib.SRFX_ACCUMULATE, _stream())
+        return Image(out, image.center, image.size)
+
+    def maplors(self, data, image, out=None):
+        """split_lors.py:116-133: sum over groups of Maplors(lors_value)."""
+        x = _world(image)
+        if out is None:
+            out = torch.zeros_like(x)
+        for a in self.AXIS:
+            lm = data[a]
+            lors, n, ld = _soa(lm.soa, 7)
+            if n == 0:
+                continue
+            v = _vec(lm.values, n, "lors_value")
+            _lib.call("srfx_tor_maplors", _ptr(lors), n, ld, _ptr(v),
+                      self._grid(Image(out, image.center, image.size), a),
+                      self.kernel_width, 1 if self.geo_sigma2_flag else 0, _ptr(out),
+                      _lib.SRFX_ACCUMULATE, _stream())
+        return Image(out, image.center, image.size)
+
+    def osem_subset(self, image, data, out=None):
+        """Fused projection + ratio backprojection of all groups
+        (recon_step.py:33-41 without writing the projections)."""
+        x = _world(image)
+        if out is None:
+            out = torch.zeros_like(x)
+        for a in self.AXIS:
+            lm = data[a]
+            lors, n, ld = _soa(lm.soa, 10)
+            if n == 0:
+                continue
+            g_in = self._grid(image, a)
+            g_out = self._grid(Image(out, image.center, image.size), a)
+            _lib.call("srfx_tor_osem_subset", _ptr(lors), n, ld, _ptr(x), g_in, g_out,
+                      self._model(), _ptr(out), _lib.SRFX_ACCUMULATE, _stream())
+        return Image(out, image.center, image.size)
+
+    def count_nnz(self, image, data):
+        """Total non-zero (event, voxel) pairs over the groups (int64 tensor)."""
+        x = _world(image)
+        tot = torch.zeros(1, dtype=torch.int64, device=x.device)
+        for a in self.AXIS:
+            lors, n, ld = _soa(data[a].soa, 10)
+            if n:
+                _lib.call("srfx_tor_count_nnz", _ptr(lors), n, ld, self._grid(image, a),
+                          self._model(), None, _ptr(tot), _stream())
+        return tot
+
+
+class CompleteLoRsModel:
+    """Siddon projection model (complete_lors.py:33-56).  The reference
+    transposes the image [2,1,0] and passes the transposed grid, center and
+    size reversed (complete_lors.py:61-70), so the op's x, y, z are the world
+    x, y, z and only the memory order changes; strides express that order."""
+
+    def __init__(self, name="siddon", tof_sigma2=None, tof_bin=None):
+        self.name = name
+        self.tof_sigma2 = 1.0e4 if tof_sigma2 is None else float(tof_sigma2)
+        self.tof_bin = 1.0e4 if tof_bin is None else float(tof_bin)
+
+    @staticmethod
+    def _grid(image):
+        data = _world(image)
+        return _lib.make_grid(list(data.shape), image.center, image.size, list(data.stride()))
+
+    def projection(self, image, data):
+        x = _world(image)
+        lors, n, ld = _soa(data.soa, 7)
+        p = torch.empty(n, dtype=torch.float32, device=x.device)
+        if n:
+            _lib.call("srfx_siddon_project", _ptr(lors), n, ld, _ptr(x), self._grid(image),
+                      self.tof_bin, self.tof_sigma2, _ptr(p), 0, _stream())
+        return data.replace_values(p)
+
+    def backprojection(self, data, image, out=None):
+        x = _world(image)
+        if out is None:
+            out = torch.zeros_like(x)
+        lors, n, ld = _soa(data.soa, 7)
+        if n:
+            v = _vec(data.values, n, "lors_value")
+            _lib.call("srfx_siddon_backproject", _ptr(lors), n, ld, _ptr(v),
+                      self._grid(Image(out, image.center, image.size)), self.tof_bin,
+                      self.tof_sigma2, _ptr(out), _lib.SRFX_ACCUMULATE, _stream())
+        return Image(out, image.center, image.size)
+
+    def maplors(self, data, image, out=None):
+        x = _world(image)
+        if out is None:
+            out = torch.zeros_like(x)
+        lors, n, ld = _soa(data.soa, 6)
+        if n:
+            v = _vec(data.values, n, "lors_value")
+            _lib.call("srfx_siddon_maplors", _ptr(lors), n, ld, _ptr(v),
+                      self._grid(Image(out, image.center, image.size)), _ptr(out),
+                      _lib.SRFX_ACCUMULATE, _stream())
+        return Image(out, image.center, image.size)
+
+
+def projection(model, image, data):
+    """model/_projection.py:37-40 dispatch."""
+    return model.projection(image, data)
+
+
+def backprojection(model, data, image):
+    """model/_backprojection.py:37-41 dispatch."""
+    return model.backprojection(data, image)

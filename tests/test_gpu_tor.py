@@ -1,0 +1,146 @@
+"""GPU parity of the TOR ops (libsrfx.so through the C ABI) against the CPU
+oracle: the FP64 restatement of tor.cu.cc and the FP32 port.
+
+The kernels carry positions in FP64 and weights in FP32, and take every
+include/exclude decision as the FP64 oracle does.  Tolerances:
+  projection / backprojection vs the FP64 oracle: rel L2 <= 1e-5
+  vs the FP32 port (the reference's own FP32 arithmetic, whose crossing-point
+  rounding alone moves the narrow tube weights by ~1e-5): rel L2 <= 1e-4
+  non-zero counts: bit-exact against the FP64 oracle.
+Backprojection uses FP32 atomics, so its last bits vary between runs; the
+spread is measured in test_bp_atomic_spread.
+"""
+import numpy as np
+import pytest
+
+from tests import helpers as H
+
+pytestmark = pytest.mark.gpu
+
+TOL64 = 1e-5
+TOL32 = 1e-4
+
+
+def _t(a, dev):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+
+
+@pytest.fixture(scope="module")
+def case(cuda):
+    lors = H.tor_lors(6000, seed=1)
+    img = H.image(H.TOR_GRID, seed=1)
+    return lors, img
+
+
+@pytest.mark.parametrize("tof", [True, False])
+@pytest.mark.parametrize("geo", [False, True])
+def test_project(cuda, case, tof, geo):
+    from oracle import lib as O
+    from srf_amd import ops
+    lors, img = case
+    ts2 = H.TOF_SIGMA2 if tof else 1.0e6
+    args = (H.TOR_GRID, H.TOR_CENTER, H.TOR_SIZE, H.KW, geo, H.TOF_BIN, ts2)
+    p = ops.projection(_t(lors, cuda), _t(img, cuda), *args).cpu().numpy()
+    r64 = O.tor_project(lors, img, *args, prec="f64")
+    r32 = O.tor_project(lors, img, *args, prec="f32")
+    assert np.count_nonzero(r64) > 0.6 * lors.shape[1]
+    e64, e32 = H.rel_l2(p, r64), H.rel_l2(p, r32)
+    print(f"tor project tof={tof} geo={geo}: rel_l2 vs f64 {e64:.2e}, vs f32 {e32:.2e}")
+    assert e64 <= TOL64
+    assert e32 <= TOL32
+
+
+@pytest.mark.parametrize("tof", [True, False])
+def test_backproject(cuda, case, tof):
+    from oracle import lib as O
+    from srf_amd import ops
+    lors, img = case
+    ts2 = H.TOF_SIGMA2 if tof else 1.0e6
+    rng = np.random.default_rng(5)
+    v = rng.uniform(0.5, 2.0, lors.shape[1]).astype(np.float32)
+    v[::17] = 0.0          # below the 1e-7 threshold: skipped
+    v[1::23] = 1e-8
+    args = (H.TOR_GRID, H.TOR_CENTER, H.TOR_SIZE)
+    b = ops.backprojection(_t(img, cuda), _t(lors, cuda), _t(v, cuda), *args, H.TOF_BIN, ts2,
+                           H.KW, False).cpu().numpy()
+    r64 = O.tor_backproject(lors, v, *args, H.KW, False, H.TOF_BIN, ts2, prec="f64")
+    r32 = O.tor_backproject(lors, v, *args, H.KW, False, H.TOF_BIN, ts2, prec="f32")
+    e64, e32 = H.rel_l2(b, r64), H.rel_l2(b, r32)
+    print(f"tor bp tof={tof}: rel_l2 vs f64 {e64:.2e}, vs f32 {e32:.2e}")
+    assert e64 <= TOL64
+    assert e32 <= TOL32
+
+
+def test_map(cuda, case):
+    from oracle import lib as O
+    from srf_amd import ops
+    lors, img = case
+    l7 = np.ascontiguousarray(np.vstack([lors[0:6], lors[9:10]]))
+    v = np.ones(l7.shape[1], np.float32)
+    args = (H.TOR_GRID, H.TOR_CENTER, H.TOR_SIZE)
+    m = ops.maplors(_t(img, cuda), *args, _t(l7, cuda), _t(v, cuda), H.KW, True).cpu().numpy()
+    r64 = O.tor_map(l7, v, *args, H.KW, True, prec="f64")
+    e64 = H.rel_l2(m, r64)
+    print(f"tor map: rel_l2 vs f64 {e64:.2e}")
+    assert e64 <= TOL64
+
+
+def test_count_exact(cuda, case):
+    from oracle import lib as O
+    from srf_amd import ops
+    lors, _ = case
+    args = (H.TOR_GRID, H.TOR_CENTER, H.TOR_SIZE, H.KW, True)
+    per, tot = ops.count_nnz(_t(lors, cuda), *args)
+    per = per.cpu().numpy()
+    ref = O.tor_count(lors, *args, prec="f64")
+    assert np.array_equal(per, ref)
+    assert int(tot.item()) == int(ref.sum())
+    ref32 = O.tor_count(lors, *args, prec="f32")
+    # the FP32 port's decisions differ from FP64 ones only at rounding boundaries
+    assert np.abs(per - ref32).sum() <= 1e-4 * ref32.sum() + 2
+
+
+def test_adjoint(cuda, case):
+    from srf_amd import ops
+    import torch
+    lors, img = case
+    rng = np.random.default_rng(7)
+    y = rng.normal(0, 1, lors.shape[1]).astype(np.float32)
+    args = (H.TOR_GRID, H.TOR_CENTER, H.TOR_SIZE)
+    L, X, Y = _t(lors, cuda), _t(img, cuda), _t(y, cuda)
+    Ax = ops.projection(L, X, *args, H.KW, False, H.TOF_BIN, H.TOF_SIGMA2)
+    ATy = ops.backprojection_adjoint(X, L, Y, *args, H.TOF_BIN, H.TOF_SIGMA2, H.KW, False)
+    lhs = float(torch.dot(Ax.double(), Y.double()))
+    rhs = float(torch.dot(X.reshape(-1).double(), ATy.reshape(-1).double()))
+    assert abs(lhs - rhs) / abs(lhs) <= 1e-5
+    # reference-semantics bp with v = 1/y (y > 0) is the same adjoint
+    yp = rng.uniform(0.5, 2.0, lors.shape[1]).astype(np.float32)
+    Yp = _t(yp, cuda)
+    b1 = ops.backprojection(X, L, 1.0 / Yp, *args, H.TOF_BIN, H.TOF_SIGMA2, H.KW, False)
+    b2 = ops.backprojection_adjoint(X, L, Yp, *args, H.TOF_BIN, H.TOF_SIGMA2, H.KW, False)
+    assert H.rel_l2(b1.cpu().numpy(), b2.cpu().numpy()) <= 1e-6
+
+
+def test_osem_fused_equals_two_ops(cuda, case):
+    from srf_amd import ops
+    lors, img = case
+    args = (H.TOR_GRID, H.TOR_CENTER, H.TOR_SIZE)
+    L, X = _t(lors, cuda), _t(img, cuda)
+    p = ops.projection(L, X, *args, H.KW, True, H.TOF_BIN, H.TOF_SIGMA2)
+    b = ops.backprojection(X, L, p, *args, H.TOF_BIN, H.TOF_SIGMA2, H.KW, True)
+    f = ops.osem_subset(L, X, *args, H.KW, True, H.TOF_BIN, H.TOF_SIGMA2)
+    assert H.rel_l2(f.cpu().numpy(), b.cpu().numpy()) <= 1e-6
+
+
+def test_bp_atomic_spread(cuda, case):
+    from srf_amd import ops
+    lors, img = case
+    args = (H.TOR_GRID, H.TOR_CENTER, H.TOR_SIZE)
+    L, X = _t(lors, cuda), _t(img, cuda)
+    v = _t(np.ones(lors.shape[1], np.float32), cuda)
+    runs = [ops.backprojection(X, L, v, *args, H.TOF_BIN, H.TOF_SIGMA2, H.KW, False).cpu().numpy()
+            for _ in range(3)]
+    spread = max(H.rel_l2(r, runs[0]) for r in runs)
+    print(f"tor bp run-to-run rel_l2 spread {spread:.2e}")
+    assert spread <= 1e-6
