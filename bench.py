@@ -1,0 +1,227 @@
+"""TOF OSEM throughput benchmark on one MI355X.
+
+Workload (tof_tor_c2): TOF tube-of-response list-mode OSEM, 200x200x100 image
+of 2.0x2.0x2.2 mm, 20M synthetic events (cylinder + 6 spheres, 530 ps FWHM,
+kernel_width 4 mm), 10 subsets.  A step is one OSEM iteration: every event is
+projected and backprojected once (10 subsets x 3 axis groups of the fused
+projection + ratio-backprojection kernel, then the image update).  Events are
+generated, preprocessed and the efficiency map built on the GPU before the
+timed region; inputs are resident in HBM when it starts.
+
+Prints one JSON line:  value = events * steps / time (Mevents/s, proj+bp);
+roofline = the fused kernel's algorithmic bytes per launch over its mean
+launch time (HIP events on the launch stream); cpu_baseline = the FP32 CPU
+port of the reference kernels on a bounded sample of the same events.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HBM_PEAK_GBS = 8000.0
+LOR_BYTES = 40  # 10 float32 columns per TOR LOR record
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--events", type=int, default=20_000_000)
+    ap.add_argument("--subsets", type=int, default=10)
+    ap.add_argument("--effmap-pairs", type=int, default=0,
+                    help="crystal pairs for the efficiency map (0 = all pairs)")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0,
+                    help="target wall time of the CPU baseline sample (0 = skip)")
+    ap.add_argument("--unfused", action="store_true",
+                    help="time separate projection + backprojection kernels")
+    return ap.parse_args()
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(groups_host, meta, target_s):
+    """FP32 CPU port of tor.cu.cc (projection + ratio backprojection) on the
+    first events of the x group, sized to about target_s seconds."""
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    from oracle import lib as O
+    O.build()
+    lors = groups_host
+    threads = len(os.sched_getaffinity(0))
+    g = meta["frame_x"]
+    args = (g["grid"], g["center"], g["size"], meta["kernel_width"], False, meta["tof_bin"],
+            meta["tof_sigma2"])
+    ones = np.ones(int(np.prod(g["grid"])), np.float32)
+
+    def run(n):
+        sub = np.ascontiguousarray(lors[:, :n])
+        t0 = time.perf_counter()
+        p = O.tor_project(sub, ones, *args, prec="f32", nthreads=threads)
+        t1 = time.perf_counter()
+        O.tor_backproject(sub, p.astype(np.float32), g["grid"], g["center"], g["size"],
+                          meta["kernel_width"], False, meta["tof_bin"], meta["tof_sigma2"],
+                          prec="f32", nthreads=threads)
+        t2 = time.perf_counter()
+        return t1 - t0, t2 - t1
+
+    n = min(20000, lors.shape[1])
+    tp, tb = run(n)
+    scale = target_s / max(tp + tb, 1e-3)
+    n = int(min(lors.shape[1], max(n, n * scale)))
+    tp, tb = run(n)
+    return {"value": n / (tp + tb) / 1e6, "unit": "Mevents/s", "cores": threads,
+            "kind": "port",
+            "sample": f"first {n} events of the x axis group, projection {tp:.2f}s + ratio "
+                      f"backprojection {tb:.2f}s, FP32 port of tor.cu.cc (all {g['grid'][2]} "
+                      f"slices tested per event, as the reference does), {threads} threads"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this benchmark runs on one GPU; multi-GPU runs are not "
+                          "part of this build (use --gpus 1)"}))
+        sys.exit(2)
+    import torch
+    from srf_amd import effmap, physics, synth, preprocess
+    from srf_amd.data import Image, split_from_groups
+    from srf_amd.recon import OSEM, subset_view
+    from srf_amd.scanner import CylindricalScanner
+
+    dev = torch.device("cuda:0")
+    torch.cuda.init()
+    scanner = CylindricalScanner()
+    t0 = time.time()
+    ph, ev, meta = synth.c2(n_events=args.events, device=dev)
+    torch.cuda.synchronize()
+    log(f"generated {args.events} events in {time.time() - t0:.1f}s")
+    t0 = time.time()
+    groups = preprocess.recon_process(ev, scanner.tof_res_ps)
+    del ev
+    torch.cuda.synchronize()
+    log(f"preprocessed in {time.time() - t0:.1f}s: " +
+        ", ".join(f"{a}={groups[a].shape[1]}" for a in "xyz"))
+    grid, center, size = meta["grid"], meta["center"], meta["size"]
+    model = physics.SplitLoRsModel(meta["kernel_width"], meta["geo_sigma2_flag"],
+                                   tof_sigma2=meta["tof_sigma2"], tof_bin=meta["tof_bin"])
+    t0 = time.time()
+    npairs = args.effmap_pairs or None
+    summed = effmap.compute_tor_effmap(scanner, model, grid, center, size, dev,
+                                       max_pairs=npairs)
+    E = effmap.finalize(summed, grid, center, size, scanner.inner_radius, scanner.axial_length)
+    torch.cuda.synchronize()
+    log(f"efficiency map over {npairs or scanner.nb_pairs()} crystal pairs in "
+        f"{time.time() - t0:.1f}s")
+    x = effmap.initial_image(grid, center, size, scanner.inner_radius, scanner.axial_length, dev)
+    image = Image(x, center, size)
+    emap = Image(E, center, size)
+    data = split_from_groups(groups)
+    osem = OSEM(model, data, emap, nb_subsets=args.subsets, fused=not args.unfused)
+
+    # exact non-zero counts per launch (x, y, z group of each subset)
+    nnz = {}
+    for k, sub in enumerate(osem.subsets):
+        for a in "xyz":
+            one = split_from_groups({b: (sub[b].soa if b == a else sub[b].soa[:, :0])
+                                     for b in "xyz"})
+            nnz[(k, a)] = int(model.count_nnz(image, one).item())
+    n_launch = {(k, a): len(osem.subsets[k][a]) for k in range(args.subsets) for a in "xyz"}
+    total_nnz = sum(nnz.values())
+    log(f"nnz per event {total_nnz / args.events:.1f}")
+
+    # per-launch HIP events around each group's fused kernel (torch's current
+    # stream is the launch stream)
+    timings = {}
+    orig = model.osem_subset
+
+    def timed_osem(img, sub, out=None):
+        for a in "xyz":
+            if len(sub[a]) == 0:
+                continue
+            only = physics.ListModeDataSplit(*[sub[b] if b == a else sub[b].replace_values(None)
+                                               for b in "xyz"])
+            for b in "xyz":
+                if b != a:
+                    only[b].soa = sub[b].soa[:, :0]
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            orig(img, only, out=out)
+            e1.record()
+            timings.setdefault(a, []).append((e0, e1, sub[a].soa.data_ptr()))
+        return Image(out, img.center, img.size)
+
+    if not args.unfused:
+        model.osem_subset = timed_osem
+    for _ in range(args.warmup):
+        osem.iterate(image, 1)
+    torch.cuda.synchronize()
+    timings.clear()
+    t0 = time.perf_counter()
+    s0, s1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    s0.record()
+    osem.iterate(image, args.steps)
+    s1.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    gpu_ms = s0.elapsed_time(s1)
+    ms_per_step = gpu_ms / args.steps
+    value = args.events * args.steps / (gpu_ms / 1e3) / 1e6
+    log(f"{args.steps} iterations: {gpu_ms:.1f} ms (wall {wall * 1e3:.1f} ms), "
+        f"{value:.1f} Mevents/s")
+    assert torch.isfinite(image.data).all().item()
+
+    roof = None
+    if timings:
+        nvox = int(np.prod(grid))
+        # dominant kernel: the x group's fused pass (the z group is empty here)
+        a = max(timings, key=lambda k: sum(e0.elapsed_time(e1) for e0, e1, _ in timings[k]))
+        durs = [e0.elapsed_time(e1) for e0, e1, _ in timings[a]]
+        per_sub = [(n_launch[(k, a)], nnz[(k, a)]) for k in range(args.subsets)]
+        nb = len(durs)
+        launches = [per_sub[i % args.subsets] for i in range(nb)]
+        byts = [n * (LOR_BYTES + 4) + 8 * z + 2 * 4 * nvox for n, z in launches]
+        mean_bytes = float(np.mean(byts))
+        mean_ms = float(np.mean(durs))
+        achieved = mean_bytes / (mean_ms / 1e3) / 1e9
+        roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel": "tor_kernel<kOsem> (fused projection + ratio backprojection)",
+                "group": a, "mean_launch_ms": round(mean_ms, 4),
+                "algorithmic_bytes_per_launch": int(mean_bytes),
+                "bytes_formula": "sum_i (40 + 4 + 8*nnz_i) + 2*4*N_vox"}
+
+    cpu = None
+    if args.cpu_seconds > 0:
+        host = groups["x"][:, :400000].cpu().numpy()
+        f = physics.group_frame(grid, center, size, (1, 1, 1), "x")
+        meta["frame_x"] = {"grid": f[0], "center": f[1], "size": f[2]}
+        try:
+            cpu = cpu_baseline(host, meta, args.cpu_seconds)
+        except Exception as exc:  # the baseline must not hide the GPU result
+            cpu = {"error": repr(exc)}
+
+    out = {
+        "metric": "TOF OSEM Mevents/s (proj+backproj) + % HBM roofline; batched/grad vs plain",
+        "value": round(value, 2), "unit": "Mevents/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "tof_tor_c2", "image": list(grid), "voxel_mm": list(meta["voxel"]),
+                   "events": args.events, "subsets": args.subsets,
+                   "step": "one OSEM iteration over all events",
+                   "kernel_width_mm": meta["kernel_width"], "tof_fwhm_ps": 530,
+                   "tof_sigma2_mm2": round(meta["tof_sigma2"], 3), "tof_bin_mm": meta["tof_bin"],
+                   "nnz_per_event": round(total_nnz / args.events, 2),
+                   "fused": not args.unfused},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,89 @@
+"""List-mode MLEM / OSEM driver.
+
+Follows the reference's semantics:
+  ReconStep.kernel + mlem_update   x <- x * E * sum_groups A^T(1 / A x)
+                                    (src/python/srf/model/recon_step.py:33-49)
+  subsets                          contiguous chunks of each group's rows,
+                                    subset s of K = rows [s*N/K, (s+1)*N/K)
+                                    (graph/reconstruction/worker.py:79-85)
+  subset counter                   (s + 1) % K (master.py:97-111)
+  iterations                       LocalReconstructionGraph.run
+                                    (local_reconstruction.py:44-51)
+E is the stored inverse efficiency map and is applied unscaled to every
+subset, as the reference does.
+"""
+import torch
+
+from . import ops
+from .data import Image, ListModeDataSplit
+
+
+def subset_bounds(n, k, nb_subsets):
+    """Rows of subset k: [k*n//K, (k+1)*n//K).  For n divisible by K this is
+    the reference's contiguous split (test_worker.py:105-121 pins N=30, K=3)."""
+    return (k * n) // nb_subsets, ((k + 1) * n) // nb_subsets
+
+
+def subset_view(data, k, nb_subsets):
+    """Column views (no copy) of each group's SoA LORs for subset k."""
+    parts = []
+    for a in ("x", "y", "z"):
+        lm = data[a]
+        lo, hi = subset_bounds(len(lm), k, nb_subsets)
+        v = lm.values[lo:hi] if lm.values is not None else None
+        part = lm.replace_values(v)
+        part.soa = lm.soa[:, lo:hi]
+        parts.append(part)
+    return ListModeDataSplit(*parts)
+
+
+def mlem_update(image_prev, image_succ, efficiency_map):
+    """recon_step.py:48-49: x * E * bp, in place on image_prev.data."""
+    ops.mlem_update(image_prev.data, efficiency_map.data, image_succ.data)
+    return image_prev
+
+
+class ReconStep:
+    """One reconstruction step with the reference's structure
+    (recon_step.py:17-45): projection, ratio backprojection, update."""
+
+    def __init__(self, model, fused=True):
+        self.model = model
+        self.fused = fused
+        self._bp = None
+
+    def __call__(self, image, data, efficiency_map):
+        if self._bp is None or self._bp.shape != image.data.shape:
+            self._bp = torch.empty_like(image.data)
+        self._bp.zero_()
+        if self.fused:
+            self.model.osem_subset(image, data, out=self._bp)
+        else:
+            proj = self.model.projection(image, data)
+            self.model.backprojection(proj, image, out=self._bp)
+        return mlem_update(image, Image(self._bp, image.center, image.size), efficiency_map)
+
+
+class OSEM:
+    """iterations x subsets of ReconStep over list-mode data split in groups."""
+
+    def __init__(self, model, data, efficiency_map, nb_subsets=1, fused=True):
+        self.step = ReconStep(model, fused=fused)
+        self.data = data
+        self.emap = efficiency_map
+        self.nb_subsets = int(nb_subsets)
+        self.subsets = [subset_view(data, k, self.nb_subsets) for k in range(self.nb_subsets)]
+        self.subset = 0
+
+    def run_subset(self, image):
+        self.step(image, self.subsets[self.subset], self.emap)
+        self.subset = (self.subset + 1) % self.nb_subsets
+        return image
+
+    def iterate(self, image, nb_iterations=1, callback=None):
+        for it in range(nb_iterations):
+            for _ in range(self.nb_subsets):
+                self.run_subset(image)
+            if callback is not None:
+                callback(it, image)
+        return image
