@@ -26,7 +26,8 @@
  *   By default an op first zeroes its output, as the reference ops do with
  *   cudaMemset (tor.cc:132, 222, 310; siddon.cc:106, 171, 242).  With
  *   SRFX_ACCUMULATE in flags the op adds into the output instead, which is how
- *   the three TOR axis groups are summed into one image.
+ *   the three TOR axis groups are summed into one image.  SRFX_FULL_PATCH
+ *   selects the reference's full patch loop in the TOR ops.
  *
  * Floating point
  *   Scatter ops use FP32 atomics: the order of additions, and so the last bits
@@ -49,6 +50,10 @@ enum srfx_status {
 };
 
 #define SRFX_ACCUMULATE 1u
+/* TOR ops: evaluate every voxel of the reference's P x P patch (its loop
+ * structure) instead of only the voxels on the tube's chords.  Same results;
+ * slower; kept for verification. */
+#define SRFX_FULL_PATCH 2u
 
 typedef struct srfx_grid {
     int32_t shape[3];  /* gx, gy, gz: voxels along the op's x, y, z */

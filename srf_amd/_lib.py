@@ -15,6 +15,7 @@ SRFX_EINVAL = 1
 SRFX_ELAUNCH = 2
 SRFX_EUNSUPPORTED = 3
 SRFX_ACCUMULATE = 1
+SRFX_FULL_PATCH = 2
 
 
 class SrfxGrid(ctypes.Structure):

@@ -46,7 +46,9 @@ struct TorParams {
     double zbase; // center_z - (lz - inter_z)/2: slice s is at zbase + s*inter_z
     double tor_sigma2;
     float finter_x, finter_y;
+    float rfinter_x, rfinter_y;
     int patch, half;
+    int full_patch; // evaluate every patch voxel (reference loop) instead of the chords
     int geo;
     int use_tof;
     float tse_inv; // 1 / (tof_sigma2 + tof_bin^2/12)
@@ -164,84 +166,158 @@ __device__ __forceinline__ bool load_event(const TorParams &p, int64_t e, Event 
     return slice_range(p, ev);
 }
 
-// Crossing point of slice s and the patch origin (tor.cu.cc:35-36, 135-136).
-// Returns offsets (FP32) of the crossing point from the first patch voxel
-// centre along x and y, in mm.
-struct Cross {
-    double cx, cy;
-    int ix0, iy0;
-    float dx0, dy0;
+// Per-event geometry carried across slices.  The crossing point of slice s,
+// in voxel units from the grid corner, is affine in s: u(s) = ax + s*bx,
+// w(s) = ay + s*by (FP64).  The vector from the crossing point to the TOF
+// centre is v0 - (s - s_lo)*dv (FP32).
+struct Track {
+    double ax, bx, ay, by;
+    float v0x, v0y, v0z, dvx, dvy, dvz;
+    // in-slice ellipse d2 < tm: d2 = a (dx - cxy dy)^2 + q dy^2
+    float A;     // 1 / (1 - ux^2)
+    float cxy;   // ux uy / (1 - ux^2)
+    float q;     // uz^2 / (1 - ux^2)
+    float tm;    // cut-off padded so the candidate set contains every kept voxel
+    float dymax; // sqrt(tm / q)
 };
 
-__device__ __forceinline__ bool near_boundary(double c, double bound, double rinter)
+__device__ __forceinline__ void make_track(const TorParams &p, const Event &ev, Track &t)
 {
-    double u = (c - bound) * rinter;
+    double r0 = (p.zbase - ev.z1) * ev.rd2;        // ratio at slice 0
+    double rs = p.inter_z * ev.rd2;                // ratio step per slice
+    t.ax = (ev.x1 + ev.d0 * r0 - p.l_bound) * p.rinter_x;
+    t.bx = ev.d0 * rs * p.rinter_x;
+    t.ay = (ev.y1 + ev.d1 * r0 - p.b_bound) * p.rinter_y;
+    t.by = ev.d1 * rs * p.rinter_y;
+    double rl = r0 + (double)ev.s_lo * rs;
+    t.v0x = (float)((double)ev.xc - (ev.x1 + ev.d0 * rl));
+    t.v0y = (float)((double)ev.yc - (ev.y1 + ev.d1 * rl));
+    t.v0z = (float)((double)ev.zc - slice_z(p, ev.s_lo));
+    t.dvx = (float)(ev.d0 * rs);
+    t.dvy = (float)(ev.d1 * rs);
+    t.dvz = (float)p.inter_z;
+    double ux2 = ev.ux_d * ev.ux_d, uy2 = ev.uy_d * ev.uy_d;
+    double a = 1.0 - ux2;
+    double dis2 = ev.d0 * ev.d0 + ev.d1 * ev.d1 + ev.d2 * ev.d2;
+    double uz2 = ev.d2 * ev.d2 / dis2;
+    t.A = (float)(1.0 / a);
+    t.cxy = (float)(ev.ux_d * ev.uy_d / a);
+    t.q = (float)(uz2 / a);
+    t.tm = ev.thr * (1.0f + 1.0e-5f) + 2.0f * kThrMargin;
+    t.dymax = sqrtf(t.tm / t.q);
+    (void)uy2;
+}
+
+struct Cross {
+    int ix0, iy0;
+    float dx0, dy0; // crossing point minus centre of the first patch voxel, mm
+};
+
+__device__ __forceinline__ bool near_int(double u)
+{
     double fr = u - floor(u);
     return fr < 1.0e-9 || fr > 1.0 - 1.0e-9;
 }
 
-// Crossing point and patch origin evaluated exactly as the restated reference
-// does (tor.cu.cc:30-36, 135-136): division, then multiply and add, in FP64.
-__device__ __forceinline__ void cross_exact(const TorParams &p, const Event &ev, double zs, Cross &c)
+// The reference's crossing point of slice s (tor.cu.cc:30-36) in FP64 with
+// its operation order.
+__device__ __forceinline__ void cross_point_exact(const TorParams &p, const Event &ev, int s,
+                                                  double &cx, double &cy)
 {
-    double ratio = __ddiv_rn(__dsub_rn(zs, ev.z1), ev.d2);
-    c.cx = __dadd_rn(ev.x1, __dmul_rn(ev.d0, ratio));
-    c.cy = __dadd_rn(ev.y1, __dmul_rn(ev.d1, ratio));
-    c.ix0 = (int)__ddiv_rn(__dsub_rn(c.cx, p.l_bound), p.inter_x) - p.half;
-    c.iy0 = (int)__ddiv_rn(__dsub_rn(c.cy, p.b_bound), p.inter_y) - p.half;
+    double ratio = __ddiv_rn(__dsub_rn(slice_z(p, s), ev.z1), ev.d2);
+    cx = __dadd_rn(ev.x1, __dmul_rn(ev.d0, ratio));
+    cy = __dadd_rn(ev.y1, __dmul_rn(ev.d1, ratio));
 }
 
-__device__ __forceinline__ void cross_at(const TorParams &p, const Event &ev, int s, Cross &c)
+__device__ __forceinline__ double mesh_x(const TorParams &p, int ix)
 {
-    double zs = slice_z(p, s);
-    double ratio = (zs - ev.z1) * ev.rd2;
-    c.cx = ev.x1 + ev.d0 * ratio;
-    c.cy = ev.y1 + ev.d1 * ratio;
-    if (near_boundary(c.cx, p.l_bound, p.rinter_x) || near_boundary(c.cy, p.b_bound, p.rinter_y)) {
-        cross_exact(p, ev, zs, c);
+    return __dadd_rn(__dmul_rn(p.inter_x, (double)ix + 0.5), p.l_bound);
+}
+__device__ __forceinline__ double mesh_y(const TorParams &p, int iy)
+{
+    return __dadd_rn(__dmul_rn(p.inter_y, (double)iy + 0.5), p.b_bound);
+}
+
+// Patch origin (tor.cu.cc:135-136) and offsets for slice s.  Near a voxel
+// boundary the origin is recomputed with the reference's exact divisions.
+__device__ __forceinline__ void cross_at(const TorParams &p, const Event &ev, const Track &t, int s,
+                                         Cross &c)
+{
+    double u = t.ax + (double)s * t.bx;
+    double w = t.ay + (double)s * t.by;
+    if (near_int(u) || near_int(w)) {
+        double cx, cy;
+        cross_point_exact(p, ev, s, cx, cy);
+        c.ix0 = (int)__ddiv_rn(__dsub_rn(cx, p.l_bound), p.inter_x) - p.half;
+        c.iy0 = (int)__ddiv_rn(__dsub_rn(cy, p.b_bound), p.inter_y) - p.half;
+        c.dx0 = (float)(cx - mesh_x(p, c.ix0));
+        c.dy0 = (float)(cy - mesh_y(p, c.iy0));
     } else {
-        c.ix0 = (int)((c.cx - p.l_bound) * p.rinter_x) - p.half;
-        c.iy0 = (int)((c.cy - p.b_bound) * p.rinter_y) - p.half;
+        c.ix0 = (int)u - p.half;
+        c.iy0 = (int)w - p.half;
+        c.dx0 = (float)((u - ((double)c.ix0 + 0.5)) * p.inter_x);
+        c.dy0 = (float)((w - ((double)c.iy0 + 0.5)) * p.inter_y);
     }
-    double mx0 = p.inter_x * ((double)c.ix0 + 0.5) + p.l_bound;
-    double my0 = p.inter_y * ((double)c.iy0 + 0.5) + p.b_bound;
-    c.dx0 = (float)(c.cx - mx0);
-    c.dy0 = (float)(c.cy - my0);
 }
 
 // Exact FP64 cut-off decision of the reference (tor.cu.cc:57-71) for voxel
-// (ix, iy) of a slice whose crossing point is (cx, cy).
-__device__ __forceinline__ bool inside_exact(const TorParams &p, const Event &ev, double cx,
-                                          double cy, int ix, int iy)
+// (ix, iy) of slice s.
+__device__ __forceinline__ bool inside_exact(const TorParams &p, const Event &ev, int s, int ix,
+                                             int iy)
 {
-    double mx = __dadd_rn(__dmul_rn(p.inter_x, (double)ix + 0.5), p.l_bound);
-    double my = __dadd_rn(__dmul_rn(p.inter_y, (double)iy + 0.5), p.b_bound);
-    double dx = __dsub_rn(cx, mx), dy = __dsub_rn(cy, my);
+    double cx, cy;
+    cross_point_exact(p, ev, s, cx, cy);
+    double dx = __dsub_rn(cx, mesh_x(p, ix)), dy = __dsub_rn(cy, mesh_y(p, iy));
     double rc = __dadd_rn(__dmul_rn(dx, ev.ux_d), __dmul_rn(dy, ev.uy_d));
     double d2 = __dsub_rn(__dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy)), __dmul_rn(rc, rc));
     return d2 < 9.0 * ev.sc_d;
 }
 
-// Per-slice TOF factor tof_bin/sqrt(2 pi s2) * exp(-D/(2 tse)) where D is the
-// squared distance from the TOF centre to the crossing point; the reference's
+// Per-slice TOF factor tof_bin/sqrt(2 pi s2) * exp(-D/(2 tse)), D the squared
+// distance from the TOF centre to the crossing point; the reference's
 // d2_tof = D - d2 (tor.cu.cc:87) has its -d2 part folded into the per-voxel
 // exponent.
-__device__ __forceinline__ float tof_factor(const TorParams &p, const Event &ev, const Cross &c,
+__device__ __forceinline__ float tof_factor(const TorParams &p, const Event &ev, const Track &t,
                                             int s)
 {
-    float ax = (float)((double)ev.xc - c.cx);
-    float ay = (float)((double)ev.yc - c.cy);
-    float az = (float)((double)ev.zc - slice_z(p, s));
-    float D = ax * ax + ay * ay + az * az;
+    float k = (float)(s - ev.s_lo);
+    float vx = t.v0x - k * t.dvx, vy = t.v0y - k * t.dvy, vz = t.v0z - k * t.dvz;
+    float D = vx * vx + vy * vy + vz * vz;
     return p.tofk * __builtin_amdgcn_exp2f(D * ev.ctof);
 }
 
-// Walk one slice's patch (tor.cu.cc:124-166 and siblings): j outer, i inner,
-// skipping voxels outside the grid.
-//   KIND 0: returns sum w*img; 1: out += w*scale; 2: returns the count.
+// One voxel: exact include test, then gather / scatter / count.
 template <int KIND>
-__device__ __forceinline__ float patch_walk(const TorParams &p, const Event &ev, const Cross &c,
-                                            float amp, int s, float scale)
+__device__ __forceinline__ void visit(const TorParams &p, const Event &ev, int s, int ix, int iy,
+                                      float dx, float dy, float amp, float scale, float &acc)
+{
+    float rc = dx * ev.ux + dy * ev.uy;
+    float d2 = dx * dx + dy * dy - rc * rc;
+    bool in = d2 < ev.thr - kThrMargin;
+    if (!in && d2 < ev.thr + kThrMargin)
+        in = inside_exact(p, ev, s, ix, iy);
+    if (in) {
+        if (KIND == 2) {
+            acc += 1.0f;
+        } else {
+            float w = amp * __builtin_amdgcn_exp2f(d2 * ev.cw);
+            if (KIND == 0)
+                acc += w * p.img[ix * p.sx + iy * p.sy + s * p.sz];
+            else
+                unsafeAtomicAdd(p.out + (ix * p.ox + iy * p.oy + s * p.oz), w * scale);
+        }
+    }
+}
+
+// Walk one slice's patch (tor.cu.cc:124-166 and siblings), j outer, i inner,
+// voxels outside the grid skipped.  FULL evaluates all P x P voxels as the
+// reference loop does; otherwise each row only evaluates the voxels whose
+// centres fall in the analytically computed chord of the (padded) tube
+// ellipse, a superset of the kept voxels, and the exact test decides.
+//   KIND 0: returns sum w*img; 1: out += w*scale; 2: returns the count.
+template <int KIND, bool FULL>
+__device__ __forceinline__ float patch_walk(const TorParams &p, const Event &ev, const Track &t,
+                                            const Cross &c, float amp, int s, float scale)
 {
     const int P = p.patch;
     // A patch entirely outside the grid contributes nothing; this also keeps
@@ -249,33 +325,32 @@ __device__ __forceinline__ float patch_walk(const TorParams &p, const Event &ev,
     if (c.ix0 <= -P || c.ix0 >= p.gx || c.iy0 <= -P || c.iy0 >= p.gy)
         return 0.0f;
     int i_lo = c.ix0 < 0 ? -c.ix0 : 0;
-    int i_hi = min(P, p.gx - c.ix0);
+    int i_hi = min(P, p.gx - c.ix0) - 1;
     int j_lo = c.iy0 < 0 ? -c.iy0 : 0;
-    int j_hi = min(P, p.gy - c.iy0);
+    int j_hi = min(P, p.gy - c.iy0) - 1;
     float acc = 0.0f;
-    const float lo = ev.thr - kThrMargin, hi = ev.thr + kThrMargin;
-    for (int j = j_lo; j < j_hi; j++) {
+    if (!FULL) {
+        const float eps = 1.0e-4f;
+        j_lo = max(j_lo, (int)ceilf((c.dy0 - t.dymax) * p.rfinter_y - eps));
+        j_hi = min(j_hi, (int)floorf((c.dy0 + t.dymax) * p.rfinter_y + eps));
+    }
+    for (int j = j_lo; j <= j_hi; j++) {
         int iy = c.iy0 + j;
         float dy = c.dy0 - (float)j * p.finter_y;
-        for (int i = i_lo; i < i_hi; i++) {
-            int ix = c.ix0 + i;
+        int il = i_lo, ih = i_hi;
+        if (!FULL) {
+            float tt = t.tm - dy * dy * t.q;
+            if (tt <= 0.0f)
+                continue;
+            float hw = sqrtf(tt * t.A);
+            float ctr = c.dx0 - t.cxy * dy;
+            const float eps = 1.0e-4f;
+            il = max(il, (int)ceilf((ctr - hw) * p.rfinter_x - eps));
+            ih = min(ih, (int)floorf((ctr + hw) * p.rfinter_x + eps));
+        }
+        for (int i = il; i <= ih; i++) {
             float dx = c.dx0 - (float)i * p.finter_x;
-            float rc = dx * ev.ux + dy * ev.uy;
-            float d2 = dx * dx + dy * dy - rc * rc;
-            bool in = d2 < lo;
-            if (!in && d2 < hi)
-                in = inside_exact(p, ev, c.cx, c.cy, ix, iy);
-            if (in) {
-                if (KIND == 2) {
-                    acc += 1.0f;
-                } else {
-                    float w = amp * __builtin_amdgcn_exp2f(d2 * ev.cw);
-                    if (KIND == 0)
-                        acc += w * p.img[ix * p.sx + iy * p.sy + s * p.sz];
-                    else
-                        unsafeAtomicAdd(p.out + (ix * p.ox + iy * p.oy + s * p.oz), w * scale);
-                }
-            }
+            visit<KIND>(p, ev, s, c.ix0 + i, iy, dx, dy, amp, scale, acc);
         }
     }
     return acc;
@@ -285,12 +360,17 @@ template <int KIND>
 __device__ __forceinline__ float walk_event(const TorParams &p, const Event &ev, bool tof,
                                             float scale)
 {
+    Track t;
+    make_track(p, ev, t);
     float acc = 0.0f;
     for (int s = ev.s_lo; s <= ev.s_hi; s++) {
         Cross c;
-        cross_at(p, ev, s, c);
-        float amp = tof ? tof_factor(p, ev, c, s) : 1.0f;
-        acc += patch_walk<KIND>(p, ev, c, amp, s, scale);
+        cross_at(p, ev, t, s, c);
+        float amp = tof ? tof_factor(p, ev, t, s) : 1.0f;
+        if (p.full_patch)
+            acc += patch_walk<KIND, true>(p, ev, t, c, amp, s, scale);
+        else
+            acc += patch_walk<KIND, false>(p, ev, t, c, amp, s, scale);
     }
     return acc;
 }
@@ -363,6 +443,8 @@ int setup(TorParams &p, const float *lors, int64_t n, int64_t ld, const srfx_gri
     p.rinter_y = 1.0 / p.inter_y;
     p.finter_x = (float)p.inter_x;
     p.finter_y = (float)p.inter_y;
+    p.rfinter_x = (float)(1.0 / p.inter_x);
+    p.rfinter_y = (float)(1.0 / p.inter_y);
     p.l_bound = (double)g.center[0] - lx / 2;
     p.b_bound = (double)g.center[1] - ly / 2;
     p.zbase = (double)g.center[2] - (lz - p.inter_z) / 2;
@@ -417,7 +499,6 @@ extern "C" int srfx_tor_project(const float *lors, int64_t n, int64_t ld, const 
                                 srfx_grid grid, srfx_tor_model m, float *proj, uint32_t flags,
                                 void *stream)
 {
-    (void)flags;
     TorParams p;
     int rc = setup(p, lors, n, ld, grid, m.kernel_width, m.geo_sigma2_flag, m.tof_bin,
                    m.tof_sigma2);
@@ -425,6 +506,7 @@ extern "C" int srfx_tor_project(const float *lors, int64_t n, int64_t ld, const 
         return rc;
     if (n > 0 && (!image || !proj))
         return SRFX_EINVAL;
+    p.full_patch = (flags & SRFX_FULL_PATCH) ? 1 : 0;
     p.img = image;
     p.out = proj;
     return launch<kProj>(p, (hipStream_t)stream);
@@ -445,6 +527,7 @@ static int tor_scatter_common(int mode, const float *lors, int64_t n, int64_t ld
     rc = zero_grid(image, grid, flags, st);
     if (rc)
         return rc;
+    p.full_patch = (flags & SRFX_FULL_PATCH) ? 1 : 0;
     p.val = vals;
     p.out = image;
     switch (mode) {
@@ -498,6 +581,7 @@ extern "C" int srfx_tor_osem_subset(const float *lors, int64_t n, int64_t ld, co
     p.ox = (int)so.x;
     p.oy = (int)so.y;
     p.oz = (int)so.z;
+    p.full_patch = (flags & SRFX_FULL_PATCH) ? 1 : 0;
     p.img = image;
     p.out = bp;
     return launch<kOsem>(p, st);
@@ -513,5 +597,6 @@ extern "C" int srfx_tor_count_nnz(const float *lors, int64_t n, int64_t ld, srfx
         return rc;
     p.nnz = nnz_per_event;
     p.nnz_total = (unsigned long long *)nnz_total;
+    p.full_patch = 0;
     return launch<kCount>(p, (hipStream_t)stream);
 }

@@ -144,3 +144,16 @@ def test_bp_atomic_spread(cuda, case):
     spread = max(H.rel_l2(r, runs[0]) for r in runs)
     print(f"tor bp run-to-run rel_l2 spread {spread:.2e}")
     assert spread <= 1e-6
+
+
+@pytest.mark.parametrize("geo", [False, True])
+def test_chords_equal_full_patch(cuda, case, geo):
+    """The chord walk visits exactly the voxels the full P x P patch keeps,
+    in the same order: projections are bit-identical."""
+    from srf_amd import ops
+    lors, img = case
+    args = (H.TOR_GRID, H.TOR_CENTER, H.TOR_SIZE, H.KW, geo, H.TOF_BIN, H.TOF_SIGMA2)
+    L, X = _t(lors, cuda), _t(img, cuda)
+    a = ops.projection(L, X, *args)
+    b = ops.projection(L, X, *args, full_patch=True)
+    assert np.array_equal(a.cpu().numpy(), b.cpu().numpy())
