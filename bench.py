@@ -45,13 +45,13 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(groups_host, meta, target_s):
+def cpu_baseline(lors_dev, meta, target_s):
     """FP32 CPU port of tor.cu.cc (projection + ratio backprojection) on the
-    first events of the x group, sized to about target_s seconds."""
+    first events of the x group (a device tensor, copied to the host as
+    needed), sized to about target_s seconds of wall time."""
     sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
     from oracle import lib as O
     O.build()
-    lors = groups_host
     threads = len(os.sched_getaffinity(0))
     g = meta["frame_x"]
     args = (g["grid"], g["center"], g["size"], meta["kernel_width"], False, meta["tof_bin"],
@@ -59,7 +59,7 @@ def cpu_baseline(groups_host, meta, target_s):
     ones = np.ones(int(np.prod(g["grid"])), np.float32)
 
     def run(n):
-        sub = np.ascontiguousarray(lors[:, :n])
+        sub = np.ascontiguousarray(lors_dev[:, :n].cpu().numpy())
         t0 = time.perf_counter()
         p = O.tor_project(sub, ones, *args, prec="f32", nthreads=threads)
         t1 = time.perf_counter()
@@ -69,10 +69,11 @@ def cpu_baseline(groups_host, meta, target_s):
         t2 = time.perf_counter()
         return t1 - t0, t2 - t1
 
-    n = min(20000, lors.shape[1])
+    total = lors_dev.shape[1]
+    n = min(50000, total)
     tp, tb = run(n)
     scale = target_s / max(tp + tb, 1e-3)
-    n = int(min(lors.shape[1], max(n, n * scale)))
+    n = int(min(total, max(n, n * scale)))
     tp, tb = run(n)
     return {"value": n / (tp + tb) / 1e6, "unit": "Mevents/s", "cores": threads,
             "kind": "port",
@@ -197,11 +198,10 @@ def main():
 
     cpu = None
     if args.cpu_seconds > 0:
-        host = groups["x"][:, :400000].cpu().numpy()
         f = physics.group_frame(grid, center, size, (1, 1, 1), "x")
         meta["frame_x"] = {"grid": f[0], "center": f[1], "size": f[2]}
         try:
-            cpu = cpu_baseline(host, meta, args.cpu_seconds)
+            cpu = cpu_baseline(groups["x"], meta, args.cpu_seconds)
         except Exception as exc:  # the baseline must not hide the GPU result
             cpu = {"error": repr(exc)}
 

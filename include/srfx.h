@@ -155,6 +155,44 @@ int srfx_siddon_count_nnz(const float *lors, int64_t n, int64_t ld, srfx_grid gr
                           float tof_bin, float tof_sigma2, int32_t *nnz_per_event,
                           uint64_t *nnz_total, void *stream);
 
+/* ------------------------------------------------------------ Batched --- */
+
+/* One list-mode TOF reconstruction of a batch.  The image, the inverse
+ * efficiency map and the backprojection buffer are C-order (nx, ny, nz)
+ * float32 arrays (the reference's L3 image layout, data/image.py:4-28).  The
+ * three LOR groups are 10-row SoA arrays in their group frames, as
+ * on_tor_lors.py:251-262 produces them (op frame of group x = world (y, z, x),
+ * group y = (x, z, y), group z = (x, y, z); physics/split_lors.py:54-85). */
+typedef struct srfx_recon {
+    int32_t shape[3];
+    float center[3];
+    float size[3];
+    float *image;
+    const float *emap;
+    float *bp;
+    const float *lors[3];
+    int64_t n[3];
+    int64_t ld[3];
+} srfx_recon;
+
+typedef struct srfx_tor_batch srfx_tor_batch;
+
+/* Plan a batch of independent OSEM reconstructions that share one TOR model
+ * and one subset count.  Allocates the plan's device tables; no reference
+ * counterpart (the reference reconstructs one image per process,
+ * graph/reconstruction/local_reconstruction.py:44-51). */
+int srfx_tor_batch_create(const srfx_recon *recons, int32_t nrecon, int32_t nb_subsets,
+                          srfx_tor_model model, srfx_tor_batch **plan);
+
+/* Advance every reconstruction of the plan by OSEM subset `subset`
+ * (ReconStep + mlem_update, model/recon_step.py:33-49, on contiguous subset
+ * rows, graph/reconstruction/worker.py:79-85) in one persistent launch.
+ * Enqueues kernels only (no allocation, no memset): capturable in a HIP
+ * graph.  Calls on one plan must not overlap. */
+int srfx_tor_batch_subset(srfx_tor_batch *plan, int32_t subset, void *stream);
+
+int srfx_tor_batch_destroy(srfx_tor_batch *plan);
+
 /* ------------------------------------------------------------- Update --- */
 
 /* mlem_update, model/recon_step.py:48-49: x <- x * emap * bp (elementwise,

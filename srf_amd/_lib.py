@@ -83,8 +83,12 @@ def load():
     return lib
 
 
+# bound in srf_amd/batch.py (they take the batch structs)
+_BATCH = ["srfx_tor_batch_create", "srfx_tor_batch_subset", "srfx_tor_batch_destroy"]
+
+
 def exported_symbols():
-    return list(_PROTOS) + ["srfx_strerror"]
+    return list(_PROTOS) + ["srfx_strerror"] + _BATCH
 
 
 def call(name, *args):

@@ -1,0 +1,131 @@
+"""Batched TOF OSEM: independent reconstructions advanced together.
+
+``BatchedOSEM(recons, model, nb_subsets)`` plans a batch of reconstructions
+(each a dict with ``image`` (Image), ``emap`` (Image of the inverse efficiency
+map) and ``data`` (ListModeDataSplit from preprocess.recon_process)) that share
+one SplitLoRsModel.  ``run_subset(k)`` advances every reconstruction by OSEM
+subset k in one persistent kernel launch (srfx_tor_batch_subset);
+``capture()`` records one full iteration (all subsets) into a HIP graph that
+``replay()`` reruns.  ``serial_iterate`` runs the same reconstructions one
+after another through the per-reconstruction OSEM driver, for comparison.
+"""
+import ctypes
+
+import torch
+
+from . import _lib
+from .ops import _check_dev, _stream
+from .recon import OSEM
+
+
+class SrfxRecon(ctypes.Structure):
+    _fields_ = [("shape", ctypes.c_int32 * 3), ("center", ctypes.c_float * 3),
+                ("size", ctypes.c_float * 3), ("image", ctypes.c_void_p),
+                ("emap", ctypes.c_void_p), ("bp", ctypes.c_void_p),
+                ("lors", ctypes.c_void_p * 3), ("n", ctypes.c_int64 * 3),
+                ("ld", ctypes.c_int64 * 3)]
+
+
+def _bind():
+    lib = _lib.load()
+    if not getattr(lib, "_batch_bound", False):
+        lib.srfx_tor_batch_create.argtypes = [ctypes.POINTER(SrfxRecon), ctypes.c_int32,
+                                              ctypes.c_int32, _lib.SrfxTorModel,
+                                              ctypes.POINTER(ctypes.c_void_p)]
+        lib.srfx_tor_batch_create.restype = ctypes.c_int
+        lib.srfx_tor_batch_subset.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p]
+        lib.srfx_tor_batch_subset.restype = ctypes.c_int
+        lib.srfx_tor_batch_destroy.argtypes = [ctypes.c_void_p]
+        lib.srfx_tor_batch_destroy.restype = ctypes.c_int
+        lib._batch_bound = True
+    return lib
+
+
+def _check(lib, name, status):
+    if status != _lib.SRFX_OK:
+        raise _lib.SrfxError(name, status, lib.srfx_strerror(status).decode())
+
+
+class BatchedOSEM:
+    def __init__(self, recons, model, nb_subsets):
+        self.lib = _bind()
+        self.recons = recons
+        self.model = model
+        self.nb_subsets = int(nb_subsets)
+        self.bps = []
+        arr = (SrfxRecon * len(recons))()
+        for r, rc in zip(arr, recons):
+            x, e = rc["image"].data, rc["emap"].data
+            for t, nm in ((x, "image"), (e, "emap")):
+                _check_dev(t, nm)
+                if not t.is_contiguous() or t.dim() != 3:
+                    raise ValueError(f"{nm} must be a contiguous (nx, ny, nz) tensor")
+            if e.shape != x.shape:
+                raise ValueError("emap and image shapes differ")
+            bp = torch.empty_like(x)
+            self.bps.append(bp)
+            for k in range(3):
+                r.shape[k] = x.shape[k]
+                r.center[k] = rc["image"].center[k]
+                r.size[k] = rc["image"].size[k]
+            r.image, r.emap, r.bp = x.data_ptr(), e.data_ptr(), bp.data_ptr()
+            for k, a in enumerate("xyz"):
+                soa = rc["data"][a].soa
+                _check_dev(soa, "lors")
+                if soa.dim() != 2 or soa.shape[0] < 10 or soa.stride(1) != 1:
+                    raise ValueError("group LORs must be [10, N] row-contiguous")
+                r.lors[k] = soa.data_ptr() if soa.shape[1] else None
+                r.n[k] = soa.shape[1]
+                r.ld[k] = max(soa.stride(0), soa.shape[1])
+        self._arr = arr
+        handle = ctypes.c_void_p()
+        _check(self.lib, "srfx_tor_batch_create",
+               self.lib.srfx_tor_batch_create(arr, len(recons), self.nb_subsets, model._model(),
+                                              ctypes.byref(handle)))
+        self.handle = handle
+        self.graph = None
+
+    def run_subset(self, k):
+        _check(self.lib, "srfx_tor_batch_subset",
+               self.lib.srfx_tor_batch_subset(self.handle, int(k), _stream()))
+
+    def iterate(self, nb_iterations=1):
+        for _ in range(nb_iterations):
+            for k in range(self.nb_subsets):
+                self.run_subset(k)
+
+    def capture(self):
+        """Record one iteration (all subsets) into a HIP graph."""
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            for k in range(self.nb_subsets):
+                self.run_subset(k)
+        torch.cuda.synchronize()
+        self.graph = g
+        return g
+
+    def replay(self, nb_iterations=1):
+        for _ in range(nb_iterations):
+            self.graph.replay()
+
+    def close(self):
+        if self.handle:
+            self.lib.srfx_tor_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def serial_iterate(recons, model, nb_subsets, nb_iterations=1, fused=True):
+    """The same reconstructions, one after another, through recon.OSEM."""
+    drivers = [OSEM(model, rc["data"], rc["emap"], nb_subsets=nb_subsets, fused=fused)
+               for rc in recons]
+    for _ in range(nb_iterations):
+        for d, rc in zip(drivers, recons):
+            d.iterate(rc["image"], 1)
+    return drivers

@@ -1,497 +1,6 @@
-// TOF tube-of-response (TOR) projector for CDNA4 (gfx950).
-//
-// Semantics follow the reference CUDA op src/cpp/tf_custom_ops/tor.cu.cc (and
-// tor.cc for the op boundary).  The reference launches one 32x1024 kernel per
-// image slice and re-tests every event against every slice (tor.cu.cc:377-391).
-// Here one lane owns one event for the whole call: it finds the event's
-// strictly crossed slices from its end points, walks only those, and keeps
-// the running projection in a register.
-//
-// Precision.  The tube is narrow (sigma = kernel_width/6, 0.67 mm at
-// kernel_width 4) and the crossing point sits hundreds of mm from the origin,
-// so FP32 rounding of the crossing point alone moves the weights by ~1e-5.
-// Positions are therefore carried in FP64 (full rate on CDNA): slice centre,
-// crossing point, patch origin and the offset of the crossing point from the
-// patch's first voxel centre.  In-patch offsets, squared distances and weights
-// are FP32.  Every include/exclude decision is the FP64 one of the restated
-// reference: the FP32 cut-off test is re-evaluated in FP64 whenever d^2 lies
-// within 1e-4 mm^2 of the cut-off, and a patch origin within 1e-9 voxel of a
-// voxel boundary is recomputed with the reference's exact divisions.
-#include <math.h>
-
-#include "common.h"
-
-namespace srfx {
-namespace {
-
-enum TorMode { kProj = 0, kBack = 1, kMap = 2, kAdj = 3, kCount = 4, kOsem = 5 };
-
-constexpr float kLog2e = 1.4426950408889634f;
-constexpr float kThrMargin = 1.0e-4f; // mm^2 band around the cut-off decided in FP64
-
-struct TorParams {
-    const float *__restrict__ lors;
-    int64_t n, ld;
-    const float *__restrict__ val; // lors_value (bp, map) or g (adjoint)
-    const float *__restrict__ img; // image read by proj / osem
-    float *out;                    // proj[n] (proj) or image (bp, map, adj, osem)
-    int32_t *nnz;                  // count: per-event counts (nullable)
-    unsigned long long *nnz_total; // count: running total (nullable)
-    int gx, gy, gz;
-    int sx, sy, sz; // strides of the read image
-    int ox, oy, oz; // strides of the written image
-    double inter_x, inter_y, inter_z;
-    double rinter_x, rinter_y;
-    double l_bound, b_bound;
-    double zbase; // center_z - (lz - inter_z)/2: slice s is at zbase + s*inter_z
-    double tor_sigma2;
-    float finter_x, finter_y;
-    float rfinter_x, rfinter_y;
-    int patch, half;
-    int full_patch; // evaluate every patch voxel (reference loop) instead of the chords
-    int geo;
-    int use_tof;
-    float tse_inv; // 1 / (tof_sigma2 + tof_bin^2/12)
-    float tofk;    // tof_bin / sqrt(2 pi tof_sigma2)
-};
-
-struct Event {
-    double x1, y1, z1, d0, d1, d2;
-    double rd2;       // 1 / d2
-    double ux_d, uy_d;
-    float ux, uy;
-    float xc, yc, zc;
-    double sc_d;      // sigma_c^2 = tor_sigma2 (* sigma2 factor)
-    float thr;        // FP32 image of 9 sigma_c^2
-    float cw;         // per-voxel exponent factor (log2 domain)
-    float ctof;       // per-slice TOF exponent factor (log2 domain)
-    int s_lo, s_hi;   // strictly crossed slices
-};
-
-__device__ __forceinline__ double slice_z(const TorParams &p, int s)
-{
-    return __dadd_rn(p.zbase, __dmul_rn((double)s, p.inter_z));
-}
-
-// Slices s with 0 < (z_s - z1)/d2 < 1 (tor.cu.cc:29-33), i.e. z_s strictly
-// between z1 and z2.  Returns false when there are none.
-__device__ __forceinline__ bool slice_range(const TorParams &p, Event &ev)
-{
-    double za, zb; // open interval (za, zb)
-    if (ev.d2 > 0.0) {
-        za = ev.z1;
-        zb = ev.z1 + ev.d2;
-    } else if (ev.d2 < 0.0) {
-        za = ev.z1 + ev.d2;
-        zb = ev.z1;
-    } else {
-        return false;
-    }
-    if (!(za > -1.0e30 && zb < 1.0e30))
-        return false;
-    double fa = (za - p.zbase) / p.inter_z, fb = (zb - p.zbase) / p.inter_z;
-    if (fb < 0.0 || fa > (double)p.gz)
-        return false;
-    int a = (int)fmax(floor(fa), -1.0);
-    int b = (int)fmin(ceil(fb), (double)p.gz);
-    // first slice strictly above za, last strictly below zb (exact compares)
-    while (a > 0 && slice_z(p, a - 1) > za)
-        a--;
-    while (a <= b && !(slice_z(p, a) > za))
-        a++;
-    while (b < p.gz - 1 && slice_z(p, b + 1) < zb)
-        b++;
-    while (b >= a && !(slice_z(p, b) < zb))
-        b--;
-    if (ev.d2 > 0.0) {
-        // ratio < 1 is tested on the rounded quotient: drop a slice whose
-        // quotient rounds to 1.
-        while (b >= a && !(((slice_z(p, b) - ev.z1) / ev.d2) < 1.0))
-            b--;
-    } else {
-        while (b >= a && !(((slice_z(p, a) - ev.z1) / ev.d2) < 1.0))
-            a++;
-    }
-    ev.s_lo = a < 0 ? 0 : a;
-    ev.s_hi = b > p.gz - 1 ? p.gz - 1 : b;
-    return ev.s_lo <= ev.s_hi;
-}
-
-template <int MODE>
-__device__ __forceinline__ bool load_event(const TorParams &p, int64_t e, Event &ev)
-{
-    const float *L = p.lors;
-    const int64_t ld = p.ld;
-    float x1 = __builtin_nontemporal_load(L + 0 * ld + e);
-    float y1 = __builtin_nontemporal_load(L + 1 * ld + e);
-    float z1 = __builtin_nontemporal_load(L + 2 * ld + e);
-    float x2 = __builtin_nontemporal_load(L + 3 * ld + e);
-    float y2 = __builtin_nontemporal_load(L + 4 * ld + e);
-    float z2 = __builtin_nontemporal_load(L + 5 * ld + e);
-    float f;
-    if (MODE == kMap) {
-        f = __builtin_nontemporal_load(L + 6 * ld + e);
-        ev.xc = ev.yc = ev.zc = 0.0f;
-    } else {
-        ev.xc = __builtin_nontemporal_load(L + 6 * ld + e);
-        ev.yc = __builtin_nontemporal_load(L + 7 * ld + e);
-        ev.zc = __builtin_nontemporal_load(L + 8 * ld + e);
-        f = __builtin_nontemporal_load(L + 9 * ld + e);
-    }
-    ev.x1 = x1;
-    ev.y1 = y1;
-    ev.z1 = z1;
-    ev.d0 = (double)x2 - (double)x1;
-    ev.d1 = (double)y2 - (double)y1;
-    ev.d2 = (double)z2 - (double)z1;
-    double dis = __dsqrt_rn(__dadd_rn(__dadd_rn(__dmul_rn(ev.d0, ev.d0), __dmul_rn(ev.d1, ev.d1)),
-                                      __dmul_rn(ev.d2, ev.d2)));
-    ev.ux_d = __ddiv_rn(ev.d0, dis);
-    ev.uy_d = __ddiv_rn(ev.d1, dis);
-    ev.ux = (float)ev.ux_d;
-    ev.uy = (float)ev.uy_d;
-    ev.rd2 = 1.0 / ev.d2;
-    ev.sc_d = p.geo ? p.tor_sigma2 * (double)f : p.tor_sigma2;
-    ev.thr = (float)(9.0 * ev.sc_d);
-    float inv_sc = (float)(1.0 / ev.sc_d);
-    if (MODE != kMap && MODE != kCount && p.use_tof) {
-        ev.cw = kLog2e * 0.5f * (p.tse_inv - inv_sc);
-        ev.ctof = -0.5f * kLog2e * p.tse_inv;
-    } else {
-        ev.cw = -0.5f * kLog2e * inv_sc;
-        ev.ctof = 0.0f;
-    }
-    if (!(ev.sc_d > 0.0) || !(dis > 0.0) || !(dis < 1.0e30))
-        return false;
-    return slice_range(p, ev);
-}
-
-// Per-event geometry carried across slices.  The crossing point of slice s,
-// in voxel units from the grid corner, is affine in s: u(s) = ax + s*bx,
-// w(s) = ay + s*by (FP64).  The vector from the crossing point to the TOF
-// centre is v0 - (s - s_lo)*dv (FP32).
-struct Track {
-    double ax, bx, ay, by;
-    float v0x, v0y, v0z, dvx, dvy, dvz;
-    // in-slice ellipse d2 < tm: d2 = a (dx - cxy dy)^2 + q dy^2
-    float A;     // 1 / (1 - ux^2)
-    float cxy;   // ux uy / (1 - ux^2)
-    float q;     // uz^2 / (1 - ux^2)
-    float tm;    // cut-off padded so the candidate set contains every kept voxel
-    float dymax; // sqrt(tm / q)
-};
-
-__device__ __forceinline__ void make_track(const TorParams &p, const Event &ev, Track &t)
-{
-    double r0 = (p.zbase - ev.z1) * ev.rd2;        // ratio at slice 0
-    double rs = p.inter_z * ev.rd2;                // ratio step per slice
-    t.ax = (ev.x1 + ev.d0 * r0 - p.l_bound) * p.rinter_x;
-    t.bx = ev.d0 * rs * p.rinter_x;
-    t.ay = (ev.y1 + ev.d1 * r0 - p.b_bound) * p.rinter_y;
-    t.by = ev.d1 * rs * p.rinter_y;
-    double rl = r0 + (double)ev.s_lo * rs;
-    t.v0x = (float)((double)ev.xc - (ev.x1 + ev.d0 * rl));
-    t.v0y = (float)((double)ev.yc - (ev.y1 + ev.d1 * rl));
-    t.v0z = (float)((double)ev.zc - slice_z(p, ev.s_lo));
-    t.dvx = (float)(ev.d0 * rs);
-    t.dvy = (float)(ev.d1 * rs);
-    t.dvz = (float)p.inter_z;
-    double ux2 = ev.ux_d * ev.ux_d, uy2 = ev.uy_d * ev.uy_d;
-    double a = 1.0 - ux2;
-    double dis2 = ev.d0 * ev.d0 + ev.d1 * ev.d1 + ev.d2 * ev.d2;
-    double uz2 = ev.d2 * ev.d2 / dis2;
-    t.A = (float)(1.0 / a);
-    t.cxy = (float)(ev.ux_d * ev.uy_d / a);
-    t.q = (float)(uz2 / a);
-    t.tm = ev.thr * (1.0f + 1.0e-5f) + 2.0f * kThrMargin;
-    t.dymax = sqrtf(t.tm / t.q);
-    (void)uy2;
-}
-
-struct Cross {
-    int ix0, iy0;
-    float dx0, dy0; // crossing point minus centre of the first patch voxel, mm
-};
-
-__device__ __forceinline__ bool near_int(double u)
-{
-    double fr = u - floor(u);
-    return fr < 1.0e-9 || fr > 1.0 - 1.0e-9;
-}
-
-// The reference's crossing point of slice s (tor.cu.cc:30-36) in FP64 with
-// its operation order.
-__device__ __forceinline__ void cross_point_exact(const TorParams &p, const Event &ev, int s,
-                                                  double &cx, double &cy)
-{
-    double ratio = __ddiv_rn(__dsub_rn(slice_z(p, s), ev.z1), ev.d2);
-    cx = __dadd_rn(ev.x1, __dmul_rn(ev.d0, ratio));
-    cy = __dadd_rn(ev.y1, __dmul_rn(ev.d1, ratio));
-}
-
-__device__ __forceinline__ double mesh_x(const TorParams &p, int ix)
-{
-    return __dadd_rn(__dmul_rn(p.inter_x, (double)ix + 0.5), p.l_bound);
-}
-__device__ __forceinline__ double mesh_y(const TorParams &p, int iy)
-{
-    return __dadd_rn(__dmul_rn(p.inter_y, (double)iy + 0.5), p.b_bound);
-}
-
-// Patch origin (tor.cu.cc:135-136) and offsets for slice s.  Near a voxel
-// boundary the origin is recomputed with the reference's exact divisions.
-__device__ __forceinline__ void cross_at(const TorParams &p, const Event &ev, const Track &t, int s,
-                                         Cross &c)
-{
-    double u = t.ax + (double)s * t.bx;
-    double w = t.ay + (double)s * t.by;
-    if (near_int(u) || near_int(w)) {
-        double cx, cy;
-        cross_point_exact(p, ev, s, cx, cy);
-        c.ix0 = (int)__ddiv_rn(__dsub_rn(cx, p.l_bound), p.inter_x) - p.half;
-        c.iy0 = (int)__ddiv_rn(__dsub_rn(cy, p.b_bound), p.inter_y) - p.half;
-        c.dx0 = (float)(cx - mesh_x(p, c.ix0));
-        c.dy0 = (float)(cy - mesh_y(p, c.iy0));
-    } else {
-        c.ix0 = (int)u - p.half;
-        c.iy0 = (int)w - p.half;
-        c.dx0 = (float)((u - ((double)c.ix0 + 0.5)) * p.inter_x);
-        c.dy0 = (float)((w - ((double)c.iy0 + 0.5)) * p.inter_y);
-    }
-}
-
-// Exact FP64 cut-off decision of the reference (tor.cu.cc:57-71) for voxel
-// (ix, iy) of slice s.
-__device__ __forceinline__ bool inside_exact(const TorParams &p, const Event &ev, int s, int ix,
-                                             int iy)
-{
-    double cx, cy;
-    cross_point_exact(p, ev, s, cx, cy);
-    double dx = __dsub_rn(cx, mesh_x(p, ix)), dy = __dsub_rn(cy, mesh_y(p, iy));
-    double rc = __dadd_rn(__dmul_rn(dx, ev.ux_d), __dmul_rn(dy, ev.uy_d));
-    double d2 = __dsub_rn(__dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy)), __dmul_rn(rc, rc));
-    return d2 < 9.0 * ev.sc_d;
-}
-
-// Per-slice TOF factor tof_bin/sqrt(2 pi s2) * exp(-D/(2 tse)), D the squared
-// distance from the TOF centre to the crossing point; the reference's
-// d2_tof = D - d2 (tor.cu.cc:87) has its -d2 part folded into the per-voxel
-// exponent.
-__device__ __forceinline__ float tof_factor(const TorParams &p, const Event &ev, const Track &t,
-                                            int s)
-{
-    float k = (float)(s - ev.s_lo);
-    float vx = t.v0x - k * t.dvx, vy = t.v0y - k * t.dvy, vz = t.v0z - k * t.dvz;
-    float D = vx * vx + vy * vy + vz * vz;
-    return p.tofk * __builtin_amdgcn_exp2f(D * ev.ctof);
-}
-
-// One voxel: exact include test, then gather / scatter / count.
-template <int KIND>
-__device__ __forceinline__ void visit(const TorParams &p, const Event &ev, int s, int ix, int iy,
-                                      float dx, float dy, float amp, float scale, float &acc)
-{
-    float rc = dx * ev.ux + dy * ev.uy;
-    float d2 = dx * dx + dy * dy - rc * rc;
-    bool in = d2 < ev.thr - kThrMargin;
-    if (!in && d2 < ev.thr + kThrMargin)
-        in = inside_exact(p, ev, s, ix, iy);
-    if (in) {
-        if (KIND == 2) {
-            acc += 1.0f;
-        } else {
-            float w = amp * __builtin_amdgcn_exp2f(d2 * ev.cw);
-            if (KIND == 0)
-                acc += w * p.img[ix * p.sx + iy * p.sy + s * p.sz];
-            else
-                unsafeAtomicAdd(p.out + (ix * p.ox + iy * p.oy + s * p.oz), w * scale);
-        }
-    }
-}
-
-// Walk one slice's patch (tor.cu.cc:124-166 and siblings), j outer, i inner,
-// voxels outside the grid skipped.  FULL evaluates all P x P voxels as the
-// reference loop does; otherwise each row only evaluates the voxels whose
-// centres fall in the analytically computed chord of the (padded) tube
-// ellipse, a superset of the kept voxels, and the exact test decides.
-//   KIND 0: returns sum w*img; 1: out += w*scale; 2: returns the count.
-template <int KIND, bool FULL>
-__device__ __forceinline__ float patch_walk(const TorParams &p, const Event &ev, const Track &t,
-                                            const Cross &c, float amp, int s, float scale)
-{
-    const int P = p.patch;
-    // A patch entirely outside the grid contributes nothing; this also keeps
-    // the loop bounds below free of overflow for saturated origins.
-    if (c.ix0 <= -P || c.ix0 >= p.gx || c.iy0 <= -P || c.iy0 >= p.gy)
-        return 0.0f;
-    int i_lo = c.ix0 < 0 ? -c.ix0 : 0;
-    int i_hi = min(P, p.gx - c.ix0) - 1;
-    int j_lo = c.iy0 < 0 ? -c.iy0 : 0;
-    int j_hi = min(P, p.gy - c.iy0) - 1;
-    float acc = 0.0f;
-    if (!FULL) {
-        const float eps = 1.0e-4f;
-        j_lo = max(j_lo, (int)ceilf((c.dy0 - t.dymax) * p.rfinter_y - eps));
-        j_hi = min(j_hi, (int)floorf((c.dy0 + t.dymax) * p.rfinter_y + eps));
-    }
-    for (int j = j_lo; j <= j_hi; j++) {
-        int iy = c.iy0 + j;
-        float dy = c.dy0 - (float)j * p.finter_y;
-        int il = i_lo, ih = i_hi;
-        if (!FULL) {
-            float tt = t.tm - dy * dy * t.q;
-            if (tt <= 0.0f)
-                continue;
-            float hw = sqrtf(tt * t.A);
-            float ctr = c.dx0 - t.cxy * dy;
-            const float eps = 1.0e-4f;
-            il = max(il, (int)ceilf((ctr - hw) * p.rfinter_x - eps));
-            ih = min(ih, (int)floorf((ctr + hw) * p.rfinter_x + eps));
-        }
-        for (int i = il; i <= ih; i++) {
-            float dx = c.dx0 - (float)i * p.finter_x;
-            visit<KIND>(p, ev, s, c.ix0 + i, iy, dx, dy, amp, scale, acc);
-        }
-    }
-    return acc;
-}
-
-template <int KIND>
-__device__ __forceinline__ float walk_event(const TorParams &p, const Event &ev, bool tof,
-                                            float scale)
-{
-    Track t;
-    make_track(p, ev, t);
-    float acc = 0.0f;
-    for (int s = ev.s_lo; s <= ev.s_hi; s++) {
-        Cross c;
-        cross_at(p, ev, t, s, c);
-        float amp = tof ? tof_factor(p, ev, t, s) : 1.0f;
-        if (p.full_patch)
-            acc += patch_walk<KIND, true>(p, ev, t, c, amp, s, scale);
-        else
-            acc += patch_walk<KIND, false>(p, ev, t, c, amp, s, scale);
-    }
-    return acc;
-}
-
-template <int MODE>
-__global__ __launch_bounds__(256) void tor_kernel(TorParams p)
-{
-    int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    int cnt = 0;
-    if (e < p.n) {
-        Event ev;
-        bool any = load_event<MODE>(p, e, ev);
-        const bool tof = (MODE != kMap && MODE != kCount) && p.use_tof;
-        if (MODE == kProj) {
-            p.out[e] = any ? walk_event<0>(p, ev, tof, 0.0f) : 0.0f;
-        } else if (MODE == kBack) {
-            float v = p.val[e];
-            if (any && v > 1e-7)
-                walk_event<1>(p, ev, tof, 1.0f / v);
-        } else if (MODE == kMap) {
-            if (any)
-                walk_event<1>(p, ev, false, 1.0f / p.val[e]);
-        } else if (MODE == kAdj) {
-            float g = p.val[e];
-            if (any && g != 0.0f)
-                walk_event<1>(p, ev, tof, g);
-        } else if (MODE == kOsem) {
-            if (any) {
-                float pv = walk_event<0>(p, ev, tof, 0.0f);
-                if (pv > 1e-7)
-                    walk_event<1>(p, ev, tof, 1.0f / pv);
-            }
-        } else { // kCount
-            if (any)
-                cnt = (int)walk_event<2>(p, ev, false, 0.0f);
-            if (p.nnz)
-                p.nnz[e] = cnt;
-        }
-    }
-    if (MODE == kCount && p.nnz_total) {
-        // Wave reduction, then one 64-bit atomic per wave.
-        unsigned long long c = (unsigned long long)cnt;
-        for (int off = 32; off > 0; off >>= 1)
-            c += __shfl_down(c, off, 64);
-        if ((threadIdx.x & 63) == 0 && c)
-            atomicAdd(p.nnz_total, c);
-    }
-}
-
-// Host part of the reference setup (tor.cu.cc:364-375), in double.
-int setup(TorParams &p, const float *lors, int64_t n, int64_t ld, const srfx_grid &g,
-          float kernel_width, int geo, float tof_bin, float tof_sigma2)
-{
-    if (n < 0 || (n > 0 && (!lors || ld < n)) || !grid_ok(g))
-        return SRFX_EINVAL;
-    if (!(kernel_width > 0.0f) || !(kernel_width < 1.0e6f))
-        return SRFX_EINVAL;
-    p = TorParams{};
-    p.lors = lors;
-    p.n = n;
-    p.ld = ld;
-    p.gx = g.shape[0];
-    p.gy = g.shape[1];
-    p.gz = g.shape[2];
-    double lx = g.size[0], ly = g.size[1], lz = g.size[2];
-    p.inter_x = lx / (double)p.gx;
-    p.inter_y = ly / (double)p.gy;
-    p.inter_z = lz / (double)p.gz;
-    p.rinter_x = 1.0 / p.inter_x;
-    p.rinter_y = 1.0 / p.inter_y;
-    p.finter_x = (float)p.inter_x;
-    p.finter_y = (float)p.inter_y;
-    p.rfinter_x = (float)(1.0 / p.inter_x);
-    p.rfinter_y = (float)(1.0 / p.inter_y);
-    p.l_bound = (double)g.center[0] - lx / 2;
-    p.b_bound = (double)g.center[1] - ly / 2;
-    p.zbase = (double)g.center[2] - (lz - p.inter_z) / 2;
-    double kw = kernel_width;
-    p.tor_sigma2 = kw * kw / 36;
-    double ps = std::ceil((std::sqrt(2.0) * kw + p.inter_z) / p.inter_x);
-    if (!(ps >= 1.0) || ps > kMaxPatch)
-        return SRFX_EUNSUPPORTED;
-    p.patch = (int)ps;
-    p.half = (int)((unsigned)p.patch / 2);
-    p.geo = geo ? 1 : 0;
-    p.use_tof = tof_sigma2 < kTofThreshold ? 1 : 0;
-    if (p.use_tof) {
-        double tse = (double)tof_sigma2 + (double)tof_bin * (double)tof_bin / 12;
-        if (!(tse > 0.0) || !(tof_sigma2 > 0.0f))
-            return SRFX_EINVAL;
-        p.tse_inv = (float)(1.0 / tse);
-        p.tofk = (float)((double)tof_bin / std::sqrt(2.0 * M_PI * (double)tof_sigma2));
-    }
-    Strides s = grid_strides(g);
-    p.sx = p.ox = (int)s.x;
-    p.sy = p.oy = (int)s.y;
-    p.sz = p.oz = (int)s.z;
-    return SRFX_OK;
-}
-
-template <int MODE>
-int launch(const TorParams &p, hipStream_t st)
-{
-    if (p.n == 0)
-        return SRFX_OK;
-    hipLaunchKernelGGL(tor_kernel<MODE>, dim3(blocks_for(p.n, 256)), dim3(256), 0, st, p);
-    return launch_status();
-}
-
-srfx_tor_model model_no_tof(float kw, int geo)
-{
-    srfx_tor_model m;
-    m.kernel_width = kw;
-    m.geo_sigma2_flag = geo;
-    m.tof_bin = 0.0f;
-    m.tof_sigma2 = 1.0e6f;
-    return m;
-}
-
-} // namespace
-} // namespace srfx
+// TOF tube-of-response (TOR) projector entry points (C ABI, include/srfx.h).
+// The device code is in tor_core.h.
+#include "tor_core.h"
 
 using namespace srfx;
 
