@@ -54,6 +54,18 @@ enum srfx_status {
  * structure) instead of only the voxels on the tube's chords.  Same results;
  * slower; kept for verification. */
 #define SRFX_FULL_PATCH 2u
+/* TOR ops: launch schedule.  Lanes that share one event (its crossed slices
+ * interleaved between them): SRFX_LPE_1 (default), SRFX_LPE_4, SRFX_LPE_16.
+ * Threads per block: SRFX_BLOCK_256 (default), SRFX_BLOCK_64, SRFX_BLOCK_128.
+ * Results agree to FP32 rounding across schedules. */
+#define SRFX_LPE_SHIFT 4
+#define SRFX_LPE_1 (1u << SRFX_LPE_SHIFT)
+#define SRFX_LPE_4 (2u << SRFX_LPE_SHIFT)
+#define SRFX_LPE_16 (3u << SRFX_LPE_SHIFT)
+#define SRFX_BLOCK_SHIFT 6
+#define SRFX_BLOCK_64 (1u << SRFX_BLOCK_SHIFT)
+#define SRFX_BLOCK_128 (2u << SRFX_BLOCK_SHIFT)
+#define SRFX_BLOCK_256 (3u << SRFX_BLOCK_SHIFT)
 
 typedef struct srfx_grid {
     int32_t shape[3];  /* gx, gy, gz: voxels along the op's x, y, z */

@@ -16,6 +16,21 @@ SRFX_ELAUNCH = 2
 SRFX_EUNSUPPORTED = 3
 SRFX_ACCUMULATE = 1
 SRFX_FULL_PATCH = 2
+SRFX_LPE_SHIFT = 4
+SRFX_BLOCK_SHIFT = 6
+_LPE_CODE = {1: 1, 4: 2, 16: 3}
+_BLOCK_CODE = {64: 1, 128: 2, 256: 3}
+
+
+def schedule_flags(schedule):
+    """Flags for a TOR launch schedule (lanes_per_event, block) or None."""
+    if schedule is None:
+        return 0
+    lpe, block = schedule
+    if lpe not in _LPE_CODE or block not in _BLOCK_CODE:
+        raise ValueError(f"unsupported schedule {schedule}: lanes per event in (1, 4, 16), "
+                         "block in (64, 128, 256)")
+    return (_LPE_CODE[lpe] << SRFX_LPE_SHIFT) | (_BLOCK_CODE[block] << SRFX_BLOCK_SHIFT)
 
 
 class SrfxGrid(ctypes.Structure):

@@ -18,7 +18,7 @@ extern "C" int srfx_tor_project(const float *lors, int64_t n, int64_t ld, const 
     p.full_patch = (flags & SRFX_FULL_PATCH) ? 1 : 0;
     p.img = image;
     p.out = proj;
-    return launch<kProj>(p, (hipStream_t)stream);
+    return launch<kProj>(p, flags, (hipStream_t)stream);
 }
 
 static int tor_scatter_common(int mode, const float *lors, int64_t n, int64_t ld,
@@ -41,11 +41,11 @@ static int tor_scatter_common(int mode, const float *lors, int64_t n, int64_t ld
     p.out = image;
     switch (mode) {
     case kBack:
-        return launch<kBack>(p, st);
+        return launch<kBack>(p, flags, st);
     case kAdj:
-        return launch<kAdj>(p, st);
+        return launch<kAdj>(p, flags, st);
     default:
-        return launch<kMap>(p, st);
+        return launch<kMap>(p, flags, st);
     }
 }
 
@@ -93,7 +93,7 @@ extern "C" int srfx_tor_osem_subset(const float *lors, int64_t n, int64_t ld, co
     p.full_patch = (flags & SRFX_FULL_PATCH) ? 1 : 0;
     p.img = image;
     p.out = bp;
-    return launch<kOsem>(p, st);
+    return launch<kOsem>(p, flags, st);
 }
 
 extern "C" int srfx_tor_count_nnz(const float *lors, int64_t n, int64_t ld, srfx_grid grid,
@@ -107,5 +107,5 @@ extern "C" int srfx_tor_count_nnz(const float *lors, int64_t n, int64_t ld, srfx
     p.nnz = nnz_per_event;
     p.nnz_total = (unsigned long long *)nnz_total;
     p.full_patch = 0;
-    return launch<kCount>(p, (hipStream_t)stream);
+    return launch<kCount>(p, 0u, (hipStream_t)stream);
 }

@@ -431,16 +431,16 @@ __device__ __forceinline__ float slice_gather(const TorParams &p, const Event &e
     return acc;
 }
 
-// Walk all crossed slices of an event.  PG > 0: projection with the batched
-// gather of a PG x PG patch.
+// Walk crossed slices s_lo + r0, s_lo + r0 + rstep, ... of an event.  PG > 0:
+// projection with the batched gather of a PG x PG patch.
 template <int KIND, int PG>
 __device__ __forceinline__ float walk_event(const TorParams &p, const Event &ev, bool tof,
-                                            float scale)
+                                            float scale, int r0, int rstep)
 {
     Track t;
     make_track(p, ev, t);
     float acc = 0.0f;
-    for (int s = ev.s_lo; s <= ev.s_hi; s++) {
+    for (int s = ev.s_lo + r0; s <= ev.s_hi; s += rstep) {
         Cross c;
         cross_at(p, ev, t, s, c);
         float amp = tof ? tof_factor(p, ev, t, s) : 1.0f;
@@ -454,51 +454,69 @@ __device__ __forceinline__ float walk_event(const TorParams &p, const Event &ev,
     return acc;
 }
 
-// One event of mode MODE (the body of the reference's per-slice kernels,
-// tor.cu.cc:244-346, for all of the event's slices).  Returns the non-zero
-// count in kCount mode.
-template <int MODE, int PG>
-__device__ __forceinline__ int run_event(const TorParams &p, int64_t e)
+// Sum over the LPE consecutive lanes that share an event.  Every lane of the
+// wave must reach this call.
+template <int LPE>
+__device__ __forceinline__ float group_sum(float v)
 {
-    int cnt = 0;
-    Event ev;
-    bool any = load_event<MODE>(p, e, ev);
-    const bool tof = (MODE != kMap && MODE != kCount) && p.use_tof;
-    if (MODE == kProj) {
-        p.out[e] = any ? walk_event<0, PG>(p, ev, tof, 0.0f) : 0.0f;
-    } else if (MODE == kBack) {
-        float v = p.val[e];
-        if (any && v > 1e-7)
-            walk_event<1, 0>(p, ev, tof, 1.0f / v);
-    } else if (MODE == kMap) {
-        if (any)
-            walk_event<1, 0>(p, ev, false, 1.0f / p.val[e]);
-    } else if (MODE == kAdj) {
-        float g = p.val[e];
-        if (any && g != 0.0f)
-            walk_event<1, 0>(p, ev, tof, g);
-    } else if (MODE == kOsem) {
-        if (any) {
-            float pv = walk_event<0, PG>(p, ev, tof, 0.0f);
-            if (pv > 1e-7)
-                walk_event<1, 0>(p, ev, tof, 1.0f / pv);
-        }
-    } else { // kCount
-        if (any)
-            cnt = (int)walk_event<2, 0>(p, ev, false, 0.0f);
-        if (p.nnz)
-            p.nnz[e] = cnt;
-    }
-    return cnt;
+#pragma unroll
+    for (int m = 1; m < LPE; m <<= 1)
+        v += __shfl_xor(v, m, 64);
+    return v;
 }
 
-template <int MODE, int PG>
+// One event of mode MODE (the body of the reference's per-slice kernels,
+// tor.cu.cc:244-346, for all of the event's slices), shared by LPE lanes:
+// lane `sub` takes every LPE-th crossed slice and partial sums are combined
+// across the lanes.  Every lane of the wave calls this; lanes with
+// valid == false contribute nothing.  Returns the event's non-zero count on
+// its first lane in kCount mode.
+template <int MODE, int PG, int LPE>
+__device__ __forceinline__ int run_event(const TorParams &p, int64_t e, int sub, bool valid)
+{
+    Event ev;
+    bool any = valid ? load_event<MODE>(p, e, ev) : false;
+    const bool tof = (MODE != kMap && MODE != kCount) && p.use_tof;
+    if (MODE == kProj) {
+        float part = any ? walk_event<0, PG>(p, ev, tof, 0.0f, sub, LPE) : 0.0f;
+        float pv = group_sum<LPE>(part);
+        if (valid && sub == 0)
+            p.out[e] = pv;
+    } else if (MODE == kBack) {
+        float v = valid ? p.val[e] : 0.0f;
+        if (any && v > 1e-7)
+            walk_event<1, 0>(p, ev, tof, 1.0f / v, sub, LPE);
+    } else if (MODE == kMap) {
+        if (any)
+            walk_event<1, 0>(p, ev, false, 1.0f / p.val[e], sub, LPE);
+    } else if (MODE == kAdj) {
+        float g = valid ? p.val[e] : 0.0f;
+        if (any && g != 0.0f)
+            walk_event<1, 0>(p, ev, tof, g, sub, LPE);
+    } else if (MODE == kOsem) {
+        float part = any ? walk_event<0, PG>(p, ev, tof, 0.0f, sub, LPE) : 0.0f;
+        float pv = group_sum<LPE>(part);
+        if (any && pv > 1e-7)
+            walk_event<1, 0>(p, ev, tof, 1.0f / pv, sub, LPE);
+    } else { // kCount
+        float part = any ? walk_event<2, 0>(p, ev, false, 0.0f, sub, LPE) : 0.0f;
+        int cnt = (int)group_sum<LPE>(part);
+        if (valid && sub == 0) {
+            if (p.nnz)
+                p.nnz[e] = cnt;
+            return cnt;
+        }
+    }
+    return 0;
+}
+
+template <int MODE, int PG, int LPE>
 __global__ __launch_bounds__(256) void tor_kernel(TorParams p)
 {
-    int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    int cnt = 0;
-    if (e < p.n)
-        cnt = run_event<MODE, PG>(p, e);
+    int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t e = gid / LPE;
+    int sub = (int)(gid % LPE);
+    int cnt = run_event<MODE, PG, LPE>(p, e, sub, e < p.n);
     if (MODE == kCount && p.nnz_total) {
         // Wave reduction, then one 64-bit atomic per wave.
         unsigned long long c = (unsigned long long)cnt;
@@ -560,38 +578,55 @@ int setup(TorParams &p, const float *lors, int64_t n, int64_t ld, const srfx_gri
     return SRFX_OK;
 }
 
-template <int MODE, int PG>
-int launch_pg(const TorParams &p, hipStream_t st)
+template <int MODE, int PG, int LPE>
+int launch_pg(const TorParams &p, int block, hipStream_t st)
 {
-    hipLaunchKernelGGL((tor_kernel<MODE, PG>), dim3(blocks_for(p.n, 256)), dim3(256), 0, st, p);
+    hipLaunchKernelGGL((tor_kernel<MODE, PG, LPE>), dim3(blocks_for(p.n * LPE, block)),
+                       dim3(block), 0, st, p);
     return launch_status();
 }
 
-// Projection-type modes get a kernel specialised for the patch size (batched
-// gathers); the full-patch verification path and other sizes use the
-// generic loops.
+template <int MODE, int LPE>
+int launch_lpe(const TorParams &p, int block, hipStream_t st)
+{
+    if constexpr (MODE == kProj || MODE == kOsem) {
+        if (!p.full_patch) {
+            switch (p.patch) {
+            case 3:
+                return launch_pg<MODE, 3, LPE>(p, block, st);
+            case 4:
+                return launch_pg<MODE, 4, LPE>(p, block, st);
+            case 5:
+                return launch_pg<MODE, 5, LPE>(p, block, st);
+            case 6:
+                return launch_pg<MODE, 6, LPE>(p, block, st);
+            default:
+                break;
+            }
+        }
+    }
+    return launch_pg<MODE, 0, LPE>(p, block, st);
+}
+
+// Launch schedule from the flags: lanes per event (1, 4, 16) and block size
+// (64, 128, 256).  Projection-type modes get a kernel specialised for the
+// patch size (batched gathers); the full-patch verification path and other
+// sizes use the generic loops.
 template <int MODE>
-int launch(const TorParams &p, hipStream_t st)
+int launch(const TorParams &p, uint32_t flags, hipStream_t st)
 {
     if (p.n == 0)
         return SRFX_OK;
-    if constexpr (MODE == kProj || MODE == kOsem) {
-      if (!p.full_patch) {
-        switch (p.patch) {
-        case 3:
-            return launch_pg<MODE, 3>(p, st);
-        case 4:
-            return launch_pg<MODE, 4>(p, st);
-        case 5:
-            return launch_pg<MODE, 5>(p, st);
-        case 6:
-            return launch_pg<MODE, 6>(p, st);
-        default:
-            break;
-        }
-      }
+    static const int kBlock[4] = {256, 64, 128, 256};
+    const int block = kBlock[(flags >> SRFX_BLOCK_SHIFT) & 3u];
+    switch ((flags >> SRFX_LPE_SHIFT) & 3u) {
+    case 2:
+        return launch_lpe<MODE, 4>(p, block, st);
+    case 3:
+        return launch_lpe<MODE, 16>(p, block, st);
+    default:
+        return launch_lpe<MODE, 1>(p, block, st);
     }
-    return launch_pg<MODE, 0>(p, st);
 }
 
 srfx_tor_model model_no_tof(float kw, int geo)

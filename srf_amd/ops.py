@@ -96,21 +96,23 @@ def _ptr(t):
 # ------------------------------------------------------------------- TOR ---
 
 def projection(lors, image, grid, center, size, kernel_width, geo_sigma2_flag=False,
-               tof_bin=1.0e4, tof_sigma2=1.0e4, full_patch=False):
+               tof_bin=1.0e4, tof_sigma2=1.0e4, full_patch=False, schedule=None):
     """Op ``Projection`` of tor.so (tor.cc:14-28, 106-181).  full_patch walks
-    every voxel of the reference's patch (verification path)."""
+    every voxel of the reference's patch (verification path); schedule is
+    (lanes_per_event, block) or None for the default."""
     lors, n, ld = _soa(lors, 10)
     g, cg = _geom(grid, center, size)
     image = _image_like(image, g)
     out = torch.empty(n, dtype=torch.float32, device=lors.device)
     m = _lib.make_tor_model(kernel_width, geo_sigma2_flag, tof_bin, tof_sigma2)
     _lib.call("srfx_tor_project", _ptr(lors), n, ld, _ptr(image), cg, m, _ptr(out),
-              _lib.SRFX_FULL_PATCH if full_patch else 0, _stream())
+              (_lib.SRFX_FULL_PATCH if full_patch else 0) | _lib.schedule_flags(schedule),
+              _stream())
     return out
 
 
 def backprojection(image, lors, lors_value, grid, center, size, tof_bin=1.0e4, tof_sigma2=1.0e4,
-                   kernel_width=None, geo_sigma2_flag=False, full_patch=False):
+                   kernel_width=None, geo_sigma2_flag=False, full_patch=False, schedule=None):
     """Op ``Backprojection`` of tor.so (tor.cc:30-47, 203-271): returns an
     image shaped like ``image`` holding sum_i w_ij / v_i over v_i > 1e-7."""
     if kernel_width is None:
@@ -122,7 +124,8 @@ def backprojection(image, lors, lors_value, grid, center, size, tof_bin=1.0e4, t
     out = torch.empty_like(image, memory_format=torch.contiguous_format)
     m = _lib.make_tor_model(kernel_width, geo_sigma2_flag, tof_bin, tof_sigma2)
     _lib.call("srfx_tor_backproject", _ptr(lors), n, ld, _ptr(v), cg, m, _ptr(out),
-              _lib.SRFX_FULL_PATCH if full_patch else 0, _stream())
+              (_lib.SRFX_FULL_PATCH if full_patch else 0) | _lib.schedule_flags(schedule),
+              _stream())
     return out
 
 
@@ -156,7 +159,7 @@ def maplors(image, grid, center, size, lors, lors_value, kernel_width, geo_sigma
 
 
 def osem_subset(lors, image, grid, center, size, kernel_width, geo_sigma2_flag=False,
-                tof_bin=1.0e4, tof_sigma2=1.0e4, out=None, accumulate=False):
+                tof_bin=1.0e4, tof_sigma2=1.0e4, out=None, accumulate=False, schedule=None):
     """Fused projection + ratio backprojection of one group (no reference op:
     equals backprojection(lors_value=projection(...)))."""
     lors, n, ld = _soa(lors, 10)
@@ -167,7 +170,8 @@ def osem_subset(lors, image, grid, center, size, kernel_width, geo_sigma2_flag=F
         accumulate = False
     m = _lib.make_tor_model(kernel_width, geo_sigma2_flag, tof_bin, tof_sigma2)
     _lib.call("srfx_tor_osem_subset", _ptr(lors), n, ld, _ptr(image), cg, cg, m, _ptr(out),
-              _lib.SRFX_ACCUMULATE if accumulate else 0, _stream())
+              (_lib.SRFX_ACCUMULATE if accumulate else 0) | _lib.schedule_flags(schedule),
+              _stream())
     return out
 
 

@@ -51,6 +51,38 @@ class SplitLoRsModel:
         self.geo_sigma2_flag = bool(geo_sigma2_flag)
         self.tof_sigma2 = 1.0e4 if tof_sigma2 is None else float(tof_sigma2)
         self.tof_bin = 1.0e4 if tof_bin is None else float(tof_bin)
+        # launch schedule per op: (lanes_per_event, block) or None (default)
+        self.schedule = {"projection": None, "backprojection": None, "osem_subset": None}
+
+    def _flags(self, op):
+        return _lib.schedule_flags(self.schedule.get(op))
+
+    def autotune(self, image, data, sample=262144, reps=3, ops_=("osem_subset",)):
+        """Time every (lanes_per_event, block) schedule of each op in ``ops_``
+        on the first ``sample`` events of every group of ``data`` and keep the
+        fastest.  Returns {op: {schedule: ms}}."""
+        from .autotune import SCHEDULES, _time
+        sub = ListModeDataSplit(*[data[a].replace_values(
+            None if data[a].values is None else data[a].values[:sample]) for a in self.AXIS])
+        for a in self.AXIS:
+            sub[a].soa = data[a].soa[:, :sample]
+        report = {}
+        for op in ops_:
+            times = {}
+            if op == "backprojection":
+                proj = self.projection(image, sub)
+            for sch in SCHEDULES:
+                self.schedule[op] = sch
+                if op == "osem_subset":
+                    fn = lambda: self.osem_subset(image, sub)
+                elif op == "projection":
+                    fn = lambda: self.projection(image, sub)
+                else:
+                    fn = lambda: self.backprojection(proj, image)
+                times[sch] = _time(fn, reps)
+            self.schedule[op] = min(times, key=times.get)
+            report[op] = times
+        return report
 
     def _model(self):
         return _lib.make_tor_model(self.kernel_width, self.geo_sigma2_flag, self.tof_bin,
@@ -71,7 +103,7 @@ class SplitLoRsModel:
             p = torch.empty(n, dtype=torch.float32, device=x.device)
             if n:
                 _lib.call("srfx_tor_project", _ptr(lors), n, ld, _ptr(x), self._grid(image, a),
-                          self._model(), _ptr(p), 0, _stream())
+                          self._model(), _ptr(p), self._flags("projection"), _stream())
             out[a] = lm.replace_values(p)
         return ListModeDataSplit(out["x"], out["y"], out["z"])
 
@@ -88,7 +120,7 @@ class SplitLoRsModel:
             v = _vec(lm.values, n, "lors_value")
             _lib.call("srfx_tor_backproject", _ptr(lors), n, ld, _ptr(v),
                       self._grid(Image(out, image.center, image.size), a), self._model(),
-                      _ptr(out), _lib.SRFX_ACCUMULATE, _stream())
+                      _ptr(out), _lib.SRFX_ACCUMULATE | self._flags("backprojection"), _stream())
         return Image(out, image.center, image.size)
 
     def maplors(self, data, image, out=None):
@@ -122,7 +154,8 @@ class SplitLoRsModel:
             g_in = self._grid(image, a)
             g_out = self._grid(Image(out, image.center, image.size), a)
             _lib.call("srfx_tor_osem_subset", _ptr(lors), n, ld, _ptr(x), g_in, g_out,
-                      self._model(), _ptr(out), _lib.SRFX_ACCUMULATE, _stream())
+                      self._model(), _ptr(out), _lib.SRFX_ACCUMULATE | self._flags("osem_subset"),
+                      _stream())
         return Image(out, image.center, image.size)
 
     def count_nnz(self, image, data):

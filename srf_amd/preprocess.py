@@ -120,3 +120,63 @@ def map_process(lors, return_order=False):
         out[a] = g.to(torch.float32).t().contiguous()
         orders[a] = order
     return (out, orders) if return_order else out
+
+
+def _spread8(v):
+    """Spread the low 8 bits of v so that bit i moves to bit 4 i (for a
+    4-way Morton interleave)."""
+    v = v & 0xFF
+    out = torch.zeros_like(v)
+    for i in range(8):
+        out |= ((v >> i) & 1) << (4 * i)
+    return out
+
+
+def locality_key(soa, center_z, zc_bin=40.0, slope_bin=0.04, pos_bin=6.0):
+    """int64 key per event of a [10, N] group (op frame) that places events
+    with nearby, nearly parallel tubes and nearby TOF centres next to each
+    other: coarse TOF-centre position along the slice axis first, then a
+    Morton interleave of the two slopes and the crossing point with the
+    plane z = center_z."""
+    x1, y1, z1 = soa[0].double(), soa[1].double(), soa[2].double()
+    dx = soa[3].double() - x1
+    dy = soa[4].double() - y1
+    dz = soa[5].double() - z1
+    dz = torch.where(dz == 0, torch.full_like(dz, 1e-12), dz)
+    sx, sy = dx / dz, dy / dz
+    X0 = x1 + sx * (center_z - z1)
+    Y0 = y1 + sy * (center_z - z1)
+
+    def q(v, step):
+        return torch.clamp(torch.floor(v / step) + 128, 0, 255).long()
+
+    zc = torch.clamp(torch.floor(soa[8].double() / zc_bin) + 64, 0, 127).long()
+    m = _spread8(q(sx, slope_bin)) | (_spread8(q(sy, slope_bin)) << 1) | \
+        (_spread8(q(X0, pos_bin)) << 2) | (_spread8(q(Y0, pos_bin)) << 3)
+    return (zc << 32) | m
+
+
+def locality_order(soa, nb_subsets, center_z=0.0):
+    """Permutation of a group's events that sorts each OSEM subset's rows
+    [k N / K, (k+1) N / K) by locality_key and leaves subset membership
+    unchanged."""
+    n = soa.shape[1]
+    key = locality_key(soa, center_z)
+    order = torch.empty(n, dtype=torch.int64, device=soa.device)
+    for k in range(nb_subsets):
+        lo, hi = (k * n) // nb_subsets, ((k + 1) * n) // nb_subsets
+        if hi > lo:
+            order[lo:hi] = lo + torch.argsort(key[lo:hi], stable=True)
+    return order
+
+
+def reorder_groups(groups, nb_subsets, centers=None):
+    """Apply locality_order to every group; returns (groups, orders) with
+    orders[a][i] = original row of output row i."""
+    out, orders = {}, {}
+    for a, soa in groups.items():
+        cz = 0.0 if centers is None else centers[a]
+        o = locality_order(soa, nb_subsets, cz)
+        out[a] = soa[:, o].contiguous()
+        orders[a] = o
+    return out, orders

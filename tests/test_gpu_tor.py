@@ -157,3 +157,25 @@ def test_chords_equal_full_patch(cuda, case, geo):
     a = ops.projection(L, X, *args)
     b = ops.projection(L, X, *args, full_patch=True)
     assert np.array_equal(a.cpu().numpy(), b.cpu().numpy())
+
+
+@pytest.mark.parametrize("sch", [(1, 64), (4, 128), (16, 256), (4, 64)])
+def test_schedules_agree(cuda, case, sch):
+    """Every (lanes per event, block) schedule gives the same results to FP32
+    rounding (partial sums combined in another order)."""
+    from srf_amd import ops
+    from oracle import lib as O
+    lors, img = case
+    args = (H.TOR_GRID, H.TOR_CENTER, H.TOR_SIZE, H.KW, True, H.TOF_BIN, H.TOF_SIGMA2)
+    L, X = _t(lors, cuda), _t(img, cuda)
+    ref = O.tor_project(lors, img, *args, prec="f64")
+    p = ops.projection(L, X, *args, schedule=sch)
+    assert H.rel_l2(p.cpu().numpy(), ref) <= TOL64
+    b0 = ops.backprojection(X, L, p, *args[:3], H.TOF_BIN, H.TOF_SIGMA2, H.KW, True)
+    b1 = ops.backprojection(X, L, p, *args[:3], H.TOF_BIN, H.TOF_SIGMA2, H.KW, True, schedule=sch)
+    assert H.rel_l2(b1.cpu().numpy(), b0.cpu().numpy()) <= 1e-6
+    f0 = ops.osem_subset(L, X, *args)
+    f1 = ops.osem_subset(L, X, *args, schedule=sch)
+    assert H.rel_l2(f1.cpu().numpy(), f0.cpu().numpy()) <= 1e-6
+    per, tot = ops.count_nnz(L, *args[:5])
+    assert np.array_equal(per.cpu().numpy(), O.tor_count(lors, *args[:5], prec="f64"))
