@@ -118,3 +118,22 @@ def test_tof_info():
     tb, ts2 = OP.tof_info(530.0, 40.0)
     assert tb == pytest.approx(6.0)
     assert np.sqrt(ts2) == pytest.approx(530 * 0.15 / 2.35482005)
+
+
+def test_locality_order_keeps_subsets():
+    """Reordering for cache locality permutes rows only inside each OSEM
+    subset [k N / K, (k+1) N / K), so subset membership (worker.py:79-85) and
+    therefore the OSEM result are unchanged."""
+    from srf_amd import preprocess as P
+    rows = _events(4000, seed=5)
+    g = P.recon_process(torch.from_numpy(rows), 530.0)
+    for K in (1, 3, 7):
+        out, orders = P.reorder_groups(g, K)
+        for a in "xyz":
+            n = g[a].shape[1]
+            o = orders[a].numpy()
+            assert sorted(o.tolist()) == list(range(n))
+            for k in range(K):
+                lo, hi = k * n // K, (k + 1) * n // K
+                assert set(o[lo:hi].tolist()) == set(range(lo, hi))
+            np.testing.assert_array_equal(out[a].numpy(), g[a].numpy()[:, o])
