@@ -38,6 +38,13 @@ def parse():
                     help="target wall time of the CPU baseline sample (0 = skip)")
     ap.add_argument("--unfused", action="store_true",
                     help="time separate projection + backprojection kernels")
+    ap.add_argument("--workload", default="tof_tor_c2",
+                    choices=["tof_tor_c2", "siddon_c1", "autograd_c3", "batched_c4",
+                             "large_c5"])
+    ap.add_argument("--autotune", action="store_true",
+                    help="tune the launch schedule on a sample before timing")
+    ap.add_argument("--sort", action="store_true",
+                    help="reorder events inside each subset for cache locality")
     return ap.parse_args()
 
 
@@ -88,6 +95,11 @@ def main():
         print(json.dumps({"error": "this benchmark runs on one GPU; multi-GPU runs are not "
                           "part of this build (use --gpus 1)"}))
         sys.exit(2)
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    if args.workload != "tof_tor_c2":
+        import bench_workloads
+        bench_workloads.run(args)
+        return
     import torch
     from srf_amd import effmap, physics, synth, preprocess
     from srf_amd.data import Image, split_from_groups
@@ -121,7 +133,15 @@ def main():
     x = effmap.initial_image(grid, center, size, scanner.inner_radius, scanner.axial_length, dev)
     image = Image(x, center, size)
     emap = Image(E, center, size)
+    if args.sort:
+        groups, _ = preprocess.reorder_groups(groups, args.subsets)
     data = split_from_groups(groups)
+    tuned = None
+    if args.autotune:
+        t0 = time.time()
+        tuned = model.autotune(image, data, ops_=("osem_subset",))
+        log(f"autotuned osem_subset schedule {model.schedule['osem_subset']} in "
+            f"{time.time() - t0:.1f}s")
     osem = OSEM(model, data, emap, nb_subsets=args.subsets, fused=not args.unfused)
 
     # exact non-zero counts per launch (x, y, z group of each subset)
@@ -216,7 +236,10 @@ def main():
                    "kernel_width_mm": meta["kernel_width"], "tof_fwhm_ps": 530,
                    "tof_sigma2_mm2": round(meta["tof_sigma2"], 3), "tof_bin_mm": meta["tof_bin"],
                    "nnz_per_event": round(total_nnz / args.events, 2),
-                   "fused": not args.unfused},
+                   "fused": not args.unfused, "sorted": bool(args.sort),
+                   "schedule": model.schedule["osem_subset"],
+                   "autotune_ms": ({str(k): round(v, 3) for k, v in tuned["osem_subset"].items()}
+                                   if tuned else None)},
         "roofline": roof,
         "cpu_baseline": cpu,
     }

@@ -1,0 +1,366 @@
+"""Secondary benchmark workloads of bench.py (``--workload``); each prints one
+JSON line in bench.py's format.
+
+siddon_c1    non-TOF Siddon projection + backprojection, 2 iterations x 4
+             subsets of list-mode OSEM with Poisson counts; 128x128x64 image
+             (3.0x3.0x3.4 mm), 1M random crystal pairs.  A step is one OSEM
+             iteration.
+autograd_c3  differentiable projector: batch of 8 lumpy 128x128x64 images,
+             2M TOF events each, Poisson NLL; forward + backward through
+             torch.autograd against the plain projection + adjoint calls, and
+             the memory of recomputation against stored voxel lists.
+batched_c4   64 independent TOF OSEM reconstructions (image sizes mixed from
+             96x96x48, 128x128x64, 200x200x100; events log-uniform in
+             [1e5, 5e6]), 2 iterations x 8 subsets: one persistent launch per
+             subset replayed from a HIP graph, against the same
+             reconstructions run one after another.
+large_c5     256x256x160 (1.6x1.6x1.7 mm) torso phantom, 300M events at
+             400 ps, attenuated efficiency map, 2 iterations x 20 subsets.
+"""
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def _base(metric, value, unit, steps, warmup, ms_per_step, config, **extra):
+    out = {"metric": metric, "value": round(value, 3), "unit": unit, "n_gpus": 1,
+           "steps": steps, "warmup": warmup, "ms_per_step": round(ms_per_step, 3),
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+           "data": "synthetic", "config": config}
+    out.update(extra)
+    return out
+
+
+def _timer():
+    import torch
+    a = torch.cuda.Event(enable_timing=True)
+    b = torch.cuda.Event(enable_timing=True)
+    return a, b
+
+
+def run(args):
+    {"siddon_c1": siddon_c1, "autograd_c3": autograd_c3, "batched_c4": batched_c4,
+     "large_c5": large_c5}[args.workload](args)
+
+
+# --------------------------------------------------------------------- C1 ---
+
+def siddon_c1(args):
+    import torch
+    from srf_amd import effmap, physics, synth
+    from srf_amd.data import Image, ListModeData
+    from srf_amd.scanner import CylindricalScanner
+    dev = torch.device("cuda:0")
+    sc = CylindricalScanner()
+    grid, center, size = (128, 128, 64), (0.0, 0.0, 0.0), (384.0, 384.0, 217.6)
+    ph = synth.c1_phantom(seed=1)
+    truth = torch.from_numpy(ph.voxelize(grid, center, size)).to(dev)
+    pairs = torch.from_numpy(synth.c1_pairs(1_000_000, seed=1, scanner=sc)).to(dev)
+    soa = torch.zeros(7, pairs.shape[0], device=dev)
+    soa[0:6] = pairs.t()
+    model = physics.CompleteLoRsModel("siddon", tof_sigma2=1.0e6, tof_bin=1.0)
+    p_true = model.projection(Image(truth, center, size), ListModeData(soa, soa=True)).values
+    alpha = 10.0 / float(p_true.mean())
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(1)
+    counts = torch.poisson(alpha * p_true, generator=gen)
+    keep = counts > 0
+    soa, counts = soa[:, keep].contiguous(), counts[keep]
+    n = soa.shape[1]
+    log(f"C1: {n} LORs with counts (alpha {alpha:.3g})")
+    # sensitivity: Siddon Maplors over all crystal pairs, then merge_map's finalisation
+    sens = torch.zeros(grid, device=dev)
+    total = sc.nb_pairs()
+    for s0 in range(0, total, 8_000_000):
+        pr = torch.from_numpy(sc.all_pairs((s0, min(total, s0 + 8_000_000)))).to(dev)
+        ps = torch.zeros(7, pr.shape[0], device=dev)
+        ps[0:6] = pr.t()
+        model.maplors(ListModeData(ps, torch.ones(pr.shape[0], device=dev), soa=True),
+                      Image(sens, center, size), out=sens)
+    E = effmap.finalize(sens, grid, center, size, sc.inner_radius, sc.axial_length)
+    x = effmap.initial_image(grid, center, size, sc.inner_radius, sc.axial_length, dev)
+    K = 4
+    bounds = [((k * n) // K, ((k + 1) * n) // K) for k in range(K)]
+    bp = torch.empty_like(x)
+    from srf_amd import ops
+
+    def iteration():
+        for lo, hi in bounds:
+            lm = ListModeData(soa[:, lo:hi], soa=True)
+            p = model.projection(Image(x, center, size), lm).values
+            v = p / counts[lo:hi]  # lors_value = p / c: sum_i c_i a_ij / p_i
+            bp.zero_()
+            model.backprojection(lm.replace_values(v), Image(x, center, size), out=bp)
+            ops.mlem_update(x, E, bp)
+
+    for _ in range(args.warmup):
+        iteration()
+    torch.cuda.synchronize()
+    a, b = _timer()
+    a.record()
+    for _ in range(args.steps):
+        iteration()
+    b.record()
+    torch.cuda.synchronize()
+    ms = a.elapsed_time(b) / args.steps
+    value = n / (ms / 1e3) / 1e6
+    # per-kernel timing for the roofline: projection of the first subset
+    lm = ListModeData(soa[:, bounds[0][0]:bounds[0][1]], soa=True)
+    # the Siddon op frame is the world frame (complete_lors.py:61-70)
+    per, tot = ops.siddon_count_nnz(lm.soa, grid, center, size, 1.0, 1.0e6)
+    nnz = int(tot.item())
+    a, b = _timer()
+    a.record()
+    for _ in range(5):
+        model.projection(Image(x, center, size), lm)
+    b.record()
+    torch.cuda.synchronize()
+    pms = a.elapsed_time(b) / 5
+    nl = bounds[0][1] - bounds[0][0]
+    byts = nl * (28 + 4) + 4 * nnz + 4 * x.numel()
+    ach = byts / (pms / 1e3) / 1e9
+    cpu = None
+    if args.cpu_seconds > 0:
+        from oracle import lib as O
+        O.build()
+        thr = len(os.sched_getaffinity(0))
+        xt = np.ascontiguousarray(x.permute(2, 1, 0).cpu().numpy())  # [gz][gy][gx] layout
+        m = min(nl, 20000)
+        sub = np.ascontiguousarray(lm.soa[:, :m].cpu().numpy())
+        t0 = time.perf_counter()
+        p = O.siddon_project(sub, xt, grid, center, size, 1.0, 1.0e6, prec="f32", nthreads=thr)
+        O.siddon_backproject(sub, p.astype(np.float32), grid, center, size, 1.0, 1.0e6,
+                             prec="f32", nthreads=thr)
+        dt = time.perf_counter() - t0
+        m2 = int(min(nl, max(m, m * args.cpu_seconds / max(dt, 1e-3))))
+        sub = np.ascontiguousarray(lm.soa[:, :m2].cpu().numpy())
+        t0 = time.perf_counter()
+        p = O.siddon_project(sub, xt, grid, center, size, 1.0, 1.0e6, prec="f32", nthreads=thr)
+        O.siddon_backproject(sub, p.astype(np.float32), grid, center, size, 1.0, 1.0e6,
+                             prec="f32", nthreads=thr)
+        dt = time.perf_counter() - t0
+        cpu = {"value": m2 / dt / 1e6, "unit": "Mevents/s", "cores": thr, "kind": "port",
+               "sample": f"first {m2} LORs, Siddon projection + ratio backprojection, FP32 "
+                         f"port of siddon.cu.cc, {thr} threads"}
+    print(json.dumps(_base(
+        "Siddon OSEM Mevents/s (proj+backproj), non-TOF", value, "Mevents/s", args.steps,
+        args.warmup, ms,
+        {"workload": "siddon_c1", "image": list(grid), "voxel_mm": [3.0, 3.0, 3.4], "lors": n,
+         "subsets": K, "step": "one OSEM iteration over all LORs",
+         "nnz_per_lor": round(nnz / nl, 1)},
+        roofline={"bound": "hbm", "achieved": round(ach, 1), "peak": 8000.0, "unit": "GB/s",
+                  "frac": round(ach / 8000.0, 4), "traffic": None,
+                  "kernel": "siddon_kernel<kProj>", "mean_launch_ms": round(pms, 4),
+                  "bytes_formula": "sum_i (28 + 4 + 4*nnz_i) + 4*N_vox"},
+        cpu_baseline=cpu)), flush=True)
+
+
+# --------------------------------------------------------------------- C3 ---
+
+def autograd_c3(args):
+    import torch
+    from srf_amd import autograd, physics, preprocess, synth
+    from srf_amd.data import Image, split_from_groups
+    from srf_amd.scanner import CylindricalScanner
+    dev = torch.device("cuda:0")
+    sc = CylindricalScanner()
+    grid, center, size = (128, 128, 64), (0.0, 0.0, 0.0), (384.0, 384.0, 217.6)
+    tb, ts2 = sc.tof_info()
+    model = physics.SplitLoRsModel(4.0, False, tof_sigma2=ts2, tof_bin=tb)
+    B, nev = 8, 2_000_000
+    items = []
+    for b in range(B):
+        img = synth.lumpy_image(grid, center, size, seed=100 + b, device=dev)
+        ev, _ = synth.tof_events_from_image(nev, img, center, size, seed=200 + b, scanner=sc,
+                                            device=dev)
+        data = split_from_groups(preprocess.recon_process(ev, sc.tof_res_ps))
+        items.append((img, data))
+    sens = torch.ones(grid, device=dev)
+
+    def grad_autograd():
+        grads = []
+        for img, data in items:
+            x = img.clone().requires_grad_(True)
+            p = autograd.project(Image(x, center, size), data, model)
+            loss = autograd.poisson_nll(p.clamp_min(1e-7), x, sens)
+            loss.backward()
+            grads.append(x.grad)
+        return grads
+
+    def grad_plain():
+        grads = []
+        for img, data in items:
+            with torch.no_grad():
+                proj = model.projection(Image(img, center, size), data)
+                bp = model.backprojection(proj, Image(img, center, size)).data
+                grads.append(sens - bp)
+        return grads
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        a, b = _timer()
+        a.record()
+        for _ in range(args.steps):
+            out = fn()
+        b.record()
+        torch.cuda.synchronize()
+        return a.elapsed_time(b) / args.steps, out
+
+    torch.cuda.reset_peak_memory_stats()
+    base_mem = torch.cuda.memory_allocated()
+    ms_ag, g_ag = timed(grad_autograd)
+    peak_ag = torch.cuda.max_memory_allocated() - base_mem
+    ms_pl, g_pl = timed(grad_plain)
+    err = max(float(torch.linalg.norm(a - b) / torch.linalg.norm(b)) for a, b in zip(g_ag, g_pl))
+    stored = sum(autograd.stored_list_bytes(model, Image(img, center, size), data)
+                 for img, data in items)
+    total = B * nev
+    print(json.dumps(_base(
+        "differentiable TOF projector Mevents/s (forward + backward)", total / (ms_ag / 1e3) / 1e6,
+        "Mevents/s", args.steps, 1, ms_ag,
+        {"workload": "autograd_c3", "images": B, "image": list(grid), "events_per_image": nev,
+         "loss": "list-mode Poisson NLL", "step": "gradient of all 8 images"},
+        grad_vs_plain={"autograd_ms": round(ms_ag, 3), "plain_ms": round(ms_pl, 3),
+                       "overhead": round(ms_ag / ms_pl, 3), "max_rel_l2_grad_diff": err},
+        memory={"recompute_peak_bytes": int(peak_ag), "stored_voxel_list_bytes": int(stored),
+                "note": "backward recomputes the weights; a stored-list backward keeps an "
+                        "int32 index and a float32 weight per non-zero entry"})), flush=True)
+
+
+# --------------------------------------------------------------------- C4 ---
+
+def batched_c4(args):
+    import torch
+    from srf_amd import effmap, physics, preprocess, synth
+    from srf_amd.batch import BatchedOSEM, serial_iterate
+    from srf_amd.data import Image, split_from_groups
+    from srf_amd.scanner import CylindricalScanner
+    dev = torch.device("cuda:0")
+    sc = CylindricalScanner()
+    tb, ts2 = sc.tof_info()
+    model = physics.SplitLoRsModel(4.0, False, tof_sigma2=ts2, tof_bin=tb)
+    kinds = [((96, 96, 48), (384.0, 384.0, 216.0)), ((128, 128, 64), (384.0, 384.0, 217.6)),
+             ((200, 200, 100), (400.0, 400.0, 220.0))]
+    center = (0.0, 0.0, 0.0)
+    npairs = args.effmap_pairs or None
+    emaps = {}
+    for g, s in kinds:
+        summed = effmap.compute_tor_effmap(sc, model, g, center, s, dev, max_pairs=npairs)
+        emaps[g] = effmap.finalize(summed, g, center, s, sc.inner_radius, sc.axial_length)
+    rng = np.random.default_rng(4)
+    R, K, iters = 64, 8, 2
+    recons, total = [], 0
+    for r in range(R):
+        g, s = kinds[int(rng.integers(0, 3))]
+        n = int(np.exp(rng.uniform(np.log(1e5), np.log(5e6))))
+        ph = synth.c2_phantom(seed=300 + r, scanner=sc)
+        ev, _ = synth.tof_events(n, ph, seed=400 + r, scanner=sc, device=dev)
+        data = split_from_groups(preprocess.recon_process(ev, sc.tof_res_ps))
+        x0 = effmap.initial_image(g, center, s, sc.inner_radius, sc.axial_length, dev)
+        recons.append({"image": Image(x0, center, s), "emap": Image(emaps[g], center, s),
+                       "data": data, "x0": x0.clone()})
+        total += n
+    log(f"C4: {R} reconstructions, {total} events")
+
+    def reset():
+        for rc in recons:
+            rc["image"].data.copy_(rc["x0"])
+
+    bat = BatchedOSEM(recons, model, nb_subsets=K)
+    bat.capture()
+    reset()
+    bat.replay(1)  # warm-up
+    reset()
+    torch.cuda.synchronize()
+    a, b = _timer()
+    a.record()
+    bat.replay(iters)
+    b.record()
+    torch.cuda.synchronize()
+    ms_b = a.elapsed_time(b)
+    out_b = [rc["image"].data.clone() for rc in recons]
+    reset()
+    serial_iterate(recons, model, K, 1)  # warm-up
+    reset()
+    torch.cuda.synchronize()
+    a, b = _timer()
+    a.record()
+    serial_iterate(recons, model, K, iters)
+    b.record()
+    torch.cuda.synchronize()
+    ms_s = a.elapsed_time(b)
+    err = max(float(torch.linalg.norm(x - rc["image"].data) / torch.linalg.norm(rc["image"].data))
+              for x, rc in zip(out_b, recons))
+    bat.close()
+    print(json.dumps(_base(
+        "batched TOF OSEM Mevents/s (proj+backproj), 64 reconstructions",
+        total * iters / (ms_b / 1e3) / 1e6, "Mevents/s", iters, 1, ms_b / iters,
+        {"workload": "batched_c4", "reconstructions": R, "events": total, "subsets": K,
+         "iterations": iters, "image_sizes": [list(k[0]) for k in kinds],
+         "step": "one OSEM iteration of all reconstructions"},
+        batched_vs_serial={"batched_graph_ms": round(ms_b, 2), "serial_ms": round(ms_s, 2),
+                           "speedup": round(ms_s / ms_b, 3),
+                           "serial_Mevents_per_s": round(total * iters / (ms_s / 1e3) / 1e6, 2),
+                           "max_rel_l2_image_diff": err})), flush=True)
+
+
+# --------------------------------------------------------------------- C5 ---
+
+def large_c5(args):
+    import torch
+    from srf_amd import attenuation, effmap, physics, preprocess, synth
+    from srf_amd.data import Image, split_from_groups
+    from srf_amd.recon import OSEM
+    from srf_amd.scanner import CylindricalScanner
+    dev = torch.device("cuda:0")
+    sc = CylindricalScanner(tof_res_ps=400.0)
+    grid, center, size = (256, 256, 160), (0.0, 0.0, 0.0), (409.6, 409.6, 272.0)
+    tb, ts2 = sc.tof_info()
+    model = physics.SplitLoRsModel(4.0, False, tof_sigma2=ts2, tof_bin=tb)
+    act, mu = synth.torso_image(grid, center, size, seed=5, device=dev)
+    nev = args.events if args.events != 20_000_000 else 300_000_000
+    t0 = time.time()
+    groups = {a: [] for a in "xyz"}
+    for c0 in range(0, nev, 25_000_000):
+        m = min(25_000_000, nev - c0)
+        ev, _ = synth.tof_events_from_image(m, act, center, size, seed=500 + c0 // 25_000_000,
+                                            scanner=sc, device=dev)
+        g = preprocess.recon_process(ev, sc.tof_res_ps)
+        del ev
+        for a in "xyz":
+            groups[a].append(g[a])
+    groups = {a: torch.cat(v, 1) for a, v in groups.items()}
+    torch.cuda.synchronize()
+    log(f"C5: {nev} events generated + preprocessed in {time.time() - t0:.1f}s")
+    t0 = time.time()
+    summed = attenuation.attenuated_tor_effmap(sc, model, Image(mu, center, size), grid, center,
+                                               size, dev, max_pairs=args.effmap_pairs or None)
+    E = effmap.finalize(summed, grid, center, size, sc.inner_radius, sc.axial_length)
+    torch.cuda.synchronize()
+    log(f"attenuated efficiency map in {time.time() - t0:.1f}s")
+    x = effmap.initial_image(grid, center, size, sc.inner_radius, sc.axial_length, dev)
+    K = 20
+    osem = OSEM(model, split_from_groups(groups), Image(E, center, size), nb_subsets=K)
+    img = Image(x, center, size)
+    torch.cuda.synchronize()
+    a, b = _timer()
+    a.record()
+    osem.iterate(img, args.steps)
+    b.record()
+    torch.cuda.synchronize()
+    ms = a.elapsed_time(b) / args.steps
+    assert torch.isfinite(x).all().item()
+    print(json.dumps(_base(
+        "TOF OSEM Mevents/s (proj+backproj), large frame", nev / (ms / 1e3) / 1e6, "Mevents/s",
+        args.steps, 0, ms,
+        {"workload": "large_c5", "image": list(grid), "voxel_mm": [1.6, 1.6, 1.7],
+         "events": nev, "subsets": K, "tof_fwhm_ps": 400, "attenuation": "mu 0.0096/mm body",
+         "step": "one OSEM iteration over all events"})), flush=True)

@@ -1,7 +1,9 @@
 """Seeded synthetic phantoms and list-mode data for the benchmark workloads.
 
-All generators use numpy.random.Generator(PCG64(seed)); every parameter that
-shapes the data is returned in the ``meta`` dict.
+Phantom parameters come from numpy.random.Generator(PCG64(seed)); event
+streams from a seeded torch.Generator on the requested device (so they depend
+on seed and device type).  Every parameter that shapes the data is returned in
+the ``meta`` dict.
 
 Workloads
   siddon_c1  non-TOF Siddon, 128x128x64 image (3.0x3.0x3.4 mm), 1M random
@@ -108,18 +110,8 @@ def c2_phantom(seed=2, scanner=CylindricalScanner()):
     return Phantom(150.0, half_h, _place_spheres(rng, radii, acts, 150.0, half_h))
 
 
-def _snap(scanner, pts):
-    ang = np.arctan2(pts[:, 1], pts[:, 0])
-    step = 2 * np.pi / scanner.nb_crystals
-    crystal = np.mod(np.rint(ang / step).astype(np.int64), scanner.nb_crystals)
-    z0 = scanner.ring_z()[0]
-    pitch = scanner.ring_pitch + scanner.gap
-    ring = np.clip(np.rint((pts[:, 2] - z0) / pitch).astype(np.int64), 0, scanner.nb_rings - 1)
-    return scanner.crystal_xyz(ring, crystal)
-
-
 def tof_events(n, phantom, seed=2, scanner=CylindricalScanner(), device="cpu",
-               batch=8_000_000):
+               batch=8_000_000, gen=None):
     """List-mode TOF events, torch float64 [n, 7] on ``device``:
     x1,y1,z1,x2,y2,z2,tof.
 
@@ -132,8 +124,9 @@ def tof_events(n, phantom, seed=2, scanner=CylindricalScanner(), device="cpu",
     CutLoRs: positive tof = closer to p1).  The stream is seeded with a torch
     generator on ``device``, so the events depend on seed and device type."""
     import torch
-    gen = torch.Generator(device=device)
-    gen.manual_seed(seed)
+    if gen is None:
+        gen = torch.Generator(device=device)
+        gen.manual_seed(seed)
     R = scanner.radius
     half = scanner.axial_length / 2
     sigma = tof_sigma_mm(scanner.tof_res_ps)
@@ -217,3 +210,88 @@ def c1_pairs(n, seed=1, scanner=CylindricalScanner(), min_sep=96):
     p1 = scanner.crystal_xyz(r1, c1)
     p2 = scanner.crystal_xyz(r2, c2)
     return np.hstack([p1, p2]).astype(np.float32)
+
+
+def tof_events_from_image(n, image, center, size, seed=2, scanner=CylindricalScanner(),
+                          tof_res_ps=None, device="cpu", batch=8_000_000):
+    """List-mode TOF events like tof_events, with emission points drawn from a
+    voxelized activity image (torch (nx, ny, nz), non-negative): a voxel by its
+    value, then a uniform point inside it."""
+    import torch
+    gen = torch.Generator(device=device)
+    gen.manual_seed(seed)
+    img = image.to(device=device, dtype=torch.float64).reshape(-1)
+    grid = list(image.shape)
+    vox = torch.tensor([size[k] / grid[k] for k in range(3)], dtype=torch.float64, device=device)
+    lo = torch.tensor([center[k] - size[k] / 2 for k in range(3)], dtype=torch.float64,
+                      device=device)
+    strides = [grid[1] * grid[2], grid[2], 1]
+
+    class _ImagePhantom:
+        radius = float(min(np.hypot(size[0], size[1]) / 2, 0.99 * scanner.radius))
+
+        @staticmethod
+        def sample(g, m, dev):
+            idx = torch.multinomial(img, m, replacement=True, generator=g)
+            ijk = torch.stack([idx // strides[0], (idx // strides[1]) % grid[1],
+                               idx % grid[2]], 1).to(torch.float64)
+            u = torch.rand(m, 3, dtype=torch.float64, device=dev, generator=g)
+            return lo + (ijk + u) * vox
+
+    sc = scanner
+    if tof_res_ps is not None:
+        from dataclasses import replace
+        sc = replace(scanner, tof_res_ps=tof_res_ps)
+    return tof_events(n, _ImagePhantom(), seed=seed, scanner=sc, device=device, batch=batch,
+                      gen=gen)
+
+
+def lumpy_image(grid, center, size, seed, n_blobs=50, cyl_radius=150.0, device="cpu"):
+    """Sum of n_blobs Gaussians (sigma uniform in [5, 20] mm, centres uniform in
+    the cylinder of radius cyl_radius and the image's axial extent)."""
+    import torch
+    rng = np.random.default_rng(seed)
+    axes = [center[k] - size[k] / 2 + (np.arange(grid[k]) + 0.5) * size[k] / grid[k]
+            for k in range(3)]
+    X, Y, Z = [torch.tensor(a, dtype=torch.float64, device=device) for a in axes]
+    img = torch.zeros(grid, dtype=torch.float64, device=device)
+    for _ in range(n_blobs):
+        r = cyl_radius * np.sqrt(rng.uniform())
+        ph = rng.uniform(0, 2 * np.pi)
+        c = (r * np.cos(ph), r * np.sin(ph), rng.uniform(-size[2] / 2, size[2] / 2))
+        s = rng.uniform(5, 20)
+        a = rng.uniform(0.5, 2.0)
+        gx = torch.exp(-(X - c[0]) ** 2 / (2 * s * s))
+        gy = torch.exp(-(Y - c[1]) ** 2 / (2 * s * s))
+        gz = torch.exp(-(Z - c[2]) ** 2 / (2 * s * s))
+        img += a * gx[:, None, None] * gy[None, :, None] * gz[None, None, :]
+    return img.to(torch.float32)
+
+
+def torso_image(grid, center, size, seed=5, device="cpu"):
+    """Torso-like activity and mu-map (nx, ny, nz): elliptic body 350 x 250 mm
+    (activity 1, mu 0.0096/mm), two lungs (activity 0.3), liver (1.5) and
+    10 spherical lesions of 5-15 mm radius at 3-6x the background."""
+    import torch
+    rng = np.random.default_rng(seed)
+    axes = [center[k] - size[k] / 2 + (np.arange(grid[k]) + 0.5) * size[k] / grid[k]
+            for k in range(3)]
+    X, Y, Z = torch.meshgrid(*[torch.tensor(a, dtype=torch.float64, device=device)
+                               for a in axes], indexing="ij")
+    body = (X / 175.0) ** 2 + (Y / 125.0) ** 2 <= 1.0
+    act = body.to(torch.float64)
+    for cx in (-80.0, 80.0):
+        lung = ((X - cx) / 55.0) ** 2 + ((Y - 10.0) / 75.0) ** 2 + (Z / 120.0) ** 2 <= 1.0
+        act = torch.where(lung, torch.full_like(act, 0.3), act)
+    liver = ((X + 60.0) / 80.0) ** 2 + ((Y + 20.0) / 60.0) ** 2 + ((Z + 90.0) / 60.0) ** 2 <= 1
+    act = torch.where(liver & body, torch.full_like(act, 1.5), act)
+    for _ in range(10):
+        r = rng.uniform(5, 15)
+        while True:
+            c = (rng.uniform(-150, 150), rng.uniform(-100, 100), rng.uniform(-100, 100))
+            if (c[0] / 175) ** 2 + (c[1] / 125) ** 2 < 0.8:
+                break
+        les = (X - c[0]) ** 2 + (Y - c[1]) ** 2 + (Z - c[2]) ** 2 <= r * r
+        act = torch.where(les, torch.full_like(act, rng.uniform(3, 6)), act)
+    mu = torch.where(body, torch.full_like(act, 0.0096), torch.zeros_like(act))
+    return act.to(torch.float32), mu.to(torch.float32)
