@@ -3,13 +3,16 @@
 Workload (tof_tor_c2): TOF tube-of-response list-mode OSEM, 200x200x100 image
 of 2.0x2.0x2.2 mm, 20M synthetic events (cylinder + 6 spheres, 530 ps FWHM,
 kernel_width 4 mm), 10 subsets.  A step is one OSEM iteration: every event is
-projected and backprojected once (10 subsets x 3 axis groups of the fused
-projection + ratio-backprojection kernel, then the image update).  Events are
-generated, preprocessed and the efficiency map built on the GPU before the
-timed region; inputs are resident in HBM when it starts.
+projected and backprojected once.  Per subset and axis group: the projection
+kernel, then the tiled backprojection of 1/p (events binned into image tiles
+once per subset, accumulated in LDS); then the image update.  Events are
+generated, preprocessed (axis groups, locality order inside each subset), the
+tile plans built and the efficiency map computed on the GPU before the timed
+region; inputs are resident in HBM when it starts.  --untiled times the
+single fused projection + scatter kernel per group instead.
 
 Prints one JSON line:  value = events * steps / time (Mevents/s, proj+bp);
-roofline = the fused kernel's algorithmic bytes per launch over its mean
+roofline = the dominant kernel's algorithmic bytes per launch over its mean
 launch time (HIP events on the launch stream); cpu_baseline = the FP32 CPU
 port of the reference kernels on a bounded sample of the same events.
 """
@@ -43,8 +46,12 @@ def parse():
                              "large_c5"])
     ap.add_argument("--autotune", action="store_true",
                     help="tune the launch schedule on a sample before timing")
-    ap.add_argument("--sort", action="store_true",
-                    help="reorder events inside each subset for cache locality")
+    ap.add_argument("--no-sort", dest="sort", action="store_false",
+                    help="keep the generated event order inside each subset (default: "
+                         "reorder events inside each subset for locality)")
+    ap.add_argument("--untiled", action="store_true",
+                    help="one fused projection + scatter kernel per group instead of the "
+                         "projection kernel + tiled backprojection")
     return ap.parse_args()
 
 
@@ -103,6 +110,7 @@ def main():
     import torch
     from srf_amd import effmap, physics, synth, preprocess
     from srf_amd.data import Image, split_from_groups
+    from srf_amd.plan import DEFAULT_TILE
     from srf_amd.recon import OSEM, subset_view
     from srf_amd.scanner import CylindricalScanner
 
@@ -142,7 +150,18 @@ def main():
         tuned = model.autotune(image, data, ops_=("osem_subset",))
         log(f"autotuned osem_subset schedule {model.schedule['osem_subset']} in "
             f"{time.time() - t0:.1f}s")
-    osem = OSEM(model, data, emap, nb_subsets=args.subsets, fused=not args.unfused)
+    osem = OSEM(model, data, emap, nb_subsets=args.subsets, fused=not args.unfused,
+                tiled=not args.untiled)
+    plan_s, listings = None, None
+    if osem.tiled:
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        osem.plan(image)
+        torch.cuda.synchronize()
+        plan_s = time.perf_counter() - t0
+        listings = sum(p.info()[0] for ps in osem.plans for p in ps.values() if p is not None)
+        log(f"tile plans for {args.subsets} subsets in {plan_s:.2f}s: "
+            f"{listings / args.events:.1f} (event, tile) listings per event")
 
     # exact non-zero counts per launch (x, y, z group of each subset)
     nnz = {}
@@ -155,10 +174,24 @@ def main():
     total_nnz = sum(nnz.values())
     log(f"nnz per event {total_nnz / args.events:.1f}")
 
-    # per-launch HIP events around each group's fused kernel (torch's current
+    # per-launch HIP events around each group's kernels (torch's current
     # stream is the launch stream)
     timings = {}
     orig = model.osem_subset
+
+    def timed_tiled(img, sub, out=None, plans=None):
+        # the two kernels of srfx_tor_plan_osem_subset, launched one by one
+        for a in "xyz":
+            if len(sub[a]) == 0:
+                continue
+            e0, e1, e2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
+            e0.record()
+            p = model.project_group(img, sub[a], a)
+            e1.record()
+            plans[a].backprojection(p, out=out, accumulate=True)
+            e2.record()
+            timings.setdefault(a, []).append((e0, e1, e2))
+        return Image(out, img.center, img.size)
 
     def timed_osem(img, sub, out=None):
         for a in "xyz":
@@ -176,7 +209,9 @@ def main():
             timings.setdefault(a, []).append((e0, e1, sub[a].soa.data_ptr()))
         return Image(out, img.center, img.size)
 
-    if not args.unfused:
+    if osem.tiled and not args.unfused:
+        model.osem_subset = timed_tiled
+    elif not args.unfused:
         model.osem_subset = timed_osem
     for _ in range(args.warmup):
         osem.iterate(image, 1)
@@ -197,8 +232,35 @@ def main():
     assert torch.isfinite(image.data).all().item()
 
     roof = None
-    if timings:
-        nvox = int(np.prod(grid))
+    kernels = None
+    nvox = int(np.prod(grid))
+    if timings and osem.tiled:
+        # dominant kernel: the slower of the x group's projection and tiled
+        # backprojection (the z group is empty here)
+        a = max(timings, key=lambda k: sum(e0.elapsed_time(e2) for e0, _, e2 in timings[k]))
+        nb = len(timings[a])
+        launches = [(n_launch[(i % args.subsets, a)], nnz[(i % args.subsets, a)])
+                    for i in range(nb)]
+        byts = float(np.mean([n * (LOR_BYTES + 4) + 4 * z + 4 * nvox for n, z in launches]))
+        kernels = {}
+        for name, sel, kname in (("projection", lambda t: t[0].elapsed_time(t[1]),
+                                  "tor_kernel<kProj> (chord walk, batched gather)"),
+                                 ("backprojection", lambda t: t[1].elapsed_time(t[2]),
+                                  "tile_scatter<kBack> (LDS tile accumulation)")):
+            ms = float(np.mean([sel(t) for t in timings[a]]))
+            ach = byts / (ms / 1e3) / 1e9
+            kernels[name] = {"kernel": kname, "group": a, "mean_launch_ms": round(ms, 4),
+                             "algorithmic_bytes_per_launch": int(byts),
+                             "achieved_GBps": round(ach, 1),
+                             "frac": round(ach / HBM_PEAK_GBS, 4)}
+        dom = max(kernels, key=lambda k: kernels[k]["mean_launch_ms"])
+        k = kernels[dom]
+        roof = {"bound": "hbm", "achieved": k["achieved_GBps"], "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": k["frac"], "traffic": None, "kernel": k["kernel"],
+                "group": a, "mean_launch_ms": k["mean_launch_ms"],
+                "algorithmic_bytes_per_launch": k["algorithmic_bytes_per_launch"],
+                "bytes_formula": "sum_i (40 + 4 + 4*nnz_i) + 4*N_vox per op"}
+    elif timings:
         # dominant kernel: the x group's fused pass (the z group is empty here)
         a = max(timings, key=lambda k: sum(e0.elapsed_time(e1) for e0, e1, _ in timings[k]))
         durs = [e0.elapsed_time(e1) for e0, e1, _ in timings[a]]
@@ -237,10 +299,19 @@ def main():
                    "tof_sigma2_mm2": round(meta["tof_sigma2"], 3), "tof_bin_mm": meta["tof_bin"],
                    "nnz_per_event": round(total_nnz / args.events, 2),
                    "fused": not args.unfused, "sorted": bool(args.sort),
+                   "tiled": osem.tiled,
+                   "tile": list(model.tile or DEFAULT_TILE) if osem.tiled else None,
+                   "plan_build_s": round(plan_s, 3) if plan_s is not None else None,
+                   "listings_per_event": (round(listings / args.events, 2)
+                                          if listings is not None else None),
+                   "value_incl_plan_build": (round(args.events * args.steps /
+                                                   (gpu_ms / 1e3 + plan_s) / 1e6, 2)
+                                             if plan_s is not None else None),
                    "schedule": model.schedule["osem_subset"],
                    "autotune_ms": ({str(k): round(v, 3) for k, v in tuned["osem_subset"].items()}
                                    if tuned else None)},
         "roofline": roof,
+        "kernels": kernels,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -46,7 +46,8 @@ enum srfx_status {
     SRFX_OK = 0,
     SRFX_EINVAL = 1,      /* bad argument: null pointer, n < 0, ld < n, grid <= 0, size <= 0 */
     SRFX_ELAUNCH = 2,     /* a HIP call or kernel launch failed */
-    SRFX_EUNSUPPORTED = 3 /* valid but unsupported (e.g. TOR patch wider than 16 voxels) */
+    SRFX_EUNSUPPORTED = 3, /* valid but unsupported (e.g. TOR patch wider than 16 voxels) */
+    SRFX_ENOMEM = 4        /* device workspace allocation failed */
 };
 
 #define SRFX_ACCUMULATE 1u
@@ -66,6 +67,22 @@ enum srfx_status {
 #define SRFX_BLOCK_64 (1u << SRFX_BLOCK_SHIFT)
 #define SRFX_BLOCK_128 (2u << SRFX_BLOCK_SHIFT)
 #define SRFX_BLOCK_256 (3u << SRFX_BLOCK_SHIFT)
+/* Slab schedule of the TOR ops (no counterpart in the reference, which
+ * launches one kernel per slice, tor.cu.cc:377-391): the slices are cut into
+ * contiguous slabs and each event is walked once per slab, the slab's part
+ * only.  SRFX_SLAB_XCD8 puts workgroup b on slab b % 8, so that each of the
+ * chip's 8 XCDs gathers from and scatters into one eighth of the image;
+ * SRFX_SLAB_BLOCKED8 is the same work without that affinity; SRFX_SLAB_XCD16
+ * uses 16 slabs, two per XCD.  Projection and OSEM then take a stream-ordered
+ * workspace of (slabs x n) floats from the device's memory pool.  Lanes per
+ * event is 1 under a slab schedule. */
+#define SRFX_SLAB_SHIFT 8
+#define SRFX_SLAB_XCD8 (1u << SRFX_SLAB_SHIFT)
+#define SRFX_SLAB_BLOCKED8 (2u << SRFX_SLAB_SHIFT)
+#define SRFX_SLAB_XCD16 (3u << SRFX_SLAB_SHIFT)
+/* srfx_tor_plan_osem_subset: compute the projection half with the tiled
+ * gather instead of the untiled kernel (selected by the schedule bits). */
+#define SRFX_TILED_PROJECTION (1u << 10)
 
 typedef struct srfx_grid {
     int32_t shape[3];  /* gx, gy, gz: voxels along the op's x, y, z */
@@ -166,6 +183,58 @@ int srfx_siddon_osem_subset(const float *lors, int64_t n, int64_t ld, const floa
 int srfx_siddon_count_nnz(const float *lors, int64_t n, int64_t ld, srfx_grid grid,
                           float tof_bin, float tof_sigma2, int32_t *nnz_per_event,
                           uint64_t *nnz_total, void *stream);
+
+/* ------------------------------------------------------------- Tiled --- */
+
+/* Tiled TOR ops over a fixed event set (no counterpart in the reference,
+ * whose ops re-test every event against every slice, tor.cu.cc:377-391; they
+ * compute what srfx_tor_project / _backproject / _backproject_adjoint /
+ * _maplors / _osem_subset compute for the same events).  A plan bins each
+ * event into the tiles of `slices` x `wx` x `wy` op-frame voxels it can
+ * touch; scatters then accumulate per tile in LDS and flush each tile with
+ * coalesced atomics, and the tiled projection gathers from tiles staged in
+ * LDS.  `chunk` caps the events one workgroup takes from a tile; `block` is
+ * the workgroup size (64, 128, 256; 0 = 256).  slices * wx * wy <= 16384. */
+typedef struct srfx_tile {
+    int32_t slices, wx, wy, chunk, block;
+} srfx_tile;
+
+typedef struct srfx_tor_plan_s *srfx_tor_plan;
+
+/* Build a plan for n events (ncols = 10: TOF rows as on_tor_lors.py
+ * produces them; ncols = 7: x1..z2 + sigma2 factor, for Maplors) of one
+ * group frame.  Keeps the lors pointer: the events must stay unchanged and
+ * alive while the plan is used.  Allocates the plan's tables and synchronises
+ * `stream`. */
+int srfx_tor_plan_create(const float *lors, int64_t n, int64_t ld, int32_t ncols,
+                         srfx_grid grid, srfx_tor_model model, srfx_tile tile,
+                         srfx_tor_plan *plan, void *stream);
+
+/* Listings (event, tile pairs) and workgroup chunks of a plan. */
+int srfx_tor_plan_info(srfx_tor_plan plan, int64_t *nlistings, int32_t *nchunks);
+
+/* Synchronises the device, then frees the plan. */
+int srfx_tor_plan_destroy(srfx_tor_plan plan);
+
+/* Projection (tor.cc:16-28) of the plan's events; deterministic. */
+int srfx_tor_plan_project(srfx_tor_plan plan, const float *image, float *proj, uint32_t flags,
+                          void *stream);
+
+/* Backprojection (tor.cc:30-47), adjoint, Maplors (tor.cc:49-64; ncols 7
+ * plans) into an image of the plan's grid. */
+int srfx_tor_plan_backproject(srfx_tor_plan plan, const float *lors_value, float *image,
+                              uint32_t flags, void *stream);
+int srfx_tor_plan_backproject_adjoint(srfx_tor_plan plan, const float *g, float *image,
+                                      uint32_t flags, void *stream);
+int srfx_tor_plan_maplors(srfx_tor_plan plan, const float *lors_value, float *image,
+                          uint32_t flags, void *stream);
+
+/* OSEM subset step of the plan's events, as srfx_tor_osem_subset: the
+ * projection (untiled kernel with the flags' schedule, or tiled with
+ * SRFX_TILED_PROJECTION) into a plan buffer, then the tiled backprojection
+ * of 1/p into bp.  Calls on one plan must not overlap. */
+int srfx_tor_plan_osem_subset(srfx_tor_plan plan, const float *image, srfx_grid bp_grid,
+                              float *bp, uint32_t flags, void *stream);
 
 /* ------------------------------------------------------------ Batched --- */
 

@@ -14,23 +14,34 @@ SRFX_OK = 0
 SRFX_EINVAL = 1
 SRFX_ELAUNCH = 2
 SRFX_EUNSUPPORTED = 3
+SRFX_ENOMEM = 4
 SRFX_ACCUMULATE = 1
 SRFX_FULL_PATCH = 2
 SRFX_LPE_SHIFT = 4
 SRFX_BLOCK_SHIFT = 6
 _LPE_CODE = {1: 1, 4: 2, 16: 3}
 _BLOCK_CODE = {64: 1, 128: 2, 256: 3}
+SRFX_SLAB_SHIFT = 8
+SRFX_TILED_PROJECTION = 1 << 10
+_SLAB_CODE = {None: 0, "xcd8": 1, "blocked8": 2, "xcd16": 3}
 
 
 def schedule_flags(schedule):
-    """Flags for a TOR launch schedule (lanes_per_event, block) or None."""
+    """Flags for a TOR launch schedule or None: (lanes_per_event, block) or
+    (lanes_per_event, block, slabs) with slabs one of None, "xcd8",
+    "blocked8", "xcd16" (slab schedules take lanes_per_event 1)."""
     if schedule is None:
         return 0
-    lpe, block = schedule
-    if lpe not in _LPE_CODE or block not in _BLOCK_CODE:
+    lpe, block = schedule[0], schedule[1]
+    slab = schedule[2] if len(schedule) > 2 else None
+    if lpe not in _LPE_CODE or block not in _BLOCK_CODE or slab not in _SLAB_CODE:
         raise ValueError(f"unsupported schedule {schedule}: lanes per event in (1, 4, 16), "
-                         "block in (64, 128, 256)")
-    return (_LPE_CODE[lpe] << SRFX_LPE_SHIFT) | (_BLOCK_CODE[block] << SRFX_BLOCK_SHIFT)
+                         "block in (64, 128, 256), slabs in (None, 'xcd8', 'blocked8', "
+                         "'xcd16')")
+    if slab is not None and lpe != 1:
+        raise ValueError("slab schedules run one lane per event")
+    return ((_LPE_CODE[lpe] << SRFX_LPE_SHIFT) | (_BLOCK_CODE[block] << SRFX_BLOCK_SHIFT)
+            | (_SLAB_CODE[slab] << SRFX_SLAB_SHIFT))
 
 
 class SrfxGrid(ctypes.Structure):
@@ -44,6 +55,13 @@ class SrfxGrid(ctypes.Structure):
 class SrfxTorModel(ctypes.Structure):
     _fields_ = [("kernel_width", ctypes.c_float), ("geo_sigma2_flag", ctypes.c_int32),
                 ("tof_bin", ctypes.c_float), ("tof_sigma2", ctypes.c_float)]
+
+
+class SrfxTile(ctypes.Structure):
+    """srfx_tile: tile of `slices` x `wx` x `wy` op-frame voxels, at most
+    `chunk` events per workgroup, `block` threads per workgroup."""
+    _fields_ = [("slices", ctypes.c_int32), ("wx", ctypes.c_int32), ("wy", ctypes.c_int32),
+                ("chunk", ctypes.c_int32), ("block", ctypes.c_int32)]
 
 
 class SrfxError(RuntimeError):
@@ -73,6 +91,15 @@ _PROTOS = {
     "srfx_siddon_osem_subset": [_P, _I64, _I64, _P, SrfxGrid, _F, _F, _P, _U32, _P],
     "srfx_siddon_count_nnz": [_P, _I64, _I64, SrfxGrid, _F, _F, _P, _P, _P],
     "srfx_mlem_update": [_P, _P, _P, _I64, _P],
+    "srfx_tor_plan_create": [_P, _I64, _I64, _I32, SrfxGrid, SrfxTorModel, SrfxTile,
+                             ctypes.POINTER(ctypes.c_void_p), _P],
+    "srfx_tor_plan_info": [_P, ctypes.POINTER(_I64), ctypes.POINTER(_I32)],
+    "srfx_tor_plan_destroy": [_P],
+    "srfx_tor_plan_project": [_P, _P, _P, _U32, _P],
+    "srfx_tor_plan_backproject": [_P, _P, _P, _U32, _P],
+    "srfx_tor_plan_backproject_adjoint": [_P, _P, _P, _U32, _P],
+    "srfx_tor_plan_maplors": [_P, _P, _P, _U32, _P],
+    "srfx_tor_plan_osem_subset": [_P, _P, SrfxGrid, _P, _U32, _P],
 }
 
 _lib = None

@@ -72,7 +72,7 @@ class BatchedOSEM:
             for k, a in enumerate("xyz"):
                 soa = rc["data"][a].soa
                 _check_dev(soa, "lors")
-                if soa.dim() != 2 or soa.shape[0] < 10 or soa.stride(1) != 1:
+                if soa.dim() != 2 or soa.shape[0] < 10 or (soa.shape[1] and soa.stride(1) != 1):
                     raise ValueError("group LORs must be [10, N] row-contiguous")
                 r.lors[k] = soa.data_ptr() if soa.shape[1] else None
                 r.n[k] = soa.shape[1]

@@ -74,4 +74,32 @@ inline unsigned blocks_for(int64_t n, int threads)
     return (unsigned)(b < 1 ? 1 : b);
 }
 
+// Slab schedule field of the flags: (number of slabs, interleaved over XCDs).
+inline bool slab_schedule(uint32_t flags, int &nslab, int &interleave)
+{
+    switch ((flags >> SRFX_SLAB_SHIFT) & 3u) {
+    case 1:
+        nslab = 8;
+        interleave = 1;
+        return true;
+    case 2:
+        nslab = 8;
+        interleave = 0;
+        return true;
+    case 3:
+        nslab = 16;
+        interleave = 1;
+        return true;
+    default:
+        nslab = 1;
+        interleave = 0;
+        return false;
+    }
+}
+
+// Stream-ordered device workspace: allocated from the device's default memory
+// pool (kept by the pool between calls) and released on the same stream.
+int ws_alloc(void **ptr, size_t bytes, hipStream_t st);
+int ws_free(void *ptr, hipStream_t st);
+
 } // namespace srfx

@@ -1,0 +1,658 @@
+// Tiled TOR ops: a per-event-set plan that bins every event into the image
+// tiles it can touch, then gathers from / scatters into tiles staged in LDS.
+//
+// A tile is a slab of S consecutive slices times a window of Wx x Wy voxels
+// of the op frame.  Along a TOR group's dominant axis an event moves by at
+// most one voxel pitch per slice, so within one slab its voxels stay inside a
+// small footprint and it touches few tiles.  The plan lists, for every tile,
+// the events whose footprint meets it ("listings"), cut into chunks of at most
+// `chunk` listings; one workgroup takes one chunk:
+//   scatter: zero the tile in LDS, every lane walks one listing's voxels that
+//            lie in the tile and adds its weights with LDS atomics, then the
+//            tile is flushed with one coalesced global atomic per non-zero
+//            voxel (instead of one scattered global atomic per weight);
+//   gather:  load the tile into LDS, every lane sums its listing's weighted
+//            voxels from LDS into one partial per listing; a second pass sums
+//            each event's partials in the event's own listing order, so the
+//            projection is deterministic.
+// The voxels an event visits and their weights are those of the untiled
+// kernels (tor_core.h): the tile walk is the chord walk clipped to the window.
+#include <rocprim/device/device_scan.hpp>
+
+#include <vector>
+
+#include "tor_core.h"
+
+namespace srfx {
+namespace {
+
+struct TileGeom {
+    int S, Wx, Wy;       // slices per slab, window width along op x and y (voxels)
+    int nwx, nwy, nslab; // windows per axis and number of slabs
+};
+
+struct TileChunk {
+    int tile, start, count, pad;
+};
+
+__device__ __forceinline__ int trunc_clamped(double u)
+{
+    return (int)fmin(fmax(u, -1.0e9), 1.0e9);
+}
+
+// Windows wx0..wx1, wy0..wy1 holding every voxel the event can visit in the
+// slices s0..s1.  The patch origin of slice s is (int)u(s) - half with u
+// affine in s (cross_at), or the reference's exact origin, which differs from
+// it by at most one voxel; the patch spans P voxels from its origin.
+__device__ __forceinline__ bool footprint(const TorParams &p, const TileGeom &g, const Track &t,
+                                          int s0, int s1, int &wx0, int &wx1, int &wy0, int &wy1)
+{
+    int a0 = trunc_clamped(t.ax + (double)s0 * t.bx) - p.half;
+    int a1 = trunc_clamped(t.ax + (double)s1 * t.bx) - p.half;
+    int b0 = trunc_clamped(t.ay + (double)s0 * t.by) - p.half;
+    int b1 = trunc_clamped(t.ay + (double)s1 * t.by) - p.half;
+    int x0 = max(min(a0, a1) - 1, 0), x1 = min(max(a0, a1) + p.patch, p.gx - 1);
+    int y0 = max(min(b0, b1) - 1, 0), y1 = min(max(b0, b1) + p.patch, p.gy - 1);
+    if (x0 > x1 || y0 > y1)
+        return false;
+    wx0 = x0 / g.Wx;
+    wx1 = x1 / g.Wx;
+    wy0 = y0 / g.Wy;
+    wy1 = y1 / g.Wy;
+    return true;
+}
+
+// Calls f(tile) for every tile the event can touch: slabs in slice order,
+// then windows row by row.  The count and fill passes enumerate identically.
+template <class F>
+__device__ __forceinline__ void for_each_tile(const TorParams &p, const TileGeom &g,
+                                              const Event &ev, const Track &t, F f)
+{
+    const int sl0 = ev.s_lo / g.S, sl1 = ev.s_hi / g.S;
+    for (int sl = sl0; sl <= sl1; sl++) {
+        int s0 = max(ev.s_lo, sl * g.S), s1 = min(ev.s_hi, sl * g.S + g.S - 1);
+        int wx0, wx1, wy0, wy1;
+        if (!footprint(p, g, t, s0, s1, wx0, wx1, wy0, wy1))
+            continue;
+        for (int wy = wy0; wy <= wy1; wy++)
+            for (int wx = wx0; wx <= wx1; wx++)
+                f((sl * g.nwy + wy) * g.nwx + wx);
+    }
+}
+
+__device__ __forceinline__ bool plan_event(const TorParams &p, int cols7, int64_t e, Event &ev)
+{
+    return cols7 ? load_event<kMap>(p, e, ev) : load_event<kProj>(p, e, ev);
+}
+
+// Count pass (FILL false): ev_cnt[e] = number of tiles of event e and
+// tile_cnt[t] += 1 per listing.  Fill pass (FILL true): list the event in
+// each of its tiles and record the slots in its own listing order.  Both
+// passes run the same code; the fill pass still checks every slot against
+// the counted ranges and reports a mismatch in *err instead of writing out of
+// range.
+template <bool FILL>
+__global__ __launch_bounds__(256) void plan_pass(TorParams p, TileGeom g, int cols7, int *ev_cnt,
+                                                 int *tile_cnt, const int *ev_off,
+                                                 const int *tile_off, int *tile_fill, int *list,
+                                                 int *ev_slot, int *err)
+{
+    int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= p.n)
+        return;
+    Event ev;
+    int k = 0;
+    const int k0 = FILL ? ev_off[e] : 0;
+    const int k1 = FILL ? ev_off[e + 1] : 0;
+    if (plan_event(p, cols7, e, ev)) {
+        Track t;
+        make_track(p, ev, t);
+        for_each_tile(p, g, ev, t, [&](int tile) {
+            if (!FILL) {
+                atomicAdd(&tile_cnt[tile], 1);
+            } else {
+                int r = atomicAdd(&tile_fill[tile], 1);
+                if (r < tile_cnt[tile] && k0 + k < k1) {
+                    int slot = tile_off[tile] + r;
+                    list[slot] = (int)e;
+                    ev_slot[k0 + k] = slot;
+                } else {
+                    atomicAdd(err, 1);
+                }
+            }
+            k++;
+        });
+    }
+    if (!FILL)
+        ev_cnt[e] = k;
+    else if (k0 + k != k1)
+        atomicAdd(err, 1);
+}
+
+__global__ __launch_bounds__(256) void fill_int(int *v, int64_t n, int value)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        v[i] = value;
+}
+
+// Tile bounds of a chunk: slices sa..sb, op-frame voxels x0..x1, y0..y1.
+struct TileBox {
+    int sa, sb, x0, x1, y0, y1;
+};
+
+__device__ __forceinline__ TileBox tile_box(const TorParams &p, const TileGeom &g, int tile)
+{
+    TileBox b;
+    const int per = g.nwx * g.nwy;
+    const int sl = tile / per, r = tile - sl * per;
+    const int wy = r / g.nwx, wx = r - wy * g.nwx;
+    b.sa = sl * g.S;
+    b.sb = min(b.sa + g.S, p.gz) - 1;
+    b.x0 = wx * g.Wx;
+    b.x1 = min(b.x0 + g.Wx, p.gx) - 1;
+    b.y0 = wy * g.Wy;
+    b.y1 = min(b.y0 + g.Wy, p.gy) - 1;
+    return b;
+}
+
+// The chord walk of one slice (patch_walk) clipped to the tile's window.
+//   KIND 0: returns sum w * lds[voxel]; KIND 1: lds[voxel] += w * scale.
+template <int KIND>
+__device__ __forceinline__ float window_walk(const TorParams &p, const TileGeom &g,
+                                             const TileBox &b, const Event &ev, const Track &t,
+                                             const Cross &c, float amp, int s, float scale,
+                                             float *lds)
+{
+    const int P = p.patch;
+    int i_lo = max(0, b.x0 - c.ix0), i_hi = min(P - 1, b.x1 - c.ix0);
+    int j_lo = max(0, b.y0 - c.iy0), j_hi = min(P - 1, b.y1 - c.iy0);
+    const float eps = 1.0e-4f;
+    j_lo = max(j_lo, (int)ceilf((c.dy0 - t.dymax) * p.rfinter_y - eps));
+    j_hi = min(j_hi, (int)floorf((c.dy0 + t.dymax) * p.rfinter_y + eps));
+    const int row0 = ((s - b.sa) * g.Wy + (c.iy0 - b.y0)) * g.Wx + (c.ix0 - b.x0);
+    float acc = 0.0f;
+    for (int j = j_lo; j <= j_hi; j++) {
+        float dy = voxel_offset(c.dy0, j, p.finter_y);
+        float tt = t.tm - dy * dy * t.q;
+        if (tt <= 0.0f)
+            continue;
+        float hw = __builtin_amdgcn_sqrtf(tt * t.A);
+        float ctr = c.dx0 - t.cxy * dy;
+        int il = max(i_lo, (int)ceilf((ctr - hw) * p.rfinter_x - eps));
+        int ih = min(i_hi, (int)floorf((ctr + hw) * p.rfinter_x + eps));
+        for (int i = il; i <= ih; i++) {
+            float dx = voxel_offset(c.dx0, i, p.finter_x);
+            float d2 = tube_dist2(dx, dy, ev.ux, ev.uy);
+            bool in = d2 < ev.thr - kThrMargin;
+            if (!in && d2 < ev.thr + kThrMargin)
+                in = inside_exact(p, ev, s, c.ix0 + i, c.iy0 + j);
+            if (in) {
+                float w = amp * __builtin_amdgcn_exp2f(d2 * ev.cw);
+                int li = row0 + j * g.Wx + i;
+                if (KIND == 0)
+                    acc = fmaf(w, lds[li], acc);
+                else
+                    atomicAdd(&lds[li], w * scale);
+            }
+        }
+    }
+    return acc;
+}
+
+template <int KIND>
+__device__ __forceinline__ float tile_event(const TorParams &p, const TileGeom &g,
+                                            const TileBox &b, const Event &ev, bool tof,
+                                            float scale, float *lds)
+{
+    Track t;
+    make_track(p, ev, t);
+    const int s0 = max(ev.s_lo, b.sa), s1 = min(ev.s_hi, b.sb);
+    float acc = 0.0f;
+    for (int s = s0; s <= s1; s++) {
+        Cross c;
+        cross_at(p, ev, t, s, c);
+        float amp = tof ? tof_factor(p, ev, t, s) : 1.0f;
+        acc += window_walk<KIND>(p, g, b, ev, t, c, amp, s, scale, lds);
+    }
+    return acc;
+}
+
+// Scatter of one chunk (modes kBack, kMap, kAdj) into p.out.
+template <int MODE>
+__global__ __launch_bounds__(256) void tile_scatter(TorParams p, TileGeom g,
+                                                    const TileChunk *__restrict__ chunks,
+                                                    const int *__restrict__ list)
+{
+    extern __shared__ float lds[];
+    const TileChunk ch = chunks[blockIdx.x];
+    const TileBox b = tile_box(p, g, ch.tile);
+    const int nv = g.S * g.Wy * g.Wx;
+    for (int i = threadIdx.x; i < nv; i += blockDim.x)
+        lds[i] = 0.0f;
+    __syncthreads();
+    const bool tof = MODE != kMap && p.use_tof;
+    for (int j = threadIdx.x; j < ch.count; j += blockDim.x) {
+        const int64_t e = list[ch.start + j];
+        float scale;
+        bool go = true;
+        if (MODE == kBack) {
+            float v = p.val[e];
+            go = v > 1e-7;
+            scale = 1.0f / v;
+        } else if (MODE == kMap) {
+            scale = 1.0f / p.val[e];
+        } else {
+            scale = p.val[e];
+            go = scale != 0.0f;
+        }
+        Event ev;
+        if (go && load_event<MODE>(p, e, ev))
+            tile_event<1>(p, g, b, ev, tof, scale, lds);
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < nv; i += blockDim.x) {
+        float v = lds[i];
+        if (v != 0.0f) {
+            int lx = i % g.Wx, r = i / g.Wx;
+            int ly = r % g.Wy, ls = r / g.Wy;
+            int ix = b.x0 + lx, iy = b.y0 + ly, s = b.sa + ls;
+            if (ix <= b.x1 && iy <= b.y1 && s <= b.sb)
+                unsafeAtomicAdd(p.out + (ix * p.ox + iy * p.oy + s * p.oz), v);
+        }
+    }
+}
+
+// Gather of one chunk: partial[slot] = the listing's projection over the tile.
+__global__ __launch_bounds__(256) void tile_gather(TorParams p, TileGeom g,
+                                                   const TileChunk *__restrict__ chunks,
+                                                   const int *__restrict__ list, float *partial)
+{
+    extern __shared__ float lds[];
+    const TileChunk ch = chunks[blockIdx.x];
+    const TileBox b = tile_box(p, g, ch.tile);
+    const int nv = g.S * g.Wy * g.Wx;
+    for (int i = threadIdx.x; i < nv; i += blockDim.x) {
+        int lx = i % g.Wx, r = i / g.Wx;
+        int ly = r % g.Wy, ls = r / g.Wy;
+        int ix = b.x0 + lx, iy = b.y0 + ly, s = b.sa + ls;
+        float v = 0.0f;
+        if (ix <= b.x1 && iy <= b.y1 && s <= b.sb)
+            v = p.img[ix * p.sx + iy * p.sy + s * p.sz];
+        lds[i] = v;
+    }
+    __syncthreads();
+    const bool tof = p.use_tof;
+    for (int j = threadIdx.x; j < ch.count; j += blockDim.x) {
+        const int64_t e = list[ch.start + j];
+        Event ev;
+        float acc = 0.0f;
+        if (load_event<kProj>(p, e, ev))
+            acc = tile_event<0>(p, g, b, ev, tof, 0.0f, lds);
+        partial[ch.start + j] = acc;
+    }
+}
+
+// proj[e] = sum of the event's partials in its listing order.
+__global__ __launch_bounds__(256) void tile_reduce(const float *__restrict__ partial,
+                                                   const int *__restrict__ ev_off,
+                                                   const int *__restrict__ ev_slot, int64_t n,
+                                                   float *proj)
+{
+    int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n)
+        return;
+    float v = 0.0f;
+    for (int k = ev_off[e]; k < ev_off[e + 1]; k++)
+        v += partial[ev_slot[k]];
+    proj[e] = v;
+}
+
+} // namespace
+} // namespace srfx
+
+using namespace srfx;
+
+struct srfx_tor_plan_s {
+    TorParams p;  // geometry, model and events; image / value pointers set per call
+    srfx_grid grid;
+    TileGeom g;
+    int cols7;
+    int64_t nlist;
+    int nchunks;
+    int block;
+    int *list, *ev_slot, *ev_off;
+    TileChunk *chunks;
+    float *partial, *proj;
+};
+
+namespace {
+
+template <class T> int dev_alloc(T **ptr, size_t count)
+{
+    *ptr = nullptr;
+    if (count == 0)
+        count = 1;
+    if (hipMalloc((void **)ptr, sizeof(T) * count) != hipSuccess) {
+        (void)hipGetLastError();
+        *ptr = nullptr;
+        return SRFX_ENOMEM;
+    }
+    return SRFX_OK;
+}
+
+void plan_free(srfx_tor_plan_s *pl)
+{
+    if (!pl)
+        return;
+    (void)hipFree(pl->list);
+    (void)hipFree(pl->ev_slot);
+    (void)hipFree(pl->ev_off);
+    (void)hipFree(pl->chunks);
+    (void)hipFree(pl->partial);
+    (void)hipFree(pl->proj);
+    delete pl;
+}
+
+unsigned lds_bytes(const TileGeom &g)
+{
+    return (unsigned)(sizeof(float) * g.S * g.Wx * g.Wy);
+}
+
+int plan_build(srfx_tor_plan_s *pl, hipStream_t st, int chunk)
+{
+    TorParams &p = pl->p;
+    const TileGeom &g = pl->g;
+    const int64_t n = p.n;
+    const int ntiles = g.nslab * g.nwx * g.nwy;
+    int *tile_cnt = nullptr, *ev_cnt = nullptr, *tile_off_d = nullptr;
+    void *tmp = nullptr;
+    int rc = SRFX_OK;
+    std::vector<int> cnt(ntiles), off(ntiles);
+    std::vector<TileChunk> ch;
+    int64_t total = 0;
+    size_t tmp_bytes = 0;
+    int last_cnt = 0, last_off = 0, mismatch = 0;
+    if ((rc = dev_alloc(&tile_cnt, 2 * (size_t)ntiles + 1)) || (rc = dev_alloc(&ev_cnt, n)) ||
+        (rc = dev_alloc(&pl->ev_off, n + 1)) || (rc = dev_alloc(&tile_off_d, ntiles)))
+        goto done;
+    hipLaunchKernelGGL(fill_int, dim3(blocks_for(2 * (int64_t)ntiles + 1, 256)), dim3(256), 0,
+                       st, tile_cnt, 2 * (int64_t)ntiles + 1, 0);
+    if (n > 0)
+        hipLaunchKernelGGL(plan_pass<false>, dim3(blocks_for(n, 256)), dim3(256), 0, st, p, g,
+                           pl->cols7, ev_cnt, tile_cnt, nullptr, nullptr, nullptr, nullptr,
+                           nullptr, nullptr);
+    if ((rc = launch_status()))
+        goto done;
+    // ev_off = exclusive scan of the per-event listing counts
+    if (n > 0) {
+        if (rocprim::exclusive_scan(nullptr, tmp_bytes, ev_cnt, pl->ev_off, 0, (size_t)n,
+                                    rocprim::plus<int>(), st) != hipSuccess) {
+            rc = SRFX_ELAUNCH;
+            goto done;
+        }
+        if ((rc = dev_alloc((char **)&tmp, tmp_bytes)))
+            goto done;
+        if (rocprim::exclusive_scan(tmp, tmp_bytes, ev_cnt, pl->ev_off, 0, (size_t)n,
+                                    rocprim::plus<int>(), st) != hipSuccess) {
+            rc = SRFX_ELAUNCH;
+            goto done;
+        }
+    }
+    if (hipMemcpyAsync(cnt.data(), tile_cnt, sizeof(int) * ntiles, hipMemcpyDeviceToHost, st) !=
+            hipSuccess ||
+        (n > 0 && (hipMemcpyAsync(&last_cnt, ev_cnt + (n - 1), sizeof(int),
+                                  hipMemcpyDeviceToHost, st) != hipSuccess ||
+                   hipMemcpyAsync(&last_off, pl->ev_off + (n - 1), sizeof(int),
+                                  hipMemcpyDeviceToHost, st) != hipSuccess)) ||
+        hipStreamSynchronize(st) != hipSuccess) {
+        rc = SRFX_ELAUNCH;
+        goto done;
+    }
+    for (int t = 0; t < ntiles; t++) {
+        off[t] = (int)total;
+        total += cnt[t];
+    }
+    if (total > (int64_t)INT32_MAX - 1 || (n > 0 && (int64_t)last_off + last_cnt != total)) {
+        rc = SRFX_EUNSUPPORTED;
+        goto done;
+    }
+    {
+        int total32 = (int)total;
+        if (hipMemcpyAsync(pl->ev_off + n, &total32, sizeof(int), hipMemcpyHostToDevice, st) !=
+            hipSuccess) {
+            rc = SRFX_ELAUNCH;
+            goto done;
+        }
+        // the copy reads a host stack variable: finish it before leaving scope
+        if (hipStreamSynchronize(st) != hipSuccess) {
+            rc = SRFX_ELAUNCH;
+            goto done;
+        }
+    }
+    for (int t = 0; t < ntiles; t++)
+        for (int s = 0; s < cnt[t]; s += chunk)
+            ch.push_back(TileChunk{t, off[t] + s, cnt[t] - s < chunk ? cnt[t] - s : chunk, 0});
+    pl->nlist = total;
+    pl->nchunks = (int)ch.size();
+    if ((rc = dev_alloc(&pl->list, total)) || (rc = dev_alloc(&pl->ev_slot, total)) ||
+        (rc = dev_alloc(&pl->partial, total)) || (rc = dev_alloc(&pl->proj, n)) ||
+        (rc = dev_alloc(&pl->chunks, ch.size())))
+        goto done;
+    if (hipMemcpyAsync(tile_off_d, off.data(), sizeof(int) * ntiles, hipMemcpyHostToDevice, st) !=
+            hipSuccess ||
+        (!ch.empty() && hipMemcpyAsync(pl->chunks, ch.data(), sizeof(TileChunk) * ch.size(),
+                                       hipMemcpyHostToDevice, st) != hipSuccess)) {
+        rc = SRFX_ELAUNCH;
+        goto done;
+    }
+    if (n > 0)
+        hipLaunchKernelGGL(plan_pass<true>, dim3(blocks_for(n, 256)), dim3(256), 0, st, p, g,
+                           pl->cols7, nullptr, tile_cnt, pl->ev_off, tile_off_d,
+                           tile_cnt + ntiles, pl->list, pl->ev_slot, tile_cnt + 2 * ntiles);
+    if ((rc = launch_status()))
+        goto done;
+    if (hipMemcpyAsync(&mismatch, tile_cnt + 2 * ntiles, sizeof(int), hipMemcpyDeviceToHost,
+                       st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess) {
+        rc = SRFX_ELAUNCH;
+        goto done;
+    }
+    if (mismatch)
+        rc = SRFX_ELAUNCH; // count and fill passes disagreed: no valid plan
+done:
+    (void)hipStreamSynchronize(st);
+    (void)hipFree(tile_cnt);
+    (void)hipFree(ev_cnt);
+    (void)hipFree(tile_off_d);
+    (void)hipFree(tmp);
+    return rc;
+}
+
+int plan_scatter(srfx_tor_plan_s *pl, int mode, const float *vals, float *image,
+                 const srfx_grid *out_grid, uint32_t flags, hipStream_t st)
+{
+    TorParams p = pl->p;
+    p.val = vals;
+    p.out = image;
+    if (out_grid) {
+        Strides so = grid_strides(*out_grid);
+        p.ox = (int)so.x;
+        p.oy = (int)so.y;
+        p.oz = (int)so.z;
+    }
+    if (pl->nchunks == 0)
+        return SRFX_OK;
+    const dim3 grid(pl->nchunks), block(pl->block);
+    const unsigned sh = lds_bytes(pl->g);
+    switch (mode) {
+    case kBack:
+        hipLaunchKernelGGL(tile_scatter<kBack>, grid, block, sh, st, p, pl->g, pl->chunks,
+                           pl->list);
+        break;
+    case kAdj:
+        hipLaunchKernelGGL(tile_scatter<kAdj>, grid, block, sh, st, p, pl->g, pl->chunks,
+                           pl->list);
+        break;
+    default:
+        hipLaunchKernelGGL(tile_scatter<kMap>, grid, block, sh, st, p, pl->g, pl->chunks,
+                           pl->list);
+        break;
+    }
+    (void)flags;
+    return launch_status();
+}
+
+int plan_project(srfx_tor_plan_s *pl, const float *image, float *proj, hipStream_t st)
+{
+    TorParams p = pl->p;
+    p.img = image;
+    if (p.n == 0)
+        return SRFX_OK;
+    if (pl->nchunks > 0)
+        hipLaunchKernelGGL(tile_gather, dim3(pl->nchunks), dim3(pl->block), lds_bytes(pl->g), st,
+                           p, pl->g, pl->chunks, pl->list, pl->partial);
+    hipLaunchKernelGGL(tile_reduce, dim3(blocks_for(p.n, 256)), dim3(256), 0, st, pl->partial,
+                       pl->ev_off, pl->ev_slot, p.n, proj);
+    return launch_status();
+}
+
+} // namespace
+
+extern "C" int srfx_tor_plan_create(const float *lors, int64_t n, int64_t ld, int32_t ncols,
+                                    srfx_grid grid, srfx_tor_model m, srfx_tile tile,
+                                    srfx_tor_plan *out, void *stream)
+{
+    if (!out)
+        return SRFX_EINVAL;
+    *out = nullptr;
+    if (ncols != 7 && ncols != 10)
+        return SRFX_EINVAL;
+    if (n > (int64_t)INT32_MAX - 1)
+        return SRFX_EUNSUPPORTED;
+    if (tile.slices < 1 || tile.wx < 1 || tile.wy < 1 || tile.chunk < 1)
+        return SRFX_EINVAL;
+    if ((int64_t)tile.slices * tile.wx * tile.wy > 16384)
+        return SRFX_EUNSUPPORTED; // LDS tile above 64 KB
+    if (tile.block != 0 && tile.block != 64 && tile.block != 128 && tile.block != 256)
+        return SRFX_EINVAL;
+    srfx_tor_plan_s *pl = new srfx_tor_plan_s{};
+    srfx_tor_model mm = ncols == 7 ? model_no_tof(m.kernel_width, m.geo_sigma2_flag) : m;
+    int rc = setup(pl->p, lors, n, ld, grid, mm.kernel_width, mm.geo_sigma2_flag, mm.tof_bin,
+                   mm.tof_sigma2);
+    if (rc) {
+        delete pl;
+        return rc;
+    }
+    pl->grid = grid;
+    pl->cols7 = ncols == 7;
+    pl->block = tile.block ? tile.block : 256;
+    TileGeom &g = pl->g;
+    g.S = tile.slices;
+    g.Wx = tile.wx;
+    g.Wy = tile.wy;
+    g.nwx = (pl->p.gx + g.Wx - 1) / g.Wx;
+    g.nwy = (pl->p.gy + g.Wy - 1) / g.Wy;
+    g.nslab = (pl->p.gz + g.S - 1) / g.S;
+    if ((int64_t)g.nwx * g.nwy * g.nslab > (int64_t)INT32_MAX / 2) {
+        delete pl;
+        return SRFX_EUNSUPPORTED;
+    }
+    rc = plan_build(pl, (hipStream_t)stream, tile.chunk);
+    if (rc) {
+        plan_free(pl);
+        return rc;
+    }
+    *out = pl;
+    return SRFX_OK;
+}
+
+extern "C" int srfx_tor_plan_info(srfx_tor_plan plan, int64_t *nlistings, int32_t *nchunks)
+{
+    if (!plan)
+        return SRFX_EINVAL;
+    if (nlistings)
+        *nlistings = plan->nlist;
+    if (nchunks)
+        *nchunks = plan->nchunks;
+    return SRFX_OK;
+}
+
+extern "C" int srfx_tor_plan_destroy(srfx_tor_plan plan)
+{
+    if (plan)
+        (void)hipDeviceSynchronize();
+    plan_free(plan);
+    return SRFX_OK;
+}
+
+extern "C" int srfx_tor_plan_project(srfx_tor_plan plan, const float *image, float *proj,
+                                     uint32_t flags, void *stream)
+{
+    if (!plan || plan->cols7 || (plan->p.n > 0 && (!image || !proj)))
+        return SRFX_EINVAL;
+    (void)flags;
+    return plan_project(plan, image, proj, (hipStream_t)stream);
+}
+
+static int plan_scatter_entry(srfx_tor_plan plan, int mode, const float *vals, float *image,
+                              uint32_t flags, void *stream)
+{
+    if (!plan || !image || (plan->p.n > 0 && !vals))
+        return SRFX_EINVAL;
+    if ((mode == kMap) != (plan->cols7 != 0))
+        return SRFX_EINVAL;
+    hipStream_t st = (hipStream_t)stream;
+    int rc = zero_grid(image, plan->grid, flags, st);
+    if (rc)
+        return rc;
+    return plan_scatter(plan, mode, vals, image, nullptr, flags, st);
+}
+
+extern "C" int srfx_tor_plan_backproject(srfx_tor_plan plan, const float *lors_value,
+                                         float *image, uint32_t flags, void *stream)
+{
+    return plan_scatter_entry(plan, kBack, lors_value, image, flags, stream);
+}
+
+extern "C" int srfx_tor_plan_backproject_adjoint(srfx_tor_plan plan, const float *g,
+                                                 float *image, uint32_t flags, void *stream)
+{
+    return plan_scatter_entry(plan, kAdj, g, image, flags, stream);
+}
+
+extern "C" int srfx_tor_plan_maplors(srfx_tor_plan plan, const float *lors_value, float *image,
+                                     uint32_t flags, void *stream)
+{
+    return plan_scatter_entry(plan, kMap, lors_value, image, flags, stream);
+}
+
+extern "C" int srfx_tor_plan_osem_subset(srfx_tor_plan plan, const float *image,
+                                         srfx_grid bp_grid, float *bp, uint32_t flags,
+                                         void *stream)
+{
+    if (!plan || plan->cols7 || !bp || (plan->p.n > 0 && !image) || !grid_ok(bp_grid))
+        return SRFX_EINVAL;
+    const TorParams &p = plan->p;
+    if (bp_grid.shape[0] != p.gx || bp_grid.shape[1] != p.gy || bp_grid.shape[2] != p.gz)
+        return SRFX_EINVAL;
+    hipStream_t st = (hipStream_t)stream;
+    int rc = zero_grid(bp, bp_grid, flags, st);
+    if (rc)
+        return rc;
+    if (p.n == 0)
+        return SRFX_OK;
+    // Projection with the chosen schedule (untiled by default), then the
+    // tiled scatter of 1 / p over events with p > 1e-7.
+    if (flags & SRFX_TILED_PROJECTION) {
+        rc = plan_project(plan, image, plan->proj, st);
+    } else {
+        TorParams q = p;
+        q.img = image;
+        q.out = plan->proj;
+        rc = launch<kProj>(q, flags, st);
+    }
+    if (rc)
+        return rc;
+    return plan_scatter(plan, kBack, plan->proj, bp, &bp_grid, flags, st);
+}

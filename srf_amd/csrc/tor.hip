@@ -4,6 +4,21 @@
 
 using namespace srfx;
 
+// Slab projection: per-slab partial sums in a workspace, then their sum in
+// slab order into p.out.
+static int slab_project(const TorParams &p, int nslab, uint32_t flags, hipStream_t st)
+{
+    void *ws = nullptr;
+    int rc = ws_alloc(&ws, sizeof(float) * (size_t)nslab * (size_t)p.n, st);
+    if (rc)
+        return rc;
+    rc = launch_slab<kProj>(p, (float *)ws, flags, st);
+    if (!rc)
+        rc = launch_slab_sum((const float *)ws, nslab, p.n, p.out, st);
+    int rf = ws_free(ws, st);
+    return rc ? rc : rf;
+}
+
 extern "C" int srfx_tor_project(const float *lors, int64_t n, int64_t ld, const float *image,
                                 srfx_grid grid, srfx_tor_model m, float *proj, uint32_t flags,
                                 void *stream)
@@ -18,7 +33,11 @@ extern "C" int srfx_tor_project(const float *lors, int64_t n, int64_t ld, const 
     p.full_patch = (flags & SRFX_FULL_PATCH) ? 1 : 0;
     p.img = image;
     p.out = proj;
-    return launch<kProj>(p, flags, (hipStream_t)stream);
+    hipStream_t st = (hipStream_t)stream;
+    int nslab, inter;
+    if (!slab_schedule(flags, nslab, inter) || n == 0)
+        return launch<kProj>(p, flags, st);
+    return slab_project(p, nslab, flags, st);
 }
 
 static int tor_scatter_common(int mode, const float *lors, int64_t n, int64_t ld,
@@ -39,6 +58,17 @@ static int tor_scatter_common(int mode, const float *lors, int64_t n, int64_t ld
     p.full_patch = (flags & SRFX_FULL_PATCH) ? 1 : 0;
     p.val = vals;
     p.out = image;
+    int nslab, inter;
+    if (slab_schedule(flags, nslab, inter)) {
+        switch (mode) {
+        case kBack:
+            return launch_slab<kBack>(p, nullptr, flags, st);
+        case kAdj:
+            return launch_slab<kAdj>(p, nullptr, flags, st);
+        default:
+            return launch_slab<kMap>(p, nullptr, flags, st);
+        }
+    }
     switch (mode) {
     case kBack:
         return launch<kBack>(p, flags, st);
@@ -93,7 +123,20 @@ extern "C" int srfx_tor_osem_subset(const float *lors, int64_t n, int64_t ld, co
     p.full_patch = (flags & SRFX_FULL_PATCH) ? 1 : 0;
     p.img = image;
     p.out = bp;
-    return launch<kOsem>(p, flags, st);
+    int nslab, inter;
+    if (!slab_schedule(flags, nslab, inter) || n == 0)
+        return launch<kOsem>(p, flags, st);
+    // Slab schedule: two passes, the per-slab projections into a workspace,
+    // then the scatter with their sum.
+    void *ws = nullptr;
+    rc = ws_alloc(&ws, sizeof(float) * (size_t)nslab * (size_t)n, st);
+    if (rc)
+        return rc;
+    rc = launch_slab<kProj>(p, (float *)ws, flags, st);
+    if (!rc)
+        rc = launch_slab<kOsem>(p, (float *)ws, flags, st);
+    int rf = ws_free(ws, st);
+    return rc ? rc : rf;
 }
 
 extern "C" int srfx_tor_count_nnz(const float *lors, int64_t n, int64_t ld, srfx_grid grid,

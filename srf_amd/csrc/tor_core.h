@@ -299,13 +299,30 @@ __device__ __forceinline__ float tof_factor(const TorParams &p, const Event &ev,
     return p.tofk * __builtin_amdgcn_exp2f(D * ev.ctof);
 }
 
+// Offset of voxel column i (row j) from the crossing point, and the squared
+// distance of a voxel centre from the LOR axis, d2 = |D|^2 - (D.u)^2.  Every
+// walk evaluates them through these helpers with contraction off and the fused
+// steps written out, so the chord walk and the full patch produce the same
+// bits for the same voxel.
+__device__ __forceinline__ float voxel_offset(float o0, int i, float step)
+{
+    return fmaf(-(float)i, step, o0);
+}
+
+__device__ __forceinline__ float tube_dist2(float dx, float dy, float ux, float uy)
+{
+#pragma clang fp contract(off)
+    float rc = fmaf(dy, uy, dx * ux);
+    float r2 = fmaf(dy, dy, dx * dx);
+    return fmaf(-rc, rc, r2);
+}
+
 // One voxel: exact include test, then gather / scatter / count.
 template <int KIND>
 __device__ __forceinline__ void visit(const TorParams &p, const Event &ev, int s, int ix, int iy,
                                       float dx, float dy, float amp, float scale, float &acc)
 {
-    float rc = dx * ev.ux + dy * ev.uy;
-    float d2 = dx * dx + dy * dy - rc * rc;
+    float d2 = tube_dist2(dx, dy, ev.ux, ev.uy);
     bool in = d2 < ev.thr - kThrMargin;
     if (!in && d2 < ev.thr + kThrMargin)
         in = inside_exact(p, ev, s, ix, iy);
@@ -349,7 +366,7 @@ __device__ __forceinline__ float patch_walk(const TorParams &p, const Event &ev,
     }
     for (int j = j_lo; j <= j_hi; j++) {
         int iy = c.iy0 + j;
-        float dy = c.dy0 - (float)j * p.finter_y;
+        float dy = voxel_offset(c.dy0, j, p.finter_y);
         int il = i_lo, ih = i_hi;
         if (!FULL) {
             float tt = t.tm - dy * dy * t.q;
@@ -362,7 +379,7 @@ __device__ __forceinline__ float patch_walk(const TorParams &p, const Event &ev,
             ih = min(ih, (int)floorf((ctr + hw) * p.rfinter_x + eps));
         }
         for (int i = il; i <= ih; i++) {
-            float dx = c.dx0 - (float)i * p.finter_x;
+            float dx = voxel_offset(c.dx0, i, p.finter_x);
             visit<KIND>(p, ev, s, c.ix0 + i, iy, dx, dy, amp, scale, acc);
         }
     }
@@ -388,7 +405,7 @@ __device__ __forceinline__ float slice_gather(const TorParams &p, const Event &e
     int il[P], cnt[P];
 #pragma unroll
     for (int j = 0; j < P; j++) {
-        float dy = c.dy0 - (float)j * p.finter_y;
+        float dy = voxel_offset(c.dy0, j, p.finter_y);
         float tt = t.tm - dy * dy * t.q;
         float hw = __builtin_amdgcn_sqrtf(fmaxf(tt, 0.0f) * t.A);
         float ctr = c.dx0 - t.cxy * dy;
@@ -412,14 +429,13 @@ __device__ __forceinline__ float slice_gather(const TorParams &p, const Event &e
     float acc = 0.0f;
 #pragma unroll
     for (int j = 0; j < P; j++) {
-        float dy = c.dy0 - (float)j * p.finter_y;
+        float dy = voxel_offset(c.dy0, j, p.finter_y);
 #pragma unroll
         for (int k = 0; k < P; k++) {
             if (k < cnt[j]) {
                 int i = il[j] + k;
-                float dx = c.dx0 - (float)i * p.finter_x;
-                float rc = dx * ev.ux + dy * ev.uy;
-                float d2 = dx * dx + dy * dy - rc * rc;
+                float dx = voxel_offset(c.dx0, i, p.finter_x);
+                float d2 = tube_dist2(dx, dy, ev.ux, ev.uy);
                 bool in = d2 < ev.thr - kThrMargin;
                 if (!in && d2 < ev.thr + kThrMargin)
                     in = inside_exact(p, ev, s, c.ix0 + i, c.iy0 + j);
@@ -431,16 +447,17 @@ __device__ __forceinline__ float slice_gather(const TorParams &p, const Event &e
     return acc;
 }
 
-// Walk crossed slices s_lo + r0, s_lo + r0 + rstep, ... of an event.  PG > 0:
-// projection with the batched gather of a PG x PG patch.
+// Walk the crossed slices s_first, s_first + rstep, ... <= s_last of an event
+// (the caller clips to [s_lo, s_hi]).  PG > 0: projection with the batched
+// gather of a PG x PG patch.
 template <int KIND, int PG>
 __device__ __forceinline__ float walk_event(const TorParams &p, const Event &ev, bool tof,
-                                            float scale, int r0, int rstep)
+                                            float scale, int s_first, int s_last, int rstep)
 {
     Track t;
     make_track(p, ev, t);
     float acc = 0.0f;
-    for (int s = ev.s_lo + r0; s <= ev.s_hi; s += rstep) {
+    for (int s = s_first; s <= s_last; s += rstep) {
         Cross c;
         cross_at(p, ev, t, s, c);
         float amp = tof ? tof_factor(p, ev, t, s) : 1.0f;
@@ -478,28 +495,28 @@ __device__ __forceinline__ int run_event(const TorParams &p, int64_t e, int sub,
     bool any = valid ? load_event<MODE>(p, e, ev) : false;
     const bool tof = (MODE != kMap && MODE != kCount) && p.use_tof;
     if (MODE == kProj) {
-        float part = any ? walk_event<0, PG>(p, ev, tof, 0.0f, sub, LPE) : 0.0f;
+        float part = any ? walk_event<0, PG>(p, ev, tof, 0.0f, ev.s_lo + sub, ev.s_hi, LPE) : 0.0f;
         float pv = group_sum<LPE>(part);
         if (valid && sub == 0)
             p.out[e] = pv;
     } else if (MODE == kBack) {
         float v = valid ? p.val[e] : 0.0f;
         if (any && v > 1e-7)
-            walk_event<1, 0>(p, ev, tof, 1.0f / v, sub, LPE);
+            walk_event<1, 0>(p, ev, tof, 1.0f / v, ev.s_lo + sub, ev.s_hi, LPE);
     } else if (MODE == kMap) {
         if (any)
-            walk_event<1, 0>(p, ev, false, 1.0f / p.val[e], sub, LPE);
+            walk_event<1, 0>(p, ev, false, 1.0f / p.val[e], ev.s_lo + sub, ev.s_hi, LPE);
     } else if (MODE == kAdj) {
         float g = valid ? p.val[e] : 0.0f;
         if (any && g != 0.0f)
-            walk_event<1, 0>(p, ev, tof, g, sub, LPE);
+            walk_event<1, 0>(p, ev, tof, g, ev.s_lo + sub, ev.s_hi, LPE);
     } else if (MODE == kOsem) {
-        float part = any ? walk_event<0, PG>(p, ev, tof, 0.0f, sub, LPE) : 0.0f;
+        float part = any ? walk_event<0, PG>(p, ev, tof, 0.0f, ev.s_lo + sub, ev.s_hi, LPE) : 0.0f;
         float pv = group_sum<LPE>(part);
         if (any && pv > 1e-7)
-            walk_event<1, 0>(p, ev, tof, 1.0f / pv, sub, LPE);
+            walk_event<1, 0>(p, ev, tof, 1.0f / pv, ev.s_lo + sub, ev.s_hi, LPE);
     } else { // kCount
-        float part = any ? walk_event<2, 0>(p, ev, false, 0.0f, sub, LPE) : 0.0f;
+        float part = any ? walk_event<2, 0>(p, ev, false, 0.0f, ev.s_lo + sub, ev.s_hi, LPE) : 0.0f;
         int cnt = (int)group_sum<LPE>(part);
         if (valid && sub == 0) {
             if (p.nnz)
@@ -525,6 +542,94 @@ __global__ __launch_bounds__(256) void tor_kernel(TorParams p)
         if ((threadIdx.x & 63) == 0 && c)
             atomicAdd(p.nnz_total, c);
     }
+}
+
+// Slab schedule.  The slices are cut into nslab contiguous slabs and every
+// event is visited once per slab, by a lane that walks only the event's
+// slices inside that slab.  With interleave, workgroup b works on slab
+// b % nslab: the dispatcher hands consecutive workgroups to consecutive XCDs,
+// so with nslab a multiple of 8 each XCD gathers from and scatters into its
+// own slabs only, and a 16 MB image splits into L2-sized pieces instead of
+// every XCD's L2 cycling through all of it.  Without interleave each slab gets
+// a contiguous range of workgroups (same work, no XCD affinity).
+//   kProj: part[slab * n + e] = the event's projection over the slab;
+//   kOsem: the scatter half of the OSEM step, with the projection
+//          sum_k part[k * n + e] (slab order, so deterministic);
+//   kBack / kMap / kAdj: the scatter over the slab.
+__device__ __forceinline__ void slab_of(const TorParams &p, int nslab, int interleave, int &slab,
+                                        int64_t &chunk)
+{
+    if (interleave) {
+        slab = (int)(blockIdx.x % (unsigned)nslab);
+        chunk = blockIdx.x / (unsigned)nslab;
+    } else {
+        unsigned nchunk = gridDim.x / (unsigned)nslab;
+        slab = (int)(blockIdx.x / nchunk);
+        chunk = blockIdx.x % nchunk;
+    }
+}
+
+template <int MODE, int PG>
+__global__ __launch_bounds__(256) void tor_slab_kernel(TorParams p, float *part, int nslab,
+                                                       int interleave)
+{
+    int slab;
+    int64_t chunk;
+    slab_of(p, nslab, interleave, slab, chunk);
+    const int64_t e = chunk * blockDim.x + threadIdx.x;
+    const bool valid = e < p.n;
+    const int sa = (int)((int64_t)slab * p.gz / nslab);
+    const int sb = (int)((int64_t)(slab + 1) * p.gz / nslab) - 1;
+    Event ev;
+    bool any = valid ? load_event<MODE>(p, e, ev) : false;
+    int s0 = 0, s1 = -1;
+    if (any) {
+        s0 = max(ev.s_lo, sa);
+        s1 = min(ev.s_hi, sb);
+        any = s0 <= s1;
+    }
+    const bool tof = MODE != kMap && p.use_tof;
+    if (MODE == kProj) {
+        float v = any ? walk_event<0, PG>(p, ev, tof, 0.0f, s0, s1, 1) : 0.0f;
+        if (valid)
+            part[(int64_t)slab * p.n + e] = v;
+    } else {
+        float scale = 0.0f;
+        bool go = any;
+        if (MODE == kOsem) {
+            float v = 0.0f;
+            if (valid)
+                for (int k = 0; k < nslab; k++)
+                    v += part[(int64_t)k * p.n + e];
+            go = go && v > 1e-7;
+            scale = 1.0f / v;
+        } else if (MODE == kBack) {
+            float v = valid ? p.val[e] : 0.0f;
+            go = go && v > 1e-7;
+            scale = 1.0f / v;
+        } else if (MODE == kMap) {
+            scale = valid ? 1.0f / p.val[e] : 0.0f;
+        } else { // kAdj
+            float g = valid ? p.val[e] : 0.0f;
+            go = go && g != 0.0f;
+            scale = g;
+        }
+        if (go)
+            walk_event<1, 0>(p, ev, tof, scale, s0, s1, 1);
+    }
+}
+
+// out[e] = sum over slabs of part[k * n + e], slab order.
+__global__ __launch_bounds__(256) void tor_slab_sum(const float *part, int nslab, int64_t n,
+                                                    float *out)
+{
+    int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n)
+        return;
+    float v = 0.0f;
+    for (int k = 0; k < nslab; k++)
+        v += part[(int64_t)k * n + e];
+    out[e] = v;
 }
 
 // Host part of the reference setup (tor.cu.cc:364-375), in double.
@@ -606,6 +711,57 @@ int launch_lpe(const TorParams &p, int block, hipStream_t st)
         }
     }
     return launch_pg<MODE, 0, LPE>(p, block, st);
+}
+
+template <int MODE, int PG>
+int launch_slab_pg(const TorParams &p, float *part, int nslab, int interleave, int block,
+                   hipStream_t st)
+{
+    unsigned chunks = blocks_for(p.n, block);
+    hipLaunchKernelGGL((tor_slab_kernel<MODE, PG>), dim3(chunks * (unsigned)nslab), dim3(block),
+                       0, st, p, part, nslab, interleave);
+    return launch_status();
+}
+
+// Slab schedule (flags field SRFX_SLAB_*): kProj writes the per-slab partial
+// projections to part, kOsem / kBack / kMap / kAdj scatter (kOsem reading
+// part).  Returns SRFX_EUNSUPPORTED for schedules without slabs.
+template <int MODE>
+int launch_slab(const TorParams &p, float *part, uint32_t flags, hipStream_t st)
+{
+    int nslab, interleave;
+    if (!slab_schedule(flags, nslab, interleave))
+        return SRFX_EUNSUPPORTED;
+    static const int kBlock[4] = {256, 64, 128, 256};
+    const int block = kBlock[(flags >> SRFX_BLOCK_SHIFT) & 3u];
+    if (p.n == 0)
+        return SRFX_OK;
+    if constexpr (MODE == kProj) {
+        if (!p.full_patch) {
+            switch (p.patch) {
+            case 3:
+                return launch_slab_pg<MODE, 3>(p, part, nslab, interleave, block, st);
+            case 4:
+                return launch_slab_pg<MODE, 4>(p, part, nslab, interleave, block, st);
+            case 5:
+                return launch_slab_pg<MODE, 5>(p, part, nslab, interleave, block, st);
+            case 6:
+                return launch_slab_pg<MODE, 6>(p, part, nslab, interleave, block, st);
+            default:
+                break;
+            }
+        }
+    }
+    return launch_slab_pg<MODE, 0>(p, part, nslab, interleave, block, st);
+}
+
+int launch_slab_sum(const float *part, int nslab, int64_t n, float *out, hipStream_t st)
+{
+    if (n == 0)
+        return SRFX_OK;
+    hipLaunchKernelGGL(tor_slab_sum, dim3(blocks_for(n, 256)), dim3(256), 0, st, part, nslab, n,
+                       out);
+    return launch_status();
 }
 
 // Launch schedule from the flags: lanes per event (1, 4, 16) and block size

@@ -1,5 +1,7 @@
 // Elementwise kernels: output zeroing (a fill kernel, so the ops stay
 // capturable in HIP graphs) and the MLEM image update.
+#include <atomic>
+
 #include "common.h"
 
 namespace srfx {
@@ -70,6 +72,35 @@ int zero_grid(float *img, const srfx_grid &g, uint32_t flags, hipStream_t st)
     return launch_status();
 }
 
+int ws_alloc(void **ptr, size_t bytes, hipStream_t st)
+{
+    // Keep freed workspace in the pool instead of returning it to the driver
+    // at every synchronisation (once per device).
+    static std::atomic<uint64_t> pool_set{0};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess)
+        return SRFX_ELAUNCH;
+    uint64_t bit = 1ull << (dev & 63);
+    if (!(pool_set.load() & bit)) {
+        hipMemPool_t pool;
+        if (hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess) {
+            uint64_t thr = UINT64_MAX;
+            (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &thr);
+        }
+        pool_set.fetch_or(bit);
+    }
+    if (hipMallocAsync(ptr, bytes ? bytes : 256, st) != hipSuccess) {
+        (void)hipGetLastError();
+        return SRFX_ENOMEM;
+    }
+    return SRFX_OK;
+}
+
+int ws_free(void *ptr, hipStream_t st)
+{
+    return hipFreeAsync(ptr, st) == hipSuccess ? SRFX_OK : SRFX_ELAUNCH;
+}
+
 } // namespace srfx
 
 using namespace srfx;
@@ -97,6 +128,8 @@ extern "C" const char *srfx_strerror(int status)
         return "HIP launch failed";
     case SRFX_EUNSUPPORTED:
         return "unsupported configuration";
+    case SRFX_ENOMEM:
+        return "device workspace allocation failed";
     default:
         return "unknown status";
     }

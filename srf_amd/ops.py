@@ -130,7 +130,8 @@ def backprojection(image, lors, lors_value, grid, center, size, tof_bin=1.0e4, t
 
 
 def backprojection_adjoint(image, lors, g_values, grid, center, size, tof_bin=1.0e4,
-                           tof_sigma2=1.0e4, kernel_width=None, geo_sigma2_flag=False):
+                           tof_sigma2=1.0e4, kernel_width=None, geo_sigma2_flag=False,
+                           schedule=None):
     """Plain adjoint sum_i w_ij g_i (no reference counterpart)."""
     if kernel_width is None:
         raise ValueError("kernel_width is required")
@@ -140,12 +141,13 @@ def backprojection_adjoint(image, lors, g_values, grid, center, size, tof_bin=1.
     v = _vec(g_values, n, "g")
     out = torch.empty_like(image, memory_format=torch.contiguous_format)
     m = _lib.make_tor_model(kernel_width, geo_sigma2_flag, tof_bin, tof_sigma2)
-    _lib.call("srfx_tor_backproject_adjoint", _ptr(lors), n, ld, _ptr(v), cg, m, _ptr(out), 0,
-              _stream())
+    _lib.call("srfx_tor_backproject_adjoint", _ptr(lors), n, ld, _ptr(v), cg, m, _ptr(out),
+              _lib.schedule_flags(schedule), _stream())
     return out
 
 
-def maplors(image, grid, center, size, lors, lors_value, kernel_width, geo_sigma2_flag=False):
+def maplors(image, grid, center, size, lors, lors_value, kernel_width, geo_sigma2_flag=False,
+            schedule=None):
     """Op ``Maplors`` of tor.so (tor.cc:49-64, 290-348): lors [7, N] =
     x1..z2, sigma2 factor; sum_i w_ij / v_i without TOF and threshold."""
     lors, n, ld = _soa(lors, 7)
@@ -154,7 +156,7 @@ def maplors(image, grid, center, size, lors, lors_value, kernel_width, geo_sigma
     v = _vec(lors_value, n, "lors_value")
     out = torch.empty_like(image, memory_format=torch.contiguous_format)
     _lib.call("srfx_tor_maplors", _ptr(lors), n, ld, _ptr(v), cg, float(kernel_width),
-              1 if geo_sigma2_flag else 0, _ptr(out), 0, _stream())
+              1 if geo_sigma2_flag else 0, _ptr(out), _lib.schedule_flags(schedule), _stream())
     return out
 
 

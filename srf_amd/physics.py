@@ -53,6 +53,9 @@ class SplitLoRsModel:
         self.tof_bin = 1.0e4 if tof_bin is None else float(tof_bin)
         # launch schedule per op: (lanes_per_event, block) or None (default)
         self.schedule = {"projection": None, "backprojection": None, "osem_subset": None}
+        # tile shape (slices, wx, wy) and chunk of the tiled plans (make_plans)
+        self.tile = None
+        self.chunk = None
 
     def _flags(self, op):
         return _lib.schedule_flags(self.schedule.get(op))
@@ -93,22 +96,51 @@ class SplitLoRsModel:
         g, c, s, st = group_frame(data.shape, image.center, image.size, data.stride(), axis)
         return _lib.make_grid(g, c, s, st)
 
-    def projection(self, image, data):
-        """split_lors.py:73-90: per group p = Projection(...)."""
+    def make_plans(self, image, data, ncols=10):
+        """One srf_amd.plan.TorPlan per non-empty group of ``data`` for images
+        laid out like ``image`` (world (nx, ny, nz), any strides): the events
+        binned into image tiles once, for the tiled backprojection /
+        Maplors / OSEM step.  Returns {axis: TorPlan or None}."""
+        from .plan import DEFAULT_CHUNK, DEFAULT_TILE, TorPlan
         x = _world(image)
-        out = {}
+        plans = {}
         for a in self.AXIS:
             lm = data[a]
-            lors, n, ld = _soa(lm.soa, 10)
-            p = torch.empty(n, dtype=torch.float32, device=x.device)
-            if n:
-                _lib.call("srfx_tor_project", _ptr(lors), n, ld, _ptr(x), self._grid(image, a),
-                          self._model(), _ptr(p), self._flags("projection"), _stream())
-            out[a] = lm.replace_values(p)
+            if len(lm) == 0:
+                plans[a] = None
+                continue
+            g, c, sz, st = group_frame(x.shape, image.center, image.size, x.stride(), a)
+            plans[a] = TorPlan(lm.soa, g, c, sz, self.kernel_width, self.geo_sigma2_flag,
+                               self.tof_bin, self.tof_sigma2, strides=st, ncols=ncols,
+                               tile=self.tile or DEFAULT_TILE, chunk=self.chunk or DEFAULT_CHUNK,
+                               axis=a)
+        return plans
+
+    @staticmethod
+    def _check_plan_layout(plan, t):
+        if list(plan.strides) != [int(v) for v in group_frame(
+                t.shape, plan.center, plan.size, t.stride(), plan.axis)[3]]:
+            raise ValueError("image layout differs from the one the plan was built for")
+
+    def project_group(self, image, lm, a):
+        """Projection of the events ``lm`` (ListModeData) of group ``a``."""
+        x = _world(image)
+        lors, n, ld = _soa(lm.soa, 10)
+        p = torch.empty(n, dtype=torch.float32, device=x.device)
+        if n:
+            _lib.call("srfx_tor_project", _ptr(lors), n, ld, _ptr(x), self._grid(image, a),
+                      self._model(), _ptr(p), self._flags("projection"), _stream())
+        return p
+
+    def projection(self, image, data):
+        """split_lors.py:73-90: per group p = Projection(...)."""
+        out = {a: data[a].replace_values(self.project_group(image, data[a], a))
+               for a in self.AXIS}
         return ListModeDataSplit(out["x"], out["y"], out["z"])
 
-    def backprojection(self, data, image, out=None):
-        """split_lors.py:93-113: sum over groups of Backprojection(lors_value)."""
+    def backprojection(self, data, image, out=None, plans=None):
+        """split_lors.py:93-113: sum over groups of Backprojection(lors_value).
+        With ``plans`` (make_plans of the same events) the tiled kernels run."""
         x = _world(image)
         if out is None:
             out = torch.zeros_like(x)
@@ -118,6 +150,9 @@ class SplitLoRsModel:
             if n == 0:
                 continue
             v = _vec(lm.values, n, "lors_value")
+            if plans is not None:
+                self._planned(plans, a, n, out).backprojection(v, out=out, accumulate=True)
+                continue
             _lib.call("srfx_tor_backproject", _ptr(lors), n, ld, _ptr(v),
                       self._grid(Image(out, image.center, image.size), a), self._model(),
                       _ptr(out), _lib.SRFX_ACCUMULATE | self._flags("backprojection"), _stream())
@@ -140,9 +175,19 @@ class SplitLoRsModel:
                       _lib.SRFX_ACCUMULATE, _stream())
         return Image(out, image.center, image.size)
 
-    def osem_subset(self, image, data, out=None):
-        """Fused projection + ratio backprojection of all groups
-        (recon_step.py:33-41 without writing the projections)."""
+    @staticmethod
+    def _planned(plans, a, n, out):
+        plan = plans.get(a)
+        if plan is None or plan.n != n:
+            raise ValueError(f"no plan for the {n} events of group {a}")
+        SplitLoRsModel._check_plan_layout(plan, out)
+        return plan
+
+    def osem_subset(self, image, data, out=None, plans=None):
+        """Projection + ratio backprojection of all groups (recon_step.py:33-41
+        without writing the projections): one fused kernel per group, or with
+        ``plans`` (make_plans of the same events) the projection kernel and
+        the tiled backprojection of 1/p."""
         x = _world(image)
         if out is None:
             out = torch.zeros_like(x)
@@ -150,6 +195,12 @@ class SplitLoRsModel:
             lm = data[a]
             lors, n, ld = _soa(lm.soa, 10)
             if n == 0:
+                continue
+            if plans is not None:
+                plan = self._planned(plans, a, n, out)
+                self._check_plan_layout(plan, x)
+                plan.osem_subset(x, out=out, accumulate=True,
+                                 schedule=self.schedule.get("osem_subset"))
                 continue
             g_in = self._grid(image, a)
             g_out = self._grid(Image(out, image.center, image.size), a)

@@ -1,0 +1,114 @@
+"""Tiled TOR ops over a fixed event set (srfx_tor_plan_*).
+
+A ``TorPlan`` bins the events of one group frame into image tiles of
+``slices`` x ``wx`` x ``wy`` op-frame voxels once; its ops then compute the
+same projection / backprojection / Maplors / OSEM step as the untiled ops of
+``srf_amd.ops`` (src/cpp/tf_custom_ops/tor.cc:14-64 semantics) with the
+scatter accumulated per tile in LDS and flushed with coalesced atomics, and
+the projection optionally gathered from tiles staged in LDS.  OSEM reuses one
+plan per (subset, group) for every iteration, since the events of a subset
+do not change between iterations (graph/reconstruction/worker.py:79-85).
+
+The plan keeps a reference to the LOR tensor: it must not be modified while
+the plan lives.  Images use the plan's grid layout: contiguous [gz][gy][gx] in
+the op frame, or the element strides given at construction.
+"""
+import ctypes
+
+import torch
+
+from . import _lib
+from .ops import _check_dev, _geom, _ptr, _soa, _stream, _vec
+
+DEFAULT_TILE = (8, 32, 32)
+DEFAULT_CHUNK = 4096
+
+
+class TorPlan:
+    def __init__(self, lors, grid, center, size, kernel_width, geo_sigma2_flag=False,
+                 tof_bin=1.0e4, tof_sigma2=1.0e4, strides=(0, 0, 0), ncols=10,
+                 tile=DEFAULT_TILE, chunk=DEFAULT_CHUNK, block=256, axis=None):
+        if ncols not in (7, 10):
+            raise ValueError("ncols must be 10 (TOF rows) or 7 (Maplors rows)")
+        self.lors, self.n, ld = _soa(lors, ncols)
+        self.grid, self._cgrid = _geom(grid, center, size, strides)
+        self.center, self.size = [float(c) for c in center], [float(s) for s in size]
+        self.strides = tuple(int(s) for s in strides)
+        self.axis = axis
+        self.ncols = ncols
+        self.tile = tuple(int(t) for t in tile)
+        model = _lib.make_tor_model(kernel_width, geo_sigma2_flag, tof_bin, tof_sigma2)
+        t = _lib.SrfxTile(self.tile[0], self.tile[1], self.tile[2], int(chunk), int(block))
+        h = ctypes.c_void_p()
+        _lib.call("srfx_tor_plan_create", _ptr(self.lors), self.n, ld, ncols, self._cgrid, model,
+                  t, ctypes.byref(h), _stream())
+        self._h = h
+
+    def info(self):
+        """(number of (event, tile) listings, number of workgroup chunks)."""
+        a, b = ctypes.c_int64(), ctypes.c_int32()
+        _lib.call("srfx_tor_plan_info", self._h, ctypes.byref(a), ctypes.byref(b))
+        return int(a.value), int(b.value)
+
+    def _image(self, image, name="image"):
+        _check_dev(image, name)
+        g = self.grid
+        if self.strides == (0, 0, 0):
+            if image.numel() != g[0] * g[1] * g[2] or not image.is_contiguous():
+                raise ValueError(f"{name} must be a contiguous {g[::-1]} tensor")
+        return image
+
+    def _out(self, out, like):
+        if out is None:
+            if self.strides != (0, 0, 0):
+                raise ValueError("strided plans need an explicit output image")
+            g = self.grid
+            return torch.empty((g[2], g[1], g[0]), dtype=torch.float32, device=like.device)
+        return self._image(out, "out")
+
+    def projection(self, image):
+        image = self._image(image)
+        out = torch.empty(self.n, dtype=torch.float32, device=image.device)
+        _lib.call("srfx_tor_plan_project", self._h, _ptr(image), _ptr(out), 0, _stream())
+        return out
+
+    def _scatter(self, name, values, out, accumulate, device):
+        v = _vec(values, self.n, "lors_value")
+        out = self._out(out, v)
+        flags = _lib.SRFX_ACCUMULATE if accumulate else 0
+        _lib.call(name, self._h, _ptr(v), _ptr(out), flags, _stream())
+        return out
+
+    def backprojection(self, lors_value, out=None, accumulate=False):
+        return self._scatter("srfx_tor_plan_backproject", lors_value, out, accumulate, None)
+
+    def backprojection_adjoint(self, g, out=None, accumulate=False):
+        return self._scatter("srfx_tor_plan_backproject_adjoint", g, out, accumulate, None)
+
+    def maplors(self, lors_value, out=None, accumulate=False):
+        return self._scatter("srfx_tor_plan_maplors", lors_value, out, accumulate, None)
+
+    def osem_subset(self, image, out=None, accumulate=False, tiled_projection=False,
+                    schedule=None, out_strides=None):
+        """Backprojection of 1/p (p > 1e-7) with p the projection of image."""
+        image = self._image(image)
+        out = self._out(out, image)
+        og = self._cgrid if out_strides is None else _lib.make_grid(
+            self.grid, self.center, self.size, out_strides)
+        flags = ((_lib.SRFX_ACCUMULATE if accumulate else 0)
+                 | (_lib.SRFX_TILED_PROJECTION if tiled_projection else 0)
+                 | _lib.schedule_flags(schedule))
+        _lib.call("srfx_tor_plan_osem_subset", self._h, _ptr(image), og, _ptr(out), flags,
+                  _stream())
+        return out
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            _lib.call("srfx_tor_plan_destroy", self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -52,31 +52,62 @@ class ReconStep:
         self.fused = fused
         self._bp = None
 
-    def __call__(self, image, data, efficiency_map):
+    def __call__(self, image, data, efficiency_map, plans=None):
         if self._bp is None or self._bp.shape != image.data.shape:
             self._bp = torch.empty_like(image.data)
         self._bp.zero_()
         if self.fused:
-            self.model.osem_subset(image, data, out=self._bp)
+            if plans is None:
+                self.model.osem_subset(image, data, out=self._bp)
+            else:
+                self.model.osem_subset(image, data, out=self._bp, plans=plans)
         else:
             proj = self.model.projection(image, data)
-            self.model.backprojection(proj, image, out=self._bp)
+            if plans is None:
+                self.model.backprojection(proj, image, out=self._bp)
+            else:
+                self.model.backprojection(proj, image, out=self._bp, plans=plans)
         return mlem_update(image, Image(self._bp, image.center, image.size), efficiency_map)
 
 
 class OSEM:
-    """iterations x subsets of ReconStep over list-mode data split in groups."""
+    """iterations x subsets of ReconStep over list-mode data split in groups.
 
-    def __init__(self, model, data, efficiency_map, nb_subsets=1, fused=True):
+    tiled=True (models with make_plans, the TOR model): the events of every
+    subset are binned into image tiles once (``plan``, on the first subset
+    run or explicitly) and every later step of that subset reuses the plan."""
+
+    def __init__(self, model, data, efficiency_map, nb_subsets=1, fused=True, tiled=False):
         self.step = ReconStep(model, fused=fused)
+        self.model = model
         self.data = data
         self.emap = efficiency_map
         self.nb_subsets = int(nb_subsets)
         self.subsets = [subset_view(data, k, self.nb_subsets) for k in range(self.nb_subsets)]
         self.subset = 0
+        self.tiled = bool(tiled) and hasattr(model, "make_plans")
+        self.plans = None
+
+    def plan(self, image):
+        """Build the tile plans of every subset for images laid out like image."""
+        self.close()
+        self.plans = [self.model.make_plans(image, sub) for sub in self.subsets]
+        return self.plans
+
+    def close(self):
+        for ps in self.plans or []:
+            for p in ps.values():
+                if p is not None:
+                    p.close()
+        self.plans = None
 
     def run_subset(self, image):
-        self.step(image, self.subsets[self.subset], self.emap)
+        plans = None
+        if self.tiled:
+            if self.plans is None:
+                self.plan(image)
+            plans = self.plans[self.subset]
+        self.step(image, self.subsets[self.subset], self.emap, plans=plans)
         self.subset = (self.subset + 1) % self.nb_subsets
         return image
 

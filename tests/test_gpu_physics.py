@@ -84,8 +84,9 @@ def test_pipeline_projection_vs_oracle(cuda, events):
         assert e <= 1e-5
 
 
-@pytest.mark.parametrize("fused", [True, False])
-def test_osem_vs_oracle(cuda, events, fused):
+@pytest.mark.parametrize("fused,tiled", [(True, False), (False, False), (True, True),
+                                         (False, True)])
+def test_osem_vs_oracle(cuda, events, fused, tiled):
     from oracle import preprocess as OP, recon as OR
     from srf_amd import preprocess
     from srf_amd.data import Image, split_from_groups
@@ -98,13 +99,18 @@ def test_osem_vs_oracle(cuda, events, fused):
     x0 = np.ones(GRID, np.float32)
     x = torch.from_numpy(x0).to(cuda)
     osem = OSEM(model, split_from_groups(groups), Image(torch.from_numpy(E).to(cuda), CENTER, SIZE),
-                nb_subsets=3, fused=fused)
+                nb_subsets=3, fused=fused, tiled=tiled)
+    if tiled:
+        model.tile, model.chunk = (4, 8, 8), 300  # many tiles and chunks at this size
     osem.iterate(Image(x, CENTER, SIZE), 2)
+    if tiled:
+        assert osem.plans is not None and all(len(ps) == 3 for ps in osem.plans)
+        osem.close()
     ref_groups = {a: np.ascontiguousarray(g.T.astype(np.float32))
                   for a, g in OP.recon_process(ev, sc.tof_res_ps).items()}
     ref = OR.osem(ref_groups, x0, E.astype(np.float64), CENTER, SIZE, KW, True, tb, ts2, 2, 3)
     e = H.rel_l2(x.cpu().numpy(), ref)
-    print(f"OSEM 2 it x 3 subsets fused={fused}: rel_l2 vs f64 {e:.2e}")
+    print(f"OSEM 2 it x 3 subsets fused={fused} tiled={tiled}: rel_l2 vs f64 {e:.2e}")
     assert e <= 1e-4
 
 

@@ -1,0 +1,121 @@
+"""Tiled TOR ops (srfx_tor_plan_*) against the FP64 oracle.
+
+A plan bins each event into the image tiles it can touch; the tiled
+scatter accumulates per tile in LDS and the tiled gather reads tiles staged in
+LDS.  Every op must give the untiled results: rel L2 <= 1e-5 against the FP64
+oracle for every tile shape, including windows narrower than the patch (an
+event then spans many tiles) and chunk sizes that split a tile over many
+workgroups.  The tiled projection sums each event's per-tile partials in a
+fixed order, so it is deterministic.
+"""
+import numpy as np
+import pytest
+
+from tests import helpers as H
+
+pytestmark = pytest.mark.gpu
+
+TOL64 = 1e-5
+# (slices, wx, wy), chunk
+TILES = [((8, 32, 32), 4096), ((1, 4, 4), 64), ((3, 5, 7), 97), ((16, 16, 32), 1024)]
+
+
+def _t(a, dev):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+
+
+@pytest.fixture(scope="module")
+def case(cuda):
+    lors = H.tor_lors(6000, seed=11)
+    img = H.image(H.TOR_GRID, seed=4)
+    return lors, img
+
+
+def _plan(L, geo, tof, tile, chunk, ncols=10):
+    from srf_amd.plan import TorPlan
+    ts2 = H.TOF_SIGMA2 if tof else 1.0e6
+    return TorPlan(L, H.TOR_GRID, H.TOR_CENTER, H.TOR_SIZE, H.KW, geo, H.TOF_BIN, ts2,
+                   ncols=ncols, tile=tile, chunk=chunk)
+
+
+@pytest.mark.parametrize("tile,chunk", TILES)
+@pytest.mark.parametrize("geo,tof", [(False, True), (True, True), (True, False)])
+def test_plan_project_backproject(cuda, case, tile, chunk, geo, tof):
+    from oracle import lib as O
+    lors, img = case
+    ts2 = H.TOF_SIGMA2 if tof else 1.0e6
+    args = (H.TOR_GRID, H.TOR_CENTER, H.TOR_SIZE, H.KW, geo, H.TOF_BIN, ts2)
+    L, X = _t(lors, cuda), _t(img, cuda)
+    plan = _plan(L, geo, tof, tile, chunk)
+    nl, nc = plan.info()
+    assert nl >= lors.shape[1] // 2 and nc >= 1
+    p = plan.projection(X).cpu().numpy()
+    ref = O.tor_project(lors, img, *args, prec="f64")
+    e = H.rel_l2(p, ref)
+    print(f"plan {tile} chunk {chunk} geo={geo} tof={tof}: {nl} listings, {nc} chunks, "
+          f"proj rel_l2 {e:.2e}")
+    assert e <= TOL64
+    assert np.array_equal(p, plan.projection(X).cpu().numpy())
+    v = np.random.default_rng(2).uniform(0.5, 2.0, lors.shape[1]).astype(np.float32)
+    v[::11] = 0.0
+    v[1::29] = 1e-8
+    b = plan.backprojection(_t(v, cuda)).cpu().numpy()
+    rb = O.tor_backproject(lors, v, H.TOR_GRID, H.TOR_CENTER, H.TOR_SIZE, H.KW, geo, H.TOF_BIN,
+                           ts2, prec="f64")
+    assert H.rel_l2(b, rb) <= TOL64
+    a = plan.backprojection_adjoint(_t(v - 1.0, cuda)).cpu().numpy()
+    ra = O.tor_backproject_adjoint(lors, v - 1.0, H.TOR_GRID, H.TOR_CENTER, H.TOR_SIZE, H.KW,
+                                   geo, H.TOF_BIN, ts2)
+    assert H.rel_l2(a, ra) <= TOL64
+    plan.close()
+
+
+@pytest.mark.parametrize("tiled_projection", [False, True])
+def test_plan_osem_subset(cuda, case, tiled_projection):
+    import torch
+    from oracle import lib as O
+    from srf_amd import ops
+    lors, img = case
+    args = (H.TOR_GRID, H.TOR_CENTER, H.TOR_SIZE, H.KW, True, H.TOF_BIN, H.TOF_SIGMA2)
+    L, X = _t(lors, cuda), _t(img, cuda)
+    plan = _plan(L, True, True, (4, 16, 16), 512)
+    f = plan.osem_subset(X, tiled_projection=tiled_projection).cpu().numpy()
+    ref = O.tor_project(lors, img, *args, prec="f64")
+    keep = ref > 1e-7
+    inv = np.where(keep, 1.0 / np.where(keep, ref, 1.0), 0.0).astype(np.float32)
+    rf = O.tor_backproject_adjoint(lors, inv, H.TOR_GRID, H.TOR_CENTER, H.TOR_SIZE, H.KW, True,
+                                   H.TOF_BIN, H.TOF_SIGMA2)
+    assert H.rel_l2(f, rf) <= TOL64
+    # accumulate adds into the given image
+    base = torch.full_like(X, 0.5)
+    acc = plan.osem_subset(X, out=base.clone(), accumulate=True,
+                           tiled_projection=tiled_projection)
+    un = ops.osem_subset(L, X, *args)
+    assert H.rel_l2((acc - 0.5).cpu().numpy(), un.cpu().numpy()) <= 1e-6
+    plan.close()
+
+
+def test_plan_maplors(cuda, case):
+    from oracle import lib as O
+    lors, _ = case
+    l7 = np.ascontiguousarray(np.vstack([lors[0:6], lors[9:10]]))
+    w = np.random.default_rng(3).uniform(0.5, 2.0, l7.shape[1]).astype(np.float32)
+    plan = _plan(_t(l7, cuda), True, False, (8, 16, 16), 256, ncols=7)
+    m = plan.maplors(_t(w, cuda)).cpu().numpy()
+    r = O.tor_map(l7, w, H.TOR_GRID, H.TOR_CENTER, H.TOR_SIZE, H.KW, True, prec="f64")
+    assert H.rel_l2(m, r) <= TOL64
+    plan.close()
+
+
+def test_plan_empty(cuda):
+    import torch
+    from srf_amd.plan import TorPlan
+    L = torch.zeros(10, 0, device=cuda)
+    plan = TorPlan(L, H.TOR_GRID, H.TOR_CENTER, H.TOR_SIZE, H.KW)
+    assert plan.info() == (0, 0)
+    X = torch.ones(H.TOR_GRID[::-1], device=cuda)
+    assert plan.projection(X).numel() == 0
+    b = plan.backprojection(torch.zeros(0, device=cuda))
+    assert float(b.abs().sum()) == 0.0
+    plan.close()
