@@ -43,7 +43,7 @@ def parse():
                     help="time separate projection + backprojection kernels")
     ap.add_argument("--workload", default="tof_tor_c2",
                     choices=["tof_tor_c2", "siddon_c1", "autograd_c3", "batched_c4",
-                             "large_c5"])
+                             "large_c5", "hbm_copy"])
     ap.add_argument("--autotune", action="store_true",
                     help="tune the launch schedule on a sample before timing")
     ap.add_argument("--no-sort", dest="sort", action="store_false",

@@ -47,7 +47,35 @@ def _timer():
 
 def run(args):
     {"siddon_c1": siddon_c1, "autograd_c3": autograd_c3, "batched_c4": batched_c4,
-     "large_c5": large_c5}[args.workload](args)
+     "large_c5": large_c5, "hbm_copy": hbm_copy}[args.workload](args)
+
+
+# ------------------------------------------------------------ calibration ---
+
+def hbm_copy(args):
+    """A plain 1 GiB device-to-device copy (torch's copy kernel): the reference
+    point that scales the FETCH_SIZE / WRITE_SIZE counters of a profiled
+    kernel to bytes (the counters of this copy should read 1 GiB each)."""
+    import torch
+    dev = torch.device("cuda:0")
+    n = 1 << 28  # 2^28 float32 = 1 GiB
+    src = torch.rand(n, device=dev)
+    dst = torch.empty_like(src)
+    for _ in range(max(args.warmup, 1)):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    a, b = _timer()
+    a.record()
+    for _ in range(args.steps):
+        dst.copy_(src)
+    b.record()
+    torch.cuda.synchronize()
+    ms = a.elapsed_time(b) / args.steps
+    gbs = 2 * 4 * n / (ms / 1e3) / 1e9
+    print(json.dumps(_base("device copy GB/s (read + write)", round(gbs, 1), "GB/s", args.steps,
+                           args.warmup, round(ms, 4),
+                           {"workload": "hbm_copy", "bytes_read": 4 * n, "bytes_written": 4 * n},
+                           dtype="f32")), flush=True)
 
 
 # --------------------------------------------------------------------- C1 ---

@@ -194,9 +194,12 @@ int srfx_siddon_count_nnz(const float *lors, int64_t n, int64_t ld, srfx_grid gr
  * touch; scatters then accumulate per tile in LDS and flush each tile with
  * coalesced atomics, and the tiled projection gathers from tiles staged in
  * LDS.  `chunk` caps the events one workgroup takes from a tile; `block` is
- * the workgroup size (64, 128, 256; 0 = 256).  slices * wx * wy <= 16384. */
+ * the workgroup size (64, 128, 256; 0 = 256).  slices * wx * wy <= 16384.
+ * The tiled projection (srfx_tor_plan_project, SRFX_TILED_PROJECTION) needs a
+ * plan built with `projection` set (8 more bytes per listing). */
 typedef struct srfx_tile {
     int32_t slices, wx, wy, chunk, block;
+    int32_t projection; /* nonzero: keep the tables of the tiled projection */
 } srfx_tile;
 
 typedef struct srfx_tor_plan_s *srfx_tor_plan;
@@ -216,7 +219,8 @@ int srfx_tor_plan_info(srfx_tor_plan plan, int64_t *nlistings, int32_t *nchunks)
 /* Synchronises the device, then frees the plan. */
 int srfx_tor_plan_destroy(srfx_tor_plan plan);
 
-/* Projection (tor.cc:16-28) of the plan's events; deterministic. */
+/* Projection (tor.cc:16-28) of the plan's events; deterministic.  Returns
+ * SRFX_EUNSUPPORTED for plans built without `projection`. */
 int srfx_tor_plan_project(srfx_tor_plan plan, const float *image, float *proj, uint32_t flags,
                           void *stream);
 

@@ -59,9 +59,11 @@ class SrfxTorModel(ctypes.Structure):
 
 class SrfxTile(ctypes.Structure):
     """srfx_tile: tile of `slices` x `wx` x `wy` op-frame voxels, at most
-    `chunk` events per workgroup, `block` threads per workgroup."""
+    `chunk` events per workgroup, `block` threads per workgroup; `projection`
+    keeps the tables of the tiled projection."""
     _fields_ = [("slices", ctypes.c_int32), ("wx", ctypes.c_int32), ("wy", ctypes.c_int32),
-                ("chunk", ctypes.c_int32), ("block", ctypes.c_int32)]
+                ("chunk", ctypes.c_int32), ("block", ctypes.c_int32),
+                ("projection", ctypes.c_int32)]
 
 
 class SrfxError(RuntimeError):

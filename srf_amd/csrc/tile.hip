@@ -80,53 +80,124 @@ __device__ __forceinline__ void for_each_tile(const TorParams &p, const TileGeom
     }
 }
 
-__device__ __forceinline__ bool plan_event(const TorParams &p, int cols7, int64_t e, Event &ev)
+struct PlanArgs {
+    int cols7, ntiles;
+    int *tile_cnt;        // count pass: listings per tile; fill pass: read
+    int *ev_cnt;          // count pass: listings per event (nullable)
+    float4 *rec;          // count pass writes, fill pass reads: 48-byte event records
+    const int *ev_off;    // fill pass: first listing of each event (nullable)
+    const int *tile_off;  // fill pass: first slot of each tile
+    int *tile_fill;       // fill pass: slots taken per tile
+    int *list;            // fill pass: event of each slot
+    int *ev_slot;         // fill pass: slot of each of an event's listings (nullable)
+    int *err;             // fill pass: count of listings that did not fit
+};
+
+__device__ __forceinline__ bool plan_init(const TorParams &p, int cols7, const RawEvent &r,
+                                          Event &ev)
 {
-    return cols7 ? load_event<kMap>(p, e, ev) : load_event<kProj>(p, e, ev);
+    return cols7 ? init_event<kMap>(p, r, ev) : init_event<kProj>(p, r, ev);
 }
 
-// Count pass (FILL false): ev_cnt[e] = number of tiles of event e and
-// tile_cnt[t] += 1 per listing.  Fill pass (FILL true): list the event in
-// each of its tiles and record the slots in its own listing order.  Both
-// passes run the same code; the fill pass still checks every slot against
-// the counted ranges and reports a mismatch in *err instead of writing out of
+// Count pass (FILL false): pack the event's record, count its tiles per
+// event and per tile.  Fill pass (FILL true): list the event in each of its
+// tiles.  With LDS, a workgroup first counts its own listings per tile in
+// LDS, then adds them to the global counters (count pass) or reserves a
+// contiguous range of slots per tile with one global atomic (fill pass), so
+// the global atomics are per (workgroup, tile) instead of per listing and a
+// workgroup's slots of one tile are contiguous.  Both passes enumerate the
+// tiles with the same code; the fill pass checks every slot against the
+// counted ranges and reports a mismatch in *err instead of writing out of
 // range.
-template <bool FILL>
-__global__ __launch_bounds__(256) void plan_pass(TorParams p, TileGeom g, int cols7, int *ev_cnt,
-                                                 int *tile_cnt, const int *ev_off,
-                                                 const int *tile_off, int *tile_fill, int *list,
-                                                 int *ev_slot, int *err)
+template <bool FILL, bool LDS>
+__global__ __launch_bounds__(256) void plan_pass(TorParams p, TileGeom g, PlanArgs a)
 {
-    int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= p.n)
-        return;
+    extern __shared__ int sh[]; // LDS: counts [ntiles], FILL: + bases, ranks [ntiles] each
+    const int nt = a.ntiles;
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = e < p.n;
+    if (LDS) {
+        for (int i = threadIdx.x; i < (FILL ? 3 : 1) * nt; i += blockDim.x)
+            sh[i] = 0;
+        __syncthreads();
+    }
+    RawEvent r;
     Event ev;
+    Track t;
+    bool ok = false;
+    if (valid) {
+        if (FILL) {
+            fetch_event_rec(a.rec, e, r);
+        } else {
+            if (a.cols7)
+                fetch_event<kMap>(p, e, r);
+            else
+                fetch_event<kProj>(p, e, r);
+            a.rec[3 * e] = make_float4(r.x1, r.y1, r.z1, r.x2);
+            a.rec[3 * e + 1] = make_float4(r.y2, r.z2, r.xc, r.yc);
+            a.rec[3 * e + 2] = make_float4(r.zc, r.f, 0.0f, 0.0f);
+        }
+        ok = plan_init(p, a.cols7, r, ev);
+        if (ok)
+            make_track(p, ev, t);
+    }
+    if (!FILL) {
+        int k = 0;
+        if (ok)
+            for_each_tile(p, g, ev, t, [&](int tile) {
+                atomicAdd(LDS ? &sh[tile] : &a.tile_cnt[tile], 1);
+                k++;
+            });
+        if (valid && a.ev_cnt)
+            a.ev_cnt[e] = k;
+        if (LDS) {
+            __syncthreads();
+            for (int i = threadIdx.x; i < nt; i += blockDim.x) {
+                int c = sh[i];
+                if (c)
+                    atomicAdd(&a.tile_cnt[i], c);
+            }
+        }
+        return;
+    }
+    if (LDS) {
+        if (ok)
+            for_each_tile(p, g, ev, t, [&](int tile) { atomicAdd(&sh[tile], 1); });
+        __syncthreads();
+        for (int i = threadIdx.x; i < nt; i += blockDim.x) {
+            int c = sh[i];
+            sh[nt + i] = c ? atomicAdd(&a.tile_fill[i], c) : 0;
+        }
+        __syncthreads();
+    }
+    const int k0 = (valid && a.ev_slot) ? a.ev_off[e] : 0;
+    const int k1 = (valid && a.ev_slot) ? a.ev_off[e + 1] : 0;
     int k = 0;
-    const int k0 = FILL ? ev_off[e] : 0;
-    const int k1 = FILL ? ev_off[e + 1] : 0;
-    if (plan_event(p, cols7, e, ev)) {
-        Track t;
-        make_track(p, ev, t);
+    if (ok)
         for_each_tile(p, g, ev, t, [&](int tile) {
-            if (!FILL) {
-                atomicAdd(&tile_cnt[tile], 1);
+            int rank;
+            bool fits;
+            if (LDS) {
+                int loc = atomicAdd(&sh[2 * nt + tile], 1);
+                rank = sh[nt + tile] + loc;
+                fits = loc < sh[tile];
             } else {
-                int r = atomicAdd(&tile_fill[tile], 1);
-                if (r < tile_cnt[tile] && k0 + k < k1) {
-                    int slot = tile_off[tile] + r;
-                    list[slot] = (int)e;
-                    ev_slot[k0 + k] = slot;
-                } else {
-                    atomicAdd(err, 1);
-                }
+                rank = atomicAdd(&a.tile_fill[tile], 1);
+                fits = true;
+            }
+            fits = fits && rank < a.tile_cnt[tile] && (!a.ev_slot || k0 + k < k1);
+            if (fits) {
+                int slot = a.tile_off[tile] + rank;
+                a.list[slot] = (int)e;
+                if (a.ev_slot)
+                    a.ev_slot[k0 + k] = slot;
+            } else {
+                atomicAdd(a.err, 1);
             }
             k++;
         });
-    }
-    if (!FILL)
-        ev_cnt[e] = k;
-    else if (k0 + k != k1)
-        atomicAdd(err, 1);
+    if (valid && a.ev_slot && k0 + k != k1)
+        atomicAdd(a.err, 1);
 }
 
 __global__ __launch_bounds__(256) void fill_int(int *v, int64_t n, int value)
@@ -222,7 +293,8 @@ __device__ __forceinline__ float tile_event(const TorParams &p, const TileGeom &
 template <int MODE>
 __global__ __launch_bounds__(256) void tile_scatter(TorParams p, TileGeom g,
                                                     const TileChunk *__restrict__ chunks,
-                                                    const int *__restrict__ list)
+                                                    const int *__restrict__ list,
+                                                    const float4 *__restrict__ rec)
 {
     extern __shared__ float lds[];
     const TileChunk ch = chunks[blockIdx.x];
@@ -246,9 +318,13 @@ __global__ __launch_bounds__(256) void tile_scatter(TorParams p, TileGeom g,
             scale = p.val[e];
             go = scale != 0.0f;
         }
-        Event ev;
-        if (go && load_event<MODE>(p, e, ev))
-            tile_event<1>(p, g, b, ev, tof, scale, lds);
+        if (go) {
+            RawEvent r;
+            Event ev;
+            fetch_event_rec(rec, e, r);
+            if (init_event<MODE>(p, r, ev))
+                tile_event<1>(p, g, b, ev, tof, scale, lds);
+        }
     }
     __syncthreads();
     for (int i = threadIdx.x; i < nv; i += blockDim.x) {
@@ -266,7 +342,8 @@ __global__ __launch_bounds__(256) void tile_scatter(TorParams p, TileGeom g,
 // Gather of one chunk: partial[slot] = the listing's projection over the tile.
 __global__ __launch_bounds__(256) void tile_gather(TorParams p, TileGeom g,
                                                    const TileChunk *__restrict__ chunks,
-                                                   const int *__restrict__ list, float *partial)
+                                                   const int *__restrict__ list,
+                                                   const float4 *__restrict__ rec, float *partial)
 {
     extern __shared__ float lds[];
     const TileChunk ch = chunks[blockIdx.x];
@@ -285,9 +362,11 @@ __global__ __launch_bounds__(256) void tile_gather(TorParams p, TileGeom g,
     const bool tof = p.use_tof;
     for (int j = threadIdx.x; j < ch.count; j += blockDim.x) {
         const int64_t e = list[ch.start + j];
+        RawEvent r;
         Event ev;
+        fetch_event_rec(rec, e, r);
         float acc = 0.0f;
-        if (load_event<kProj>(p, e, ev))
+        if (init_event<kProj>(p, r, ev))
             acc = tile_event<0>(p, g, b, ev, tof, 0.0f, lds);
         partial[ch.start + j] = acc;
     }
@@ -321,9 +400,12 @@ struct srfx_tor_plan_s {
     int64_t nlist;
     int nchunks;
     int block;
-    int *list, *ev_slot, *ev_off;
+    int *list;             // event of each slot, tiles in order
+    float4 *rec;           // 48-byte record of each event
     TileChunk *chunks;
-    float *partial, *proj;
+    float *proj;           // projection buffer of the OSEM step
+    int *ev_off, *ev_slot; // tiled projection only: each event's slots
+    float *partial;        // tiled projection only: per-slot partial sums
 };
 
 namespace {
@@ -346,11 +428,12 @@ void plan_free(srfx_tor_plan_s *pl)
     if (!pl)
         return;
     (void)hipFree(pl->list);
-    (void)hipFree(pl->ev_slot);
-    (void)hipFree(pl->ev_off);
+    (void)hipFree(pl->rec);
     (void)hipFree(pl->chunks);
-    (void)hipFree(pl->partial);
     (void)hipFree(pl->proj);
+    (void)hipFree(pl->ev_off);
+    (void)hipFree(pl->ev_slot);
+    (void)hipFree(pl->partial);
     delete pl;
 }
 
@@ -359,33 +442,55 @@ unsigned lds_bytes(const TileGeom &g)
     return (unsigned)(sizeof(float) * g.S * g.Wx * g.Wy);
 }
 
-int plan_build(srfx_tor_plan_s *pl, hipStream_t st, int chunk)
+// Largest tile count whose per-workgroup counters (3 per tile) fit in 64 KB.
+constexpr int kLdsTiles = 16384 / 3;
+
+template <bool FILL>
+void launch_pass(const TorParams &p, const TileGeom &g, const PlanArgs &a, hipStream_t st)
+{
+    const dim3 grid(blocks_for(p.n, 256)), block(256);
+    if (a.ntiles <= kLdsTiles)
+        hipLaunchKernelGGL((plan_pass<FILL, true>), grid, block,
+                           (FILL ? 3u : 1u) * sizeof(int) * (unsigned)a.ntiles, st, p, g, a);
+    else
+        hipLaunchKernelGGL((plan_pass<FILL, false>), grid, block, 0, st, p, g, a);
+}
+
+// Count, scan (host: per tile; device: per event when slots are kept), fill.
+int plan_build(srfx_tor_plan_s *pl, hipStream_t st, int chunk, bool keep_slots)
 {
     TorParams &p = pl->p;
     const TileGeom &g = pl->g;
     const int64_t n = p.n;
     const int ntiles = g.nslab * g.nwx * g.nwy;
-    int *tile_cnt = nullptr, *ev_cnt = nullptr, *tile_off_d = nullptr;
+    int *cnt_d = nullptr, *ev_cnt = nullptr, *tile_off_d = nullptr;
     void *tmp = nullptr;
     int rc = SRFX_OK;
     std::vector<int> cnt(ntiles), off(ntiles);
     std::vector<TileChunk> ch;
     int64_t total = 0;
     size_t tmp_bytes = 0;
-    int last_cnt = 0, last_off = 0, mismatch = 0;
-    if ((rc = dev_alloc(&tile_cnt, 2 * (size_t)ntiles + 1)) || (rc = dev_alloc(&ev_cnt, n)) ||
-        (rc = dev_alloc(&pl->ev_off, n + 1)) || (rc = dev_alloc(&tile_off_d, ntiles)))
+    int last_cnt = 0, last_off = 0, mismatch = 0, total32 = 0;
+    PlanArgs a{};
+    // cnt_d: tile counts [ntiles], tile fill [ntiles], error count [1]
+    if ((rc = dev_alloc(&cnt_d, 2 * (size_t)ntiles + 1)) || (rc = dev_alloc(&pl->rec, 3 * n)) ||
+        (rc = dev_alloc(&tile_off_d, ntiles)) || (rc = dev_alloc(&pl->proj, n)))
+        goto done;
+    if (keep_slots && ((rc = dev_alloc(&ev_cnt, n)) || (rc = dev_alloc(&pl->ev_off, n + 1))))
         goto done;
     hipLaunchKernelGGL(fill_int, dim3(blocks_for(2 * (int64_t)ntiles + 1, 256)), dim3(256), 0,
-                       st, tile_cnt, 2 * (int64_t)ntiles + 1, 0);
+                       st, cnt_d, 2 * (int64_t)ntiles + 1, 0);
+    a.cols7 = pl->cols7;
+    a.ntiles = ntiles;
+    a.tile_cnt = cnt_d;
+    a.ev_cnt = ev_cnt;
+    a.rec = pl->rec;
     if (n > 0)
-        hipLaunchKernelGGL(plan_pass<false>, dim3(blocks_for(n, 256)), dim3(256), 0, st, p, g,
-                           pl->cols7, ev_cnt, tile_cnt, nullptr, nullptr, nullptr, nullptr,
-                           nullptr, nullptr);
+        launch_pass<false>(p, g, a, st);
     if ((rc = launch_status()))
         goto done;
-    // ev_off = exclusive scan of the per-event listing counts
-    if (n > 0) {
+    if (keep_slots && n > 0) {
+        // ev_off = exclusive scan of the per-event listing counts
         if (rocprim::exclusive_scan(nullptr, tmp_bytes, ev_cnt, pl->ev_off, 0, (size_t)n,
                                     rocprim::plus<int>(), st) != hipSuccess) {
             rc = SRFX_ELAUNCH;
@@ -394,17 +499,17 @@ int plan_build(srfx_tor_plan_s *pl, hipStream_t st, int chunk)
         if ((rc = dev_alloc((char **)&tmp, tmp_bytes)))
             goto done;
         if (rocprim::exclusive_scan(tmp, tmp_bytes, ev_cnt, pl->ev_off, 0, (size_t)n,
-                                    rocprim::plus<int>(), st) != hipSuccess) {
+                                    rocprim::plus<int>(), st) != hipSuccess ||
+            hipMemcpyAsync(&last_cnt, ev_cnt + (n - 1), sizeof(int), hipMemcpyDeviceToHost,
+                           st) != hipSuccess ||
+            hipMemcpyAsync(&last_off, pl->ev_off + (n - 1), sizeof(int), hipMemcpyDeviceToHost,
+                           st) != hipSuccess) {
             rc = SRFX_ELAUNCH;
             goto done;
         }
     }
-    if (hipMemcpyAsync(cnt.data(), tile_cnt, sizeof(int) * ntiles, hipMemcpyDeviceToHost, st) !=
+    if (hipMemcpyAsync(cnt.data(), cnt_d, sizeof(int) * ntiles, hipMemcpyDeviceToHost, st) !=
             hipSuccess ||
-        (n > 0 && (hipMemcpyAsync(&last_cnt, ev_cnt + (n - 1), sizeof(int),
-                                  hipMemcpyDeviceToHost, st) != hipSuccess ||
-                   hipMemcpyAsync(&last_off, pl->ev_off + (n - 1), sizeof(int),
-                                  hipMemcpyDeviceToHost, st) != hipSuccess)) ||
         hipStreamSynchronize(st) != hipSuccess) {
         rc = SRFX_ELAUNCH;
         goto done;
@@ -412,48 +517,47 @@ int plan_build(srfx_tor_plan_s *pl, hipStream_t st, int chunk)
     for (int t = 0; t < ntiles; t++) {
         off[t] = (int)total;
         total += cnt[t];
+        if (total > (int64_t)INT32_MAX - 1)
+            break;
     }
-    if (total > (int64_t)INT32_MAX - 1 || (n > 0 && (int64_t)last_off + last_cnt != total)) {
+    if (total > (int64_t)INT32_MAX - 1 ||
+        (keep_slots && n > 0 && (int64_t)last_off + last_cnt != total)) {
         rc = SRFX_EUNSUPPORTED;
         goto done;
     }
-    {
-        int total32 = (int)total;
-        if (hipMemcpyAsync(pl->ev_off + n, &total32, sizeof(int), hipMemcpyHostToDevice, st) !=
-            hipSuccess) {
-            rc = SRFX_ELAUNCH;
-            goto done;
-        }
-        // the copy reads a host stack variable: finish it before leaving scope
-        if (hipStreamSynchronize(st) != hipSuccess) {
-            rc = SRFX_ELAUNCH;
-            goto done;
-        }
-    }
+    total32 = (int)total;
     for (int t = 0; t < ntiles; t++)
         for (int s = 0; s < cnt[t]; s += chunk)
             ch.push_back(TileChunk{t, off[t] + s, cnt[t] - s < chunk ? cnt[t] - s : chunk, 0});
     pl->nlist = total;
     pl->nchunks = (int)ch.size();
-    if ((rc = dev_alloc(&pl->list, total)) || (rc = dev_alloc(&pl->ev_slot, total)) ||
-        (rc = dev_alloc(&pl->partial, total)) || (rc = dev_alloc(&pl->proj, n)) ||
-        (rc = dev_alloc(&pl->chunks, ch.size())))
+    if ((rc = dev_alloc(&pl->list, total)) || (rc = dev_alloc(&pl->chunks, ch.size())))
+        goto done;
+    if (keep_slots && ((rc = dev_alloc(&pl->ev_slot, total)) ||
+                       (rc = dev_alloc(&pl->partial, total))))
         goto done;
     if (hipMemcpyAsync(tile_off_d, off.data(), sizeof(int) * ntiles, hipMemcpyHostToDevice, st) !=
             hipSuccess ||
         (!ch.empty() && hipMemcpyAsync(pl->chunks, ch.data(), sizeof(TileChunk) * ch.size(),
-                                       hipMemcpyHostToDevice, st) != hipSuccess)) {
+                                       hipMemcpyHostToDevice, st) != hipSuccess) ||
+        (keep_slots && hipMemcpyAsync(pl->ev_off + n, &total32, sizeof(int),
+                                      hipMemcpyHostToDevice, st) != hipSuccess)) {
         rc = SRFX_ELAUNCH;
         goto done;
     }
+    a.ev_cnt = nullptr;
+    a.ev_off = keep_slots ? pl->ev_off : nullptr;
+    a.tile_off = tile_off_d;
+    a.tile_fill = cnt_d + ntiles;
+    a.list = pl->list;
+    a.ev_slot = keep_slots ? pl->ev_slot : nullptr;
+    a.err = cnt_d + 2 * ntiles;
     if (n > 0)
-        hipLaunchKernelGGL(plan_pass<true>, dim3(blocks_for(n, 256)), dim3(256), 0, st, p, g,
-                           pl->cols7, nullptr, tile_cnt, pl->ev_off, tile_off_d,
-                           tile_cnt + ntiles, pl->list, pl->ev_slot, tile_cnt + 2 * ntiles);
+        launch_pass<true>(p, g, a, st);
     if ((rc = launch_status()))
         goto done;
-    if (hipMemcpyAsync(&mismatch, tile_cnt + 2 * ntiles, sizeof(int), hipMemcpyDeviceToHost,
-                       st) != hipSuccess ||
+    if (hipMemcpyAsync(&mismatch, cnt_d + 2 * ntiles, sizeof(int), hipMemcpyDeviceToHost, st) !=
+            hipSuccess ||
         hipStreamSynchronize(st) != hipSuccess) {
         rc = SRFX_ELAUNCH;
         goto done;
@@ -461,8 +565,9 @@ int plan_build(srfx_tor_plan_s *pl, hipStream_t st, int chunk)
     if (mismatch)
         rc = SRFX_ELAUNCH; // count and fill passes disagreed: no valid plan
 done:
+    // host buffers above are read by the copies: finish them before returning
     (void)hipStreamSynchronize(st);
-    (void)hipFree(tile_cnt);
+    (void)hipFree(cnt_d);
     (void)hipFree(ev_cnt);
     (void)hipFree(tile_off_d);
     (void)hipFree(tmp);
@@ -488,15 +593,15 @@ int plan_scatter(srfx_tor_plan_s *pl, int mode, const float *vals, float *image,
     switch (mode) {
     case kBack:
         hipLaunchKernelGGL(tile_scatter<kBack>, grid, block, sh, st, p, pl->g, pl->chunks,
-                           pl->list);
+                           pl->list, pl->rec);
         break;
     case kAdj:
         hipLaunchKernelGGL(tile_scatter<kAdj>, grid, block, sh, st, p, pl->g, pl->chunks,
-                           pl->list);
+                           pl->list, pl->rec);
         break;
     default:
         hipLaunchKernelGGL(tile_scatter<kMap>, grid, block, sh, st, p, pl->g, pl->chunks,
-                           pl->list);
+                           pl->list, pl->rec);
         break;
     }
     (void)flags;
@@ -507,11 +612,13 @@ int plan_project(srfx_tor_plan_s *pl, const float *image, float *proj, hipStream
 {
     TorParams p = pl->p;
     p.img = image;
+    if (!pl->ev_slot)
+        return SRFX_EUNSUPPORTED; // plan built without the tiled projection tables
     if (p.n == 0)
         return SRFX_OK;
     if (pl->nchunks > 0)
         hipLaunchKernelGGL(tile_gather, dim3(pl->nchunks), dim3(pl->block), lds_bytes(pl->g), st,
-                           p, pl->g, pl->chunks, pl->list, pl->partial);
+                           p, pl->g, pl->chunks, pl->list, pl->rec, pl->partial);
     hipLaunchKernelGGL(tile_reduce, dim3(blocks_for(p.n, 256)), dim3(256), 0, st, pl->partial,
                        pl->ev_off, pl->ev_slot, p.n, proj);
     return launch_status();
@@ -558,7 +665,7 @@ extern "C" int srfx_tor_plan_create(const float *lors, int64_t n, int64_t ld, in
         delete pl;
         return SRFX_EUNSUPPORTED;
     }
-    rc = plan_build(pl, (hipStream_t)stream, tile.chunk);
+    rc = plan_build(pl, (hipStream_t)stream, tile.chunk, tile.projection != 0);
     if (rc) {
         plan_free(pl);
         return rc;

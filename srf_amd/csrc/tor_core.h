@@ -121,27 +121,72 @@ __device__ __forceinline__ bool slice_range(const TorParams &p, Event &ev)
     return ev.s_lo <= ev.s_hi;
 }
 
+// Raw event record: end points, TOF centre (zero for Maplors rows) and the
+// sigma2 factor.
+struct RawEvent {
+    float x1, y1, z1, x2, y2, z2, xc, yc, zc, f;
+};
+
 template <int MODE>
-__device__ __forceinline__ bool load_event(const TorParams &p, int64_t e, Event &ev)
+__device__ __forceinline__ void fetch_event(const TorParams &p, int64_t e, RawEvent &r)
 {
     const float *L = p.lors;
     const int64_t ld = p.ld;
-    float x1 = __builtin_nontemporal_load(L + 0 * ld + e);
-    float y1 = __builtin_nontemporal_load(L + 1 * ld + e);
-    float z1 = __builtin_nontemporal_load(L + 2 * ld + e);
-    float x2 = __builtin_nontemporal_load(L + 3 * ld + e);
-    float y2 = __builtin_nontemporal_load(L + 4 * ld + e);
-    float z2 = __builtin_nontemporal_load(L + 5 * ld + e);
-    float f;
+    r.x1 = __builtin_nontemporal_load(L + 0 * ld + e);
+    r.y1 = __builtin_nontemporal_load(L + 1 * ld + e);
+    r.z1 = __builtin_nontemporal_load(L + 2 * ld + e);
+    r.x2 = __builtin_nontemporal_load(L + 3 * ld + e);
+    r.y2 = __builtin_nontemporal_load(L + 4 * ld + e);
+    r.z2 = __builtin_nontemporal_load(L + 5 * ld + e);
     if (MODE == kMap) {
-        f = __builtin_nontemporal_load(L + 6 * ld + e);
-        ev.xc = ev.yc = ev.zc = 0.0f;
+        r.f = __builtin_nontemporal_load(L + 6 * ld + e);
+        r.xc = r.yc = r.zc = 0.0f;
     } else {
-        ev.xc = __builtin_nontemporal_load(L + 6 * ld + e);
-        ev.yc = __builtin_nontemporal_load(L + 7 * ld + e);
-        ev.zc = __builtin_nontemporal_load(L + 8 * ld + e);
-        f = __builtin_nontemporal_load(L + 9 * ld + e);
+        r.xc = __builtin_nontemporal_load(L + 6 * ld + e);
+        r.yc = __builtin_nontemporal_load(L + 7 * ld + e);
+        r.zc = __builtin_nontemporal_load(L + 8 * ld + e);
+        r.f = __builtin_nontemporal_load(L + 9 * ld + e);
     }
+}
+
+// The same record from an array of 48-byte records
+// [x1 y1 z1 x2][y2 z2 xc yc][zc f - -] (three 16-byte loads).
+__device__ __forceinline__ void fetch_event_rec(const float4 *__restrict__ rec, int64_t e,
+                                                RawEvent &r)
+{
+    const float4 a = rec[3 * e], b = rec[3 * e + 1], c = rec[3 * e + 2];
+    r.x1 = a.x;
+    r.y1 = a.y;
+    r.z1 = a.z;
+    r.x2 = a.w;
+    r.y2 = b.x;
+    r.z2 = b.y;
+    r.xc = b.z;
+    r.yc = b.w;
+    r.zc = c.x;
+    r.f = c.y;
+}
+
+template <int MODE>
+__device__ __forceinline__ bool init_event(const TorParams &p, const RawEvent &r, Event &ev);
+
+template <int MODE>
+__device__ __forceinline__ bool load_event(const TorParams &p, int64_t e, Event &ev)
+{
+    RawEvent r;
+    fetch_event<MODE>(p, e, r);
+    return init_event<MODE>(p, r, ev);
+}
+
+// Per-event set-up (tor.cu.cc:18-40 hoisted out of the slice loop).
+template <int MODE>
+__device__ __forceinline__ bool init_event(const TorParams &p, const RawEvent &r, Event &ev)
+{
+    const float x1 = r.x1, y1 = r.y1, z1 = r.z1, x2 = r.x2, y2 = r.y2, z2 = r.z2;
+    const float f = r.f;
+    ev.xc = r.xc;
+    ev.yc = r.yc;
+    ev.zc = r.zc;
     ev.x1 = x1;
     ev.y1 = y1;
     ev.z1 = z1;

@@ -27,7 +27,8 @@ DEFAULT_CHUNK = 4096
 class TorPlan:
     def __init__(self, lors, grid, center, size, kernel_width, geo_sigma2_flag=False,
                  tof_bin=1.0e4, tof_sigma2=1.0e4, strides=(0, 0, 0), ncols=10,
-                 tile=DEFAULT_TILE, chunk=DEFAULT_CHUNK, block=256, axis=None):
+                 tile=DEFAULT_TILE, chunk=DEFAULT_CHUNK, block=256, axis=None,
+                 tiled_projection=False):
         if ncols not in (7, 10):
             raise ValueError("ncols must be 10 (TOF rows) or 7 (Maplors rows)")
         self.lors, self.n, ld = _soa(lors, ncols)
@@ -38,7 +39,9 @@ class TorPlan:
         self.ncols = ncols
         self.tile = tuple(int(t) for t in tile)
         model = _lib.make_tor_model(kernel_width, geo_sigma2_flag, tof_bin, tof_sigma2)
-        t = _lib.SrfxTile(self.tile[0], self.tile[1], self.tile[2], int(chunk), int(block))
+        self.tiled_projection = bool(tiled_projection)
+        t = _lib.SrfxTile(self.tile[0], self.tile[1], self.tile[2], int(chunk), int(block),
+                          1 if tiled_projection else 0)
         h = ctypes.c_void_p()
         _lib.call("srfx_tor_plan_create", _ptr(self.lors), self.n, ld, ncols, self._cgrid, model,
                   t, ctypes.byref(h), _stream())

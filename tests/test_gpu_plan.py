@@ -16,8 +16,10 @@ from tests import helpers as H
 pytestmark = pytest.mark.gpu
 
 TOL64 = 1e-5
-# (slices, wx, wy), chunk
-TILES = [((8, 32, 32), 4096), ((1, 4, 4), 64), ((3, 5, 7), 97), ((16, 16, 32), 1024)]
+# (slices, wx, wy), chunk.  (1, 2, 2) has more tiles than the per-workgroup
+# LDS counters of the plan build hold (global-atomic build path).
+TILES = [((8, 32, 32), 4096), ((1, 4, 4), 64), ((3, 5, 7), 97), ((16, 16, 32), 1024),
+         ((1, 2, 2), 50)]
 
 
 def _t(a, dev):
@@ -32,11 +34,11 @@ def case(cuda):
     return lors, img
 
 
-def _plan(L, geo, tof, tile, chunk, ncols=10):
+def _plan(L, geo, tof, tile, chunk, ncols=10, tiled_projection=True):
     from srf_amd.plan import TorPlan
     ts2 = H.TOF_SIGMA2 if tof else 1.0e6
     return TorPlan(L, H.TOR_GRID, H.TOR_CENTER, H.TOR_SIZE, H.KW, geo, H.TOF_BIN, ts2,
-                   ncols=ncols, tile=tile, chunk=chunk)
+                   ncols=ncols, tile=tile, chunk=chunk, tiled_projection=tiled_projection)
 
 
 @pytest.mark.parametrize("tile,chunk", TILES)
@@ -79,7 +81,7 @@ def test_plan_osem_subset(cuda, case, tiled_projection):
     lors, img = case
     args = (H.TOR_GRID, H.TOR_CENTER, H.TOR_SIZE, H.KW, True, H.TOF_BIN, H.TOF_SIGMA2)
     L, X = _t(lors, cuda), _t(img, cuda)
-    plan = _plan(L, True, True, (4, 16, 16), 512)
+    plan = _plan(L, True, True, (4, 16, 16), 512, tiled_projection=tiled_projection)
     f = plan.osem_subset(X, tiled_projection=tiled_projection).cpu().numpy()
     ref = O.tor_project(lors, img, *args, prec="f64")
     keep = ref > 1e-7
@@ -112,10 +114,32 @@ def test_plan_empty(cuda):
     import torch
     from srf_amd.plan import TorPlan
     L = torch.zeros(10, 0, device=cuda)
-    plan = TorPlan(L, H.TOR_GRID, H.TOR_CENTER, H.TOR_SIZE, H.KW)
+    plan = TorPlan(L, H.TOR_GRID, H.TOR_CENTER, H.TOR_SIZE, H.KW, tiled_projection=True)
     assert plan.info() == (0, 0)
     X = torch.ones(H.TOR_GRID[::-1], device=cuda)
     assert plan.projection(X).numel() == 0
     b = plan.backprojection(torch.zeros(0, device=cuda))
     assert float(b.abs().sum()) == 0.0
+    plan.close()
+
+
+def test_plan_without_projection_tables(cuda, case):
+    """A plan built for scatters only refuses the tiled projection and still
+    backprojects like the untiled op."""
+    from srf_amd import ops
+    from srf_amd._lib import SRFX_EUNSUPPORTED, SrfxError
+    lors, img = case
+    L, X = _t(lors, cuda), _t(img, cuda)
+    plan = _plan(L, False, True, (8, 16, 16), 1024, tiled_projection=False)
+    with pytest.raises(SrfxError) as ei:
+        plan.projection(X)
+    assert ei.value.status == SRFX_EUNSUPPORTED
+    with pytest.raises(SrfxError):
+        plan.osem_subset(X, tiled_projection=True)
+    p = ops.projection(L, X, H.TOR_GRID, H.TOR_CENTER, H.TOR_SIZE, H.KW, False, H.TOF_BIN,
+                       H.TOF_SIGMA2)
+    b0 = ops.backprojection(X, L, p, H.TOR_GRID, H.TOR_CENTER, H.TOR_SIZE, H.TOF_BIN,
+                            H.TOF_SIGMA2, H.KW, False)
+    b1 = plan.backprojection(p)
+    assert H.rel_l2(b1.cpu().numpy(), b0.cpu().numpy()) <= 1e-6
     plan.close()
