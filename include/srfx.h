@@ -194,7 +194,8 @@ int srfx_siddon_count_nnz(const float *lors, int64_t n, int64_t ld, srfx_grid gr
  * touch; scatters then accumulate per tile in LDS and flush each tile with
  * coalesced atomics, and the tiled projection gathers from tiles staged in
  * LDS.  `chunk` caps the events one workgroup takes from a tile; `block` is
- * the workgroup size (64, 128, 256; 0 = 256).  slices * wx * wy <= 16384.
+ * the workgroup size (64, 128, 256; 0 = 256).  slices * wx * wy <= 8192
+ * (the scatter's tile is FP64 in LDS, 64 KB at most).
  * The tiled projection (srfx_tor_plan_project, SRFX_TILED_PROJECTION) needs a
  * plan built with `projection` set (8 more bytes per listing). */
 typedef struct srfx_tile {

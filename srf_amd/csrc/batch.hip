@@ -92,15 +92,12 @@ __global__ __launch_bounds__(256) void batch_osem(const TorParams *__restrict__ 
         const TorParams p = jobs[__builtin_amdgcn_readfirstlane(ch.job)];
         const bool valid = (int)threadIdx.x < ch.count;
         const int64_t e = (int64_t)ch.start + threadIdx.x;
-        switch (p.patch) {
+        switch (p.box) {
         case 3:
             run_event<kOsem, 3, 1>(p, e, 0, valid);
             break;
         case 4:
             run_event<kOsem, 4, 1>(p, e, 0, valid);
-            break;
-        case 5:
-            run_event<kOsem, 5, 1>(p, e, 0, valid);
             break;
         default:
             run_event<kOsem, 0, 1>(p, e, 0, valid);

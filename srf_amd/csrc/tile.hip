@@ -7,16 +7,18 @@
 // small footprint and it touches few tiles.  The plan lists, for every tile,
 // the events whose footprint meets it ("listings"), cut into chunks of at most
 // `chunk` listings; one workgroup takes one chunk:
-//   scatter: zero the tile in LDS, every lane walks one listing's voxels that
-//            lie in the tile and adds its weights with LDS atomics, then the
-//            tile is flushed with one coalesced global atomic per non-zero
-//            voxel (instead of one scattered global atomic per weight);
+//   scatter: zero the tile (FP64) in LDS, every lane walks one listing's
+//            voxels that lie in the tile and adds its weights with LDS
+//            atomics, then the tile is flushed with one coalesced FP32 global
+//            atomic per non-zero voxel (instead of one scattered global atomic
+//            per weight);
 //   gather:  load the tile into LDS, every lane sums its listing's weighted
 //            voxels from LDS into one partial per listing; a second pass sums
 //            each event's partials in the event's own listing order, so the
 //            projection is deterministic.
 // The voxels an event visits and their weights are those of the untiled
-// kernels (tor_core.h): the tile walk is the chord walk clipped to the window.
+// kernels (tor_core.h): the tile walk is the candidate-box (or chord) walk
+// clipped to the window.
 #include <rocprim/device/device_scan.hpp>
 
 #include <vector>
@@ -229,11 +231,11 @@ __device__ __forceinline__ TileBox tile_box(const TorParams &p, const TileGeom &
 
 // The chord walk of one slice (patch_walk) clipped to the tile's window.
 //   KIND 0: returns sum w * lds[voxel]; KIND 1: lds[voxel] += w * scale.
-template <int KIND>
+template <int KIND, typename T>
 __device__ __forceinline__ float window_walk(const TorParams &p, const TileGeom &g,
                                              const TileBox &b, const Event &ev, const Track &t,
                                              const Cross &c, float amp, int s, float scale,
-                                             float *lds)
+                                             T *lds)
 {
     const int P = p.patch;
     int i_lo = max(0, b.x0 - c.ix0), i_hi = min(P - 1, b.x1 - c.ix0);
@@ -261,50 +263,78 @@ __device__ __forceinline__ float window_walk(const TorParams &p, const TileGeom 
             if (in) {
                 float w = amp * __builtin_amdgcn_exp2f(d2 * ev.cw);
                 int li = row0 + j * g.Wx + i;
-                if (KIND == 0)
+                if constexpr (KIND == 0)
                     acc = fmaf(w, lds[li], acc);
                 else
-                    atomicAdd(&lds[li], w * scale);
+                    atomicAdd(&lds[li], (T)(w * scale));
             }
         }
     }
     return acc;
 }
 
-template <int KIND>
+// One listing: the event's slices inside the tile, each through the NB x NB
+// candidate box clipped to the window when the event's box fits, else the
+// clipped chord walk.
+template <int KIND, int NB, typename T>
 __device__ __forceinline__ float tile_event(const TorParams &p, const TileGeom &g,
                                             const TileBox &b, const Event &ev, bool tof,
-                                            float scale, float *lds)
+                                            float scale, T *lds)
 {
     Track t;
     make_track(p, ev, t);
+    const bool boxed = NB > 0 && t.nx <= NB && t.ny <= NB;
     const int s0 = max(ev.s_lo, b.sa), s1 = min(ev.s_hi, b.sb);
+    const int P = p.patch;
     float acc = 0.0f;
     for (int s = s0; s <= s1; s++) {
         Cross c;
         cross_at(p, ev, t, s, c);
         float amp = tof ? tof_factor(p, ev, t, s) : 1.0f;
-        acc += window_walk<KIND>(p, g, b, ev, t, c, amp, s, scale, lds);
+        if (NB > 0 && boxed) {
+            PatchIdx idx;
+            idx.base = ((s - b.sa) * g.Wy + (c.iy0 - b.y0)) * g.Wx + (c.ix0 - b.x0);
+            idx.si = 1;
+            idx.sj = g.Wx;
+            const float *src = nullptr;
+            if constexpr (KIND == 0)
+                src = lds;
+            acc += slice_box<KIND, (NB > 0 ? NB : 1), true>(
+                p, ev, t, c, amp, s, scale, max(0, b.x0 - c.ix0), min(P - 1, b.x1 - c.ix0),
+                max(0, b.y0 - c.iy0), min(P - 1, b.y1 - c.iy0), idx, src, lds);
+        } else {
+            acc += window_walk<KIND>(p, g, b, ev, t, c, amp, s, scale, lds);
+        }
     }
     return acc;
 }
 
 // Scatter of one chunk (modes kBack, kMap, kAdj) into p.out.
-template <int MODE>
+template <int MODE, int NB>
 __global__ __launch_bounds__(256) void tile_scatter(TorParams p, TileGeom g,
                                                     const TileChunk *__restrict__ chunks,
                                                     const int *__restrict__ list,
                                                     const float4 *__restrict__ rec)
 {
-    extern __shared__ float lds[];
+    // FP64 tile: LDS FP64 atomic adds run at several lanes per cycle where
+    // FP32 ones are serialised lane by lane, and the tile sums gain precision.
+    extern __shared__ double ldsd[];
     const TileChunk ch = chunks[blockIdx.x];
     const TileBox b = tile_box(p, g, ch.tile);
     const int nv = g.S * g.Wy * g.Wx;
     for (int i = threadIdx.x; i < nv; i += blockDim.x)
-        lds[i] = 0.0f;
+        ldsd[i] = 0.0;
     __syncthreads();
     const bool tof = MODE != kMap && p.use_tof;
-    for (int j = threadIdx.x; j < ch.count; j += blockDim.x) {
+    // The 64 lanes of a wave take listings `stride` apart in the chunk:
+    // neighbours in the list are neighbouring (often near-identical) events,
+    // whose LDS atomics would hit the same voxels in the same instruction
+    // and serialise.
+    const int stride = (ch.count + 63) >> 6;
+    for (int q = threadIdx.x; q < 64 * stride; q += blockDim.x) {
+        const int j = (q & 63) * stride + (q >> 6);
+        if (j >= ch.count)
+            continue;
         const int64_t e = list[ch.start + j];
         float scale;
         bool go = true;
@@ -323,12 +353,12 @@ __global__ __launch_bounds__(256) void tile_scatter(TorParams p, TileGeom g,
             Event ev;
             fetch_event_rec(rec, e, r);
             if (init_event<MODE>(p, r, ev))
-                tile_event<1>(p, g, b, ev, tof, scale, lds);
+                tile_event<1, NB>(p, g, b, ev, tof, scale, ldsd);
         }
     }
     __syncthreads();
     for (int i = threadIdx.x; i < nv; i += blockDim.x) {
-        float v = lds[i];
+        float v = (float)ldsd[i];
         if (v != 0.0f) {
             int lx = i % g.Wx, r = i / g.Wx;
             int ly = r % g.Wy, ls = r / g.Wy;
@@ -340,6 +370,7 @@ __global__ __launch_bounds__(256) void tile_scatter(TorParams p, TileGeom g,
 }
 
 // Gather of one chunk: partial[slot] = the listing's projection over the tile.
+template <int NB>
 __global__ __launch_bounds__(256) void tile_gather(TorParams p, TileGeom g,
                                                    const TileChunk *__restrict__ chunks,
                                                    const int *__restrict__ list,
@@ -367,7 +398,7 @@ __global__ __launch_bounds__(256) void tile_gather(TorParams p, TileGeom g,
         fetch_event_rec(rec, e, r);
         float acc = 0.0f;
         if (init_event<kProj>(p, r, ev))
-            acc = tile_event<0>(p, g, b, ev, tof, 0.0f, lds);
+            acc = tile_event<0, NB>(p, g, b, ev, tof, 0.0f, lds);
         partial[ch.start + j] = acc;
     }
 }
@@ -437,9 +468,15 @@ void plan_free(srfx_tor_plan_s *pl)
     delete pl;
 }
 
+// LDS of the gather (FP32 tile) and scatter (FP64 tile) kernels.
 unsigned lds_bytes(const TileGeom &g)
 {
     return (unsigned)(sizeof(float) * g.S * g.Wx * g.Wy);
+}
+
+unsigned lds_bytes_scatter(const TileGeom &g)
+{
+    return (unsigned)(sizeof(double) * g.S * g.Wx * g.Wy);
 }
 
 // Largest tile count whose per-workgroup counters (3 per tile) fit in 64 KB.
@@ -574,6 +611,26 @@ done:
     return rc;
 }
 
+template <int MODE>
+void launch_scatter(const TorParams &p, const srfx_tor_plan_s *pl, dim3 grid, dim3 block,
+                    unsigned sh, hipStream_t st)
+{
+    switch (p.box) {
+    case 3:
+        hipLaunchKernelGGL((tile_scatter<MODE, 3>), grid, block, sh, st, p, pl->g, pl->chunks,
+                           pl->list, pl->rec);
+        break;
+    case 4:
+        hipLaunchKernelGGL((tile_scatter<MODE, 4>), grid, block, sh, st, p, pl->g, pl->chunks,
+                           pl->list, pl->rec);
+        break;
+    default:
+        hipLaunchKernelGGL((tile_scatter<MODE, 0>), grid, block, sh, st, p, pl->g, pl->chunks,
+                           pl->list, pl->rec);
+        break;
+    }
+}
+
 int plan_scatter(srfx_tor_plan_s *pl, int mode, const float *vals, float *image,
                  const srfx_grid *out_grid, uint32_t flags, hipStream_t st)
 {
@@ -589,19 +646,16 @@ int plan_scatter(srfx_tor_plan_s *pl, int mode, const float *vals, float *image,
     if (pl->nchunks == 0)
         return SRFX_OK;
     const dim3 grid(pl->nchunks), block(pl->block);
-    const unsigned sh = lds_bytes(pl->g);
+    const unsigned sh = lds_bytes_scatter(pl->g);
     switch (mode) {
     case kBack:
-        hipLaunchKernelGGL(tile_scatter<kBack>, grid, block, sh, st, p, pl->g, pl->chunks,
-                           pl->list, pl->rec);
+        launch_scatter<kBack>(p, pl, grid, block, sh, st);
         break;
     case kAdj:
-        hipLaunchKernelGGL(tile_scatter<kAdj>, grid, block, sh, st, p, pl->g, pl->chunks,
-                           pl->list, pl->rec);
+        launch_scatter<kAdj>(p, pl, grid, block, sh, st);
         break;
     default:
-        hipLaunchKernelGGL(tile_scatter<kMap>, grid, block, sh, st, p, pl->g, pl->chunks,
-                           pl->list, pl->rec);
+        launch_scatter<kMap>(p, pl, grid, block, sh, st);
         break;
     }
     (void)flags;
@@ -616,9 +670,24 @@ int plan_project(srfx_tor_plan_s *pl, const float *image, float *proj, hipStream
         return SRFX_EUNSUPPORTED; // plan built without the tiled projection tables
     if (p.n == 0)
         return SRFX_OK;
-    if (pl->nchunks > 0)
-        hipLaunchKernelGGL(tile_gather, dim3(pl->nchunks), dim3(pl->block), lds_bytes(pl->g), st,
-                           p, pl->g, pl->chunks, pl->list, pl->rec, pl->partial);
+    if (pl->nchunks > 0) {
+        const dim3 grid(pl->nchunks), block(pl->block);
+        const unsigned sh = lds_bytes(pl->g);
+        switch (p.box) {
+        case 3:
+            hipLaunchKernelGGL(tile_gather<3>, grid, block, sh, st, p, pl->g, pl->chunks,
+                               pl->list, pl->rec, pl->partial);
+            break;
+        case 4:
+            hipLaunchKernelGGL(tile_gather<4>, grid, block, sh, st, p, pl->g, pl->chunks,
+                               pl->list, pl->rec, pl->partial);
+            break;
+        default:
+            hipLaunchKernelGGL(tile_gather<0>, grid, block, sh, st, p, pl->g, pl->chunks,
+                               pl->list, pl->rec, pl->partial);
+            break;
+        }
+    }
     hipLaunchKernelGGL(tile_reduce, dim3(blocks_for(p.n, 256)), dim3(256), 0, st, pl->partial,
                        pl->ev_off, pl->ev_slot, p.n, proj);
     return launch_status();
@@ -639,8 +708,8 @@ extern "C" int srfx_tor_plan_create(const float *lors, int64_t n, int64_t ld, in
         return SRFX_EUNSUPPORTED;
     if (tile.slices < 1 || tile.wx < 1 || tile.wy < 1 || tile.chunk < 1)
         return SRFX_EINVAL;
-    if ((int64_t)tile.slices * tile.wx * tile.wy > 16384)
-        return SRFX_EUNSUPPORTED; // LDS tile above 64 KB
+    if ((int64_t)tile.slices * tile.wx * tile.wy > 8192)
+        return SRFX_EUNSUPPORTED; // FP64 LDS tile above 64 KB
     if (tile.block != 0 && tile.block != 64 && tile.block != 128 && tile.block != 256)
         return SRFX_EINVAL;
     srfx_tor_plan_s *pl = new srfx_tor_plan_s{};

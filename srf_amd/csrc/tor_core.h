@@ -45,13 +45,14 @@ struct TorParams {
     int sx, sy, sz; // strides of the read image
     int ox, oy, oz; // strides of the written image
     double inter_x, inter_y, inter_z;
-    double rinter_x, rinter_y;
+    double rinter_x, rinter_y, rinter_z;
     double l_bound, b_bound;
     double zbase; // center_z - (lz - inter_z)/2: slice s is at zbase + s*inter_z
     double tor_sigma2;
     float finter_x, finter_y;
     float rfinter_x, rfinter_y;
     int patch, half;
+    int box;        // compiled candidate box (3, 4) or 0 (chord walk only)
     int full_patch; // evaluate every patch voxel (reference loop) instead of the chords
     int geo;
     int use_tof;
@@ -62,8 +63,8 @@ struct TorParams {
 struct Event {
     double x1, y1, z1, d0, d1, d2;
     double rd2;       // 1 / d2
-    double ux_d, uy_d;
-    float ux, uy;
+    double dis2;      // |d|^2
+    float ux, uy;     // FP32 direction cosines (the exact ones are recomputed where needed)
     float xc, yc, zc;
     double sc_d;      // sigma_c^2 = tor_sigma2 (* sigma2 factor)
     float thr;        // FP32 image of 9 sigma_c^2
@@ -93,26 +94,33 @@ __device__ __forceinline__ bool slice_range(const TorParams &p, Event &ev)
     }
     if (!(za > -1.0e30 && zb < 1.0e30))
         return false;
-    double fa = (za - p.zbase) / p.inter_z, fb = (zb - p.zbase) / p.inter_z;
-    if (fb < 0.0 || fa > (double)p.gz)
+    // starting guesses (any rounding is corrected by the exact compares below)
+    double fa = (za - p.zbase) * p.rinter_z, fb = (zb - p.zbase) * p.rinter_z;
+    if (fb < -1.0 || fa > (double)p.gz + 1.0)
         return false;
     int a = (int)fmax(floor(fa), -1.0);
     int b = (int)fmin(ceil(fb), (double)p.gz);
     // first slice strictly above za, last strictly below zb (exact compares)
+#pragma clang loop vectorize(disable) unroll(disable)
     while (a > 0 && slice_z(p, a - 1) > za)
         a--;
+#pragma clang loop vectorize(disable) unroll(disable)
     while (a <= b && !(slice_z(p, a) > za))
         a++;
+#pragma clang loop vectorize(disable) unroll(disable)
     while (b < p.gz - 1 && slice_z(p, b + 1) < zb)
         b++;
+#pragma clang loop vectorize(disable) unroll(disable)
     while (b >= a && !(slice_z(p, b) < zb))
         b--;
     if (ev.d2 > 0.0) {
         // ratio < 1 is tested on the rounded quotient: drop a slice whose
         // quotient rounds to 1.
+#pragma clang loop vectorize(disable) unroll(disable)
         while (b >= a && !(((slice_z(p, b) - ev.z1) / ev.d2) < 1.0))
             b--;
     } else {
+#pragma clang loop vectorize(disable) unroll(disable)
         while (b >= a && !(((slice_z(p, a) - ev.z1) / ev.d2) < 1.0))
             a++;
     }
@@ -193,12 +201,15 @@ __device__ __forceinline__ bool init_event(const TorParams &p, const RawEvent &r
     ev.d0 = (double)x2 - (double)x1;
     ev.d1 = (double)y2 - (double)y1;
     ev.d2 = (double)z2 - (double)z1;
-    double dis = __dsqrt_rn(__dadd_rn(__dadd_rn(__dmul_rn(ev.d0, ev.d0), __dmul_rn(ev.d1, ev.d1)),
-                                      __dmul_rn(ev.d2, ev.d2)));
-    ev.ux_d = __ddiv_rn(ev.d0, dis);
-    ev.uy_d = __ddiv_rn(ev.d1, dis);
-    ev.ux = (float)ev.ux_d;
-    ev.uy = (float)ev.uy_d;
+    ev.dis2 = __dadd_rn(__dadd_rn(__dmul_rn(ev.d0, ev.d0), __dmul_rn(ev.d1, ev.d1)),
+                        __dmul_rn(ev.d2, ev.d2));
+    const double dis = __dsqrt_rn(ev.dis2);
+    // FP32 direction cosines for the weights and the margin-guarded cut-off
+    // test; the exact d0 / dis of the reference is recomputed by the FP64
+    // decision (inside_exact).
+    const double rdis = 1.0 / dis;
+    ev.ux = (float)(ev.d0 * rdis);
+    ev.uy = (float)(ev.d1 * rdis);
     ev.rd2 = 1.0 / ev.d2;
     ev.sc_d = p.geo ? p.tor_sigma2 * (double)f : p.tor_sigma2;
     ev.thr = (float)(9.0 * ev.sc_d);
@@ -211,6 +222,9 @@ __device__ __forceinline__ bool init_event(const TorParams &p, const RawEvent &r
         ev.ctof = 0.0f;
     }
     if (!(ev.sc_d > 0.0) || !(dis > 0.0) || !(dis < 1.0e30))
+        return false;
+    // (d1, d2) = 0 means no crossed slice (d2 = 0); keeps rden finite
+    if (!(ev.d1 * ev.d1 + ev.d2 * ev.d2 > 0.0))
         return false;
     return slice_range(p, ev);
 }
@@ -227,7 +241,9 @@ struct Track {
     float cxy;   // ux uy / (1 - ux^2)
     float q;     // uz^2 / (1 - ux^2)
     float tm;    // cut-off padded so the candidate set contains every kept voxel
-    float dymax; // sqrt(tm / q)
+    float dymax; // sqrt(tm / q): half height of the ellipse
+    float hx;    // sqrt(tm (A + cxy^2 / q)): half width of the ellipse
+    int nx, ny;  // voxel columns / rows any slice's ellipse can span
 };
 
 __device__ __forceinline__ void make_track(const TorParams &p, const Event &ev, Track &t)
@@ -245,16 +261,20 @@ __device__ __forceinline__ void make_track(const TorParams &p, const Event &ev, 
     t.dvx = (float)(ev.d0 * rs);
     t.dvy = (float)(ev.d1 * rs);
     t.dvz = (float)p.inter_z;
-    double ux2 = ev.ux_d * ev.ux_d, uy2 = ev.uy_d * ev.uy_d;
-    double a = 1.0 - ux2;
-    double dis2 = ev.d0 * ev.d0 + ev.d1 * ev.d1 + ev.d2 * ev.d2;
-    double uz2 = ev.d2 * ev.d2 / dis2;
-    t.A = (float)(1.0 / a);
-    t.cxy = (float)(ev.ux_d * ev.uy_d / a);
-    t.q = (float)(uz2 / a);
+    // a = 1 - ux^2 = (d1^2 + d2^2) / |d|^2, formed without cancellation
+    // (d2 != 0 for an event with crossed slices)
+    const double den = ev.d1 * ev.d1 + ev.d2 * ev.d2;
+    const double rden = 1.0 / den;
+    t.A = (float)(ev.dis2 * rden);
+    t.cxy = (float)(ev.d0 * ev.d1 * rden);
+    t.q = (float)(ev.d2 * ev.d2 * rden);
     t.tm = ev.thr * (1.0f + 1.0e-5f) + 2.0f * kThrMargin;
     t.dymax = sqrtf(t.tm / t.q);
-    (void)uy2;
+    t.hx = sqrtf(t.tm * (t.A + t.cxy * t.cxy / t.q));
+    // integers in an open interval of length 2h/pitch, with the index
+    // rounding allowance of the box start (1e-4) on both ends
+    t.nx = (int)fminf(floorf(2.0f * t.hx * p.rfinter_x + 2.0e-4f), 1.0e6f) + 1;
+    t.ny = (int)fminf(floorf(2.0f * t.dymax * p.rfinter_y + 2.0e-4f), 1.0e6f) + 1;
 }
 
 struct Cross {
@@ -325,8 +345,10 @@ __device__ __forceinline__ bool inside_exact(const TorParams &p, const Event &ev
     s = opaque(s);
     double cx, cy;
     cross_point_exact(p, ev, s, cx, cy);
+    double dis = __dsqrt_rn(ev.dis2);
+    double ux = __ddiv_rn(ev.d0, dis), uy = __ddiv_rn(ev.d1, dis);
     double dx = __dsub_rn(cx, mesh_x(p, ix)), dy = __dsub_rn(cy, mesh_y(p, iy));
-    double rc = __dadd_rn(__dmul_rn(dx, ev.ux_d), __dmul_rn(dy, ev.uy_d));
+    double rc = __dadd_rn(__dmul_rn(dx, ux), __dmul_rn(dy, uy));
     double d2 = __dsub_rn(__dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy)), __dmul_rn(rc, rc));
     return d2 < 9.0 * ev.sc_d;
 }
@@ -354,12 +376,18 @@ __device__ __forceinline__ float voxel_offset(float o0, int i, float step)
     return fmaf(-(float)i, step, o0);
 }
 
+// From the per-column terms (dx^2, ux dx) and per-row terms (dy^2, uy dy).
+__device__ __forceinline__ float tube_d2(float dx2, float uxdx, float dy2, float uydy)
+{
+#pragma clang fp contract(off)
+    float rc = uxdx + uydy;
+    return fmaf(-rc, rc, dx2 + dy2);
+}
+
 __device__ __forceinline__ float tube_dist2(float dx, float dy, float ux, float uy)
 {
 #pragma clang fp contract(off)
-    float rc = fmaf(dy, uy, dx * ux);
-    float r2 = fmaf(dy, dy, dx * dx);
-    return fmaf(-rc, rc, r2);
+    return tube_d2(dx * dx, ux * dx, dy * dy, uy * dy);
 }
 
 // One voxel: exact include test, then gather / scatter / count.
@@ -431,83 +459,142 @@ __device__ __forceinline__ float patch_walk(const TorParams &p, const Event &ev,
     return acc;
 }
 
-// Projection of one slice with a compile-time patch of P x P: every row's
-// chord is computed first, then all candidate voxels are loaded (up to P*P
-// gathers in flight per lane), then the exact include tests and weights are
-// evaluated and accumulated in the reference's order (rows outer, columns
-// inner).  Same voxels, same order, same arithmetic as patch_walk<0>.
-template <int P>
-__device__ __forceinline__ float slice_gather(const TorParams &p, const Event &ev, const Track &t,
-                                              const Cross &c, float amp, int s)
+// Index of patch voxel (i, j) of one slice in the accessed array: the
+// strided image, or an LDS tile.
+struct PatchIdx {
+    int base, si, sj;
+    __device__ __forceinline__ int operator()(int i, int j) const { return base + i * si + j * sj; }
+};
+
+// One slice through the event's candidate box: the NB x NB voxels (at most
+// t.nx columns, t.ny rows) starting at the first column / row whose centre can
+// lie inside the padded tube ellipse, clipped to patch voxels [i_lo, i_hi] x
+// [j_lo, j_hi] (patch, grid and tile bounds).  Every kept voxel of the slice
+// is in the box when t.nx, t.ny <= NB (the caller checks).  Three passes:
+// all squared distances and the FP32 include decisions; the FP64 decision for
+// the rare voxels within the margin of the cut-off; the weights of the kept
+// voxels in the reference's order (rows outer, columns inner), same
+// arithmetic as patch_walk.
+//   KIND 0: returns sum w * src[idx]; 1: dst[idx] += w * scale (LDS: LDS
+//   atomics in dst's type, else FP32 global atomics); 2: returns the count.
+template <int KIND, int NB, bool LDS, typename D>
+__device__ __forceinline__ float slice_box(const TorParams &p, const Event &ev, const Track &t,
+                                           const Cross &c, float amp, int s, float scale, int i_lo,
+                                           int i_hi, int j_lo, int j_hi, const PatchIdx &idx,
+                                           const float *src, D *dst)
 {
-    if (c.ix0 <= -P || c.ix0 >= p.gx || c.iy0 <= -P || c.iy0 >= p.gy)
-        return 0.0f;
-    const int i_lo = c.ix0 < 0 ? -c.ix0 : 0;
-    const int i_hi = min(P, p.gx - c.ix0) - 1;
-    const int j_lo = c.iy0 < 0 ? -c.iy0 : 0;
-    const int j_hi = min(P, p.gy - c.iy0) - 1;
     const float eps = 1.0e-4f;
-    int il[P], cnt[P];
+    const int ib0 = (int)ceilf((c.dx0 - t.hx) * p.rfinter_x - eps);
+    const int jb0 = (int)ceilf((c.dy0 - t.dymax) * p.rfinter_y - eps);
+    const int ib = max(i_lo, ib0), ie = min(i_hi, ib0 + t.nx - 1);
+    const int jb = max(j_lo, jb0), je = min(j_hi, jb0 + t.ny - 1);
+    float cx2[NB], cux[NB], ry2[NB], ruy[NB];
 #pragma unroll
-    for (int j = 0; j < P; j++) {
-        float dy = voxel_offset(c.dy0, j, p.finter_y);
-        float tt = t.tm - dy * dy * t.q;
-        float hw = __builtin_amdgcn_sqrtf(fmaxf(tt, 0.0f) * t.A);
-        float ctr = c.dx0 - t.cxy * dy;
-        int a = max(i_lo, (int)ceilf((ctr - hw) * p.rfinter_x - eps));
-        int b = min(i_hi, (int)floorf((ctr + hw) * p.rfinter_x + eps));
-        bool row = j >= j_lo && j <= j_hi && tt > 0.0f;
-        il[j] = a;
-        cnt[j] = row ? max(b - a + 1, 0) : 0;
+    for (int k = 0; k < NB; k++) {
+        float dx = voxel_offset(c.dx0, ib + k, p.finter_x);
+        float dy = voxel_offset(c.dy0, jb + k, p.finter_y);
+        cx2[k] = dx * dx;
+        cux[k] = ev.ux * dx;
+        ry2[k] = dy * dy;
+        ruy[k] = ev.uy * dy;
     }
-    const int base = s * p.sz + c.ix0 * p.sx + c.iy0 * p.sy;
-    float v[P][P];
+    float d2[NB * NB];
+    uint32_t in = 0, near = 0;
 #pragma unroll
-    for (int j = 0; j < P; j++) {
+    for (int jj = 0; jj < NB; jj++) {
 #pragma unroll
-        for (int k = 0; k < P; k++) {
-            v[j][k] = 0.0f;
-            if (k < cnt[j])
-                v[j][k] = p.img[base + (il[j] + k) * p.sx + j * p.sy];
+        for (int ii = 0; ii < NB; ii++) {
+            const int k = jj * NB + ii;
+            const bool ok = ib + ii <= ie && jb + jj <= je;
+            d2[k] = tube_d2(cx2[ii], cux[ii], ry2[jj], ruy[jj]);
+            const bool a = ok && d2[k] < ev.thr - kThrMargin;
+            const bool b = ok && !a && d2[k] < ev.thr + kThrMargin;
+            in |= (uint32_t)a << k;
+            near |= (uint32_t)b << k;
         }
     }
+    if (near) {
+        // rare: the reference's FP64 decision, one voxel at a time
+        for (uint32_t m = (uint32_t)opaque((int)near); m; m &= m - 1) {
+            const int k = __builtin_ctz(m);
+            const int ii = k % NB, jj = k / NB;
+            if (inside_exact(p, ev, s, c.ix0 + ib + ii, c.iy0 + jb + jj))
+                in |= 1u << k;
+        }
+    }
+    if (KIND == 2)
+        return (float)__builtin_popcount(in);
     float acc = 0.0f;
+    if (KIND == 0) {
+        float v[NB * NB];
 #pragma unroll
-    for (int j = 0; j < P; j++) {
-        float dy = voxel_offset(c.dy0, j, p.finter_y);
+        for (int k = 0; k < NB * NB; k++)
+            v[k] = ((in >> k) & 1u) ? src[idx(ib + k % NB, jb + k / NB)] : 0.0f;
 #pragma unroll
-        for (int k = 0; k < P; k++) {
-            if (k < cnt[j]) {
-                int i = il[j] + k;
-                float dx = voxel_offset(c.dx0, i, p.finter_x);
-                float d2 = tube_dist2(dx, dy, ev.ux, ev.uy);
-                bool in = d2 < ev.thr - kThrMargin;
-                if (!in && d2 < ev.thr + kThrMargin)
-                    in = inside_exact(p, ev, s, c.ix0 + i, c.iy0 + j);
-                if (in)
-                    acc = fmaf(amp * __builtin_amdgcn_exp2f(d2 * ev.cw), v[j][k], acc);
+        for (int k = 0; k < NB * NB; k++)
+            if ((in >> k) & 1u)
+                acc = fmaf(amp * __builtin_amdgcn_exp2f(d2[k] * ev.cw), v[k], acc);
+    } else {
+#pragma unroll
+        for (int k = 0; k < NB * NB; k++) {
+            if ((in >> k) & 1u) {
+                float w = amp * __builtin_amdgcn_exp2f(d2[k] * ev.cw);
+                D *q = dst + idx(ib + k % NB, jb + k / NB);
+                if constexpr (LDS)
+                    atomicAdd(q, (D)(w * scale));
+                else
+                    unsafeAtomicAdd(q, w * scale);
             }
         }
     }
     return acc;
 }
 
+// The box walk of one slice over the strided image (patch voxels inside the
+// grid).
+template <int KIND, int NB>
+__device__ __forceinline__ float slice_box_image(const TorParams &p, const Event &ev,
+                                                 const Track &t, const Cross &c, float amp, int s,
+                                                 float scale)
+{
+    const int P = p.patch;
+    if (c.ix0 <= -P || c.ix0 >= p.gx || c.iy0 <= -P || c.iy0 >= p.gy)
+        return 0.0f;
+    const int i_lo = c.ix0 < 0 ? -c.ix0 : 0;
+    const int i_hi = min(P, p.gx - c.ix0) - 1;
+    const int j_lo = c.iy0 < 0 ? -c.iy0 : 0;
+    const int j_hi = min(P, p.gy - c.iy0) - 1;
+    PatchIdx idx;
+    if (KIND == 0) {
+        idx.base = s * p.sz + c.ix0 * p.sx + c.iy0 * p.sy;
+        idx.si = p.sx;
+        idx.sj = p.sy;
+    } else {
+        idx.base = s * p.oz + c.ix0 * p.ox + c.iy0 * p.oy;
+        idx.si = p.ox;
+        idx.sj = p.oy;
+    }
+    return slice_box<KIND, NB, false>(p, ev, t, c, amp, s, scale, i_lo, i_hi, j_lo, j_hi, idx,
+                                      p.img, p.out);
+}
+
 // Walk the crossed slices s_first, s_first + rstep, ... <= s_last of an event
-// (the caller clips to [s_lo, s_hi]).  PG > 0: projection with the batched
-// gather of a PG x PG patch.
-template <int KIND, int PG>
+// (the caller clips to [s_lo, s_hi]).  NB > 0: slices go through the NB x NB
+// candidate box when the event's box fits, else through the chord walk.
+template <int KIND, int NB>
 __device__ __forceinline__ float walk_event(const TorParams &p, const Event &ev, bool tof,
                                             float scale, int s_first, int s_last, int rstep)
 {
     Track t;
     make_track(p, ev, t);
+    const bool boxed = NB > 0 && !p.full_patch && t.nx <= NB && t.ny <= NB;
     float acc = 0.0f;
     for (int s = s_first; s <= s_last; s += rstep) {
         Cross c;
         cross_at(p, ev, t, s, c);
         float amp = tof ? tof_factor(p, ev, t, s) : 1.0f;
-        if (KIND == 0 && PG > 0)
-            acc += slice_gather<(PG > 0 ? PG : 1)>(p, ev, t, c, amp, s);
+        if (NB > 0 && boxed)
+            acc += slice_box_image<KIND, (NB > 0 ? NB : 1)>(p, ev, t, c, amp, s, scale);
         else if (p.full_patch)
             acc += patch_walk<KIND, true>(p, ev, t, c, amp, s, scale);
         else
@@ -533,35 +620,35 @@ __device__ __forceinline__ float group_sum(float v)
 // across the lanes.  Every lane of the wave calls this; lanes with
 // valid == false contribute nothing.  Returns the event's non-zero count on
 // its first lane in kCount mode.
-template <int MODE, int PG, int LPE>
+template <int MODE, int NB, int LPE>
 __device__ __forceinline__ int run_event(const TorParams &p, int64_t e, int sub, bool valid)
 {
     Event ev;
     bool any = valid ? load_event<MODE>(p, e, ev) : false;
     const bool tof = (MODE != kMap && MODE != kCount) && p.use_tof;
     if (MODE == kProj) {
-        float part = any ? walk_event<0, PG>(p, ev, tof, 0.0f, ev.s_lo + sub, ev.s_hi, LPE) : 0.0f;
+        float part = any ? walk_event<0, NB>(p, ev, tof, 0.0f, ev.s_lo + sub, ev.s_hi, LPE) : 0.0f;
         float pv = group_sum<LPE>(part);
         if (valid && sub == 0)
             p.out[e] = pv;
     } else if (MODE == kBack) {
         float v = valid ? p.val[e] : 0.0f;
         if (any && v > 1e-7)
-            walk_event<1, 0>(p, ev, tof, 1.0f / v, ev.s_lo + sub, ev.s_hi, LPE);
+            walk_event<1, NB>(p, ev, tof, 1.0f / v, ev.s_lo + sub, ev.s_hi, LPE);
     } else if (MODE == kMap) {
         if (any)
-            walk_event<1, 0>(p, ev, false, 1.0f / p.val[e], ev.s_lo + sub, ev.s_hi, LPE);
+            walk_event<1, NB>(p, ev, false, 1.0f / p.val[e], ev.s_lo + sub, ev.s_hi, LPE);
     } else if (MODE == kAdj) {
         float g = valid ? p.val[e] : 0.0f;
         if (any && g != 0.0f)
-            walk_event<1, 0>(p, ev, tof, g, ev.s_lo + sub, ev.s_hi, LPE);
+            walk_event<1, NB>(p, ev, tof, g, ev.s_lo + sub, ev.s_hi, LPE);
     } else if (MODE == kOsem) {
-        float part = any ? walk_event<0, PG>(p, ev, tof, 0.0f, ev.s_lo + sub, ev.s_hi, LPE) : 0.0f;
+        float part = any ? walk_event<0, NB>(p, ev, tof, 0.0f, ev.s_lo + sub, ev.s_hi, LPE) : 0.0f;
         float pv = group_sum<LPE>(part);
         if (any && pv > 1e-7)
-            walk_event<1, 0>(p, ev, tof, 1.0f / pv, ev.s_lo + sub, ev.s_hi, LPE);
+            walk_event<1, NB>(p, ev, tof, 1.0f / pv, ev.s_lo + sub, ev.s_hi, LPE);
     } else { // kCount
-        float part = any ? walk_event<2, 0>(p, ev, false, 0.0f, ev.s_lo + sub, ev.s_hi, LPE) : 0.0f;
+        float part = any ? walk_event<2, NB>(p, ev, false, 0.0f, ev.s_lo + sub, ev.s_hi, LPE) : 0.0f;
         int cnt = (int)group_sum<LPE>(part);
         if (valid && sub == 0) {
             if (p.nnz)
@@ -572,13 +659,13 @@ __device__ __forceinline__ int run_event(const TorParams &p, int64_t e, int sub,
     return 0;
 }
 
-template <int MODE, int PG, int LPE>
+template <int MODE, int NB, int LPE>
 __global__ __launch_bounds__(256) void tor_kernel(TorParams p)
 {
     int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     int64_t e = gid / LPE;
     int sub = (int)(gid % LPE);
-    int cnt = run_event<MODE, PG, LPE>(p, e, sub, e < p.n);
+    int cnt = run_event<MODE, NB, LPE>(p, e, sub, e < p.n);
     if (MODE == kCount && p.nnz_total) {
         // Wave reduction, then one 64-bit atomic per wave.
         unsigned long long c = (unsigned long long)cnt;
@@ -614,7 +701,7 @@ __device__ __forceinline__ void slab_of(const TorParams &p, int nslab, int inter
     }
 }
 
-template <int MODE, int PG>
+template <int MODE, int NB>
 __global__ __launch_bounds__(256) void tor_slab_kernel(TorParams p, float *part, int nslab,
                                                        int interleave)
 {
@@ -635,7 +722,7 @@ __global__ __launch_bounds__(256) void tor_slab_kernel(TorParams p, float *part,
     }
     const bool tof = MODE != kMap && p.use_tof;
     if (MODE == kProj) {
-        float v = any ? walk_event<0, PG>(p, ev, tof, 0.0f, s0, s1, 1) : 0.0f;
+        float v = any ? walk_event<0, NB>(p, ev, tof, 0.0f, s0, s1, 1) : 0.0f;
         if (valid)
             part[(int64_t)slab * p.n + e] = v;
     } else {
@@ -660,7 +747,7 @@ __global__ __launch_bounds__(256) void tor_slab_kernel(TorParams p, float *part,
             scale = g;
         }
         if (go)
-            walk_event<1, 0>(p, ev, tof, scale, s0, s1, 1);
+            walk_event<1, NB>(p, ev, tof, scale, s0, s1, 1);
     }
 }
 
@@ -698,6 +785,7 @@ int setup(TorParams &p, const float *lors, int64_t n, int64_t ld, const srfx_gri
     p.inter_z = lz / (double)p.gz;
     p.rinter_x = 1.0 / p.inter_x;
     p.rinter_y = 1.0 / p.inter_y;
+    p.rinter_z = 1.0 / p.inter_z;
     p.finter_x = (float)p.inter_x;
     p.finter_y = (float)p.inter_y;
     p.rfinter_x = (float)(1.0 / p.inter_x);
@@ -721,6 +809,18 @@ int setup(TorParams &p, const float *lors, int64_t n, int64_t ld, const srfx_gri
         p.tse_inv = (float)(1.0 / tse);
         p.tofk = (float)((double)tof_bin / std::sqrt(2.0 * M_PI * (double)tof_sigma2));
     }
+    // Candidate box: the compiled size that holds the box of every event
+    // whose slice axis is its dominant direction (as in the axis groups of
+    // on_tor_lors.py): there the ellipse's half extents are at most
+    // sqrt(2 tm).  Events with a larger box take the chord walk.
+    {
+        double tm = 9.0 * p.tor_sigma2 * (1.0 + 1.0e-5) + 2.0 * kThrMargin;
+        double h = std::sqrt(2.0 * tm);
+        int nx = (int)std::floor(2.0 * h / p.inter_x + 2.0e-4) + 1;
+        int ny = (int)std::floor(2.0 * h / p.inter_y + 2.0e-4) + 1;
+        int n = nx > ny ? nx : ny;
+        p.box = n <= 3 ? 3 : (n <= 4 ? 4 : 0);
+    }
     Strides s = grid_strides(g);
     p.sx = p.ox = (int)s.x;
     p.sy = p.oy = (int)s.y;
@@ -728,10 +828,10 @@ int setup(TorParams &p, const float *lors, int64_t n, int64_t ld, const srfx_gri
     return SRFX_OK;
 }
 
-template <int MODE, int PG, int LPE>
+template <int MODE, int NB, int LPE>
 int launch_pg(const TorParams &p, int block, hipStream_t st)
 {
-    hipLaunchKernelGGL((tor_kernel<MODE, PG, LPE>), dim3(blocks_for(p.n * LPE, block)),
+    hipLaunchKernelGGL((tor_kernel<MODE, NB, LPE>), dim3(blocks_for(p.n * LPE, block)),
                        dim3(block), 0, st, p);
     return launch_status();
 }
@@ -739,31 +839,22 @@ int launch_pg(const TorParams &p, int block, hipStream_t st)
 template <int MODE, int LPE>
 int launch_lpe(const TorParams &p, int block, hipStream_t st)
 {
-    if constexpr (MODE == kProj || MODE == kOsem) {
-        if (!p.full_patch) {
-            switch (p.patch) {
-            case 3:
-                return launch_pg<MODE, 3, LPE>(p, block, st);
-            case 4:
-                return launch_pg<MODE, 4, LPE>(p, block, st);
-            case 5:
-                return launch_pg<MODE, 5, LPE>(p, block, st);
-            case 6:
-                return launch_pg<MODE, 6, LPE>(p, block, st);
-            default:
-                break;
-            }
-        }
+    switch (p.full_patch ? 0 : p.box) {
+    case 3:
+        return launch_pg<MODE, 3, LPE>(p, block, st);
+    case 4:
+        return launch_pg<MODE, 4, LPE>(p, block, st);
+    default:
+        return launch_pg<MODE, 0, LPE>(p, block, st);
     }
-    return launch_pg<MODE, 0, LPE>(p, block, st);
 }
 
-template <int MODE, int PG>
+template <int MODE, int NB>
 int launch_slab_pg(const TorParams &p, float *part, int nslab, int interleave, int block,
                    hipStream_t st)
 {
     unsigned chunks = blocks_for(p.n, block);
-    hipLaunchKernelGGL((tor_slab_kernel<MODE, PG>), dim3(chunks * (unsigned)nslab), dim3(block),
+    hipLaunchKernelGGL((tor_slab_kernel<MODE, NB>), dim3(chunks * (unsigned)nslab), dim3(block),
                        0, st, p, part, nslab, interleave);
     return launch_status();
 }
@@ -781,23 +872,14 @@ int launch_slab(const TorParams &p, float *part, uint32_t flags, hipStream_t st)
     const int block = kBlock[(flags >> SRFX_BLOCK_SHIFT) & 3u];
     if (p.n == 0)
         return SRFX_OK;
-    if constexpr (MODE == kProj) {
-        if (!p.full_patch) {
-            switch (p.patch) {
-            case 3:
-                return launch_slab_pg<MODE, 3>(p, part, nslab, interleave, block, st);
-            case 4:
-                return launch_slab_pg<MODE, 4>(p, part, nslab, interleave, block, st);
-            case 5:
-                return launch_slab_pg<MODE, 5>(p, part, nslab, interleave, block, st);
-            case 6:
-                return launch_slab_pg<MODE, 6>(p, part, nslab, interleave, block, st);
-            default:
-                break;
-            }
-        }
+    switch (p.full_patch ? 0 : p.box) {
+    case 3:
+        return launch_slab_pg<MODE, 3>(p, part, nslab, interleave, block, st);
+    case 4:
+        return launch_slab_pg<MODE, 4>(p, part, nslab, interleave, block, st);
+    default:
+        return launch_slab_pg<MODE, 0>(p, part, nslab, interleave, block, st);
     }
-    return launch_slab_pg<MODE, 0>(p, part, nslab, interleave, block, st);
 }
 
 int launch_slab_sum(const float *part, int nslab, int64_t n, float *out, hipStream_t st)
@@ -810,9 +892,9 @@ int launch_slab_sum(const float *part, int nslab, int64_t n, float *out, hipStre
 }
 
 // Launch schedule from the flags: lanes per event (1, 4, 16) and block size
-// (64, 128, 256).  Projection-type modes get a kernel specialised for the
-// patch size (batched gathers); the full-patch verification path and other
-// sizes use the generic loops.
+// (64, 128, 256).  Kernels are specialised for the candidate box size
+// (3 or 4); the full-patch verification path and larger boxes use the
+// generic loops.
 template <int MODE>
 int launch(const TorParams &p, uint32_t flags, hipStream_t st)
 {

@@ -20,8 +20,8 @@ import torch
 from . import _lib
 from .ops import _check_dev, _geom, _ptr, _soa, _stream, _vec
 
-DEFAULT_TILE = (8, 32, 32)
-DEFAULT_CHUNK = 4096
+DEFAULT_TILE = (8, 32, 16)
+DEFAULT_CHUNK = 2048
 
 
 class TorPlan:
