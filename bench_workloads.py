@@ -268,7 +268,7 @@ def autograd_c3(args):
 def batched_c4(args):
     import torch
     from srf_amd import effmap, physics, preprocess, synth
-    from srf_amd.batch import BatchedOSEM, serial_iterate
+    from srf_amd.batch import BatchedOSEM, serial_drivers, serial_iterate
     from srf_amd.data import Image, split_from_groups
     from srf_amd.scanner import CylindricalScanner
     dev = torch.device("cuda:0")
@@ -302,7 +302,10 @@ def batched_c4(args):
         for rc in recons:
             rc["image"].data.copy_(rc["x0"])
 
+    t0 = time.time()
     bat = BatchedOSEM(recons, model, nb_subsets=K)
+    torch.cuda.synchronize()
+    log(f"batch plan ({R * 3 * K} tile plans) in {time.time() - t0:.2f}s")
     bat.capture()
     reset()
     bat.replay(1)  # warm-up
@@ -316,12 +319,16 @@ def batched_c4(args):
     ms_b = a.elapsed_time(b)
     out_b = [rc["image"].data.clone() for rc in recons]
     reset()
-    serial_iterate(recons, model, K, 1)  # warm-up
+    t0 = time.time()
+    drivers = serial_drivers(recons, model, K)
+    torch.cuda.synchronize()
+    log(f"serial tile plans in {time.time() - t0:.2f}s")
+    serial_iterate(recons, model, K, 1, drivers=drivers)  # warm-up
     reset()
     torch.cuda.synchronize()
     a, b = _timer()
     a.record()
-    serial_iterate(recons, model, K, iters)
+    serial_iterate(recons, model, K, iters, drivers=drivers)
     b.record()
     torch.cuda.synchronize()
     ms_s = a.elapsed_time(b)
@@ -376,8 +383,21 @@ def large_c5(args):
     log(f"attenuated efficiency map in {time.time() - t0:.1f}s")
     x = effmap.initial_image(grid, center, size, sc.inner_radius, sc.axial_length, dev)
     K = 20
-    osem = OSEM(model, split_from_groups(groups), Image(E, center, size), nb_subsets=K)
+    t0 = time.time()
+    groups, _ = preprocess.reorder_groups(groups, K)
+    torch.cuda.synchronize()
+    log(f"locality order inside the {K} subsets in {time.time() - t0:.1f}s")
+    osem = OSEM(model, split_from_groups(groups), Image(E, center, size), nb_subsets=K,
+                tiled=True)
     img = Image(x, center, size)
+    torch.cuda.synchronize()
+    t0 = time.time()
+    osem.plan(img)
+    torch.cuda.synchronize()
+    plan_s = time.time() - t0
+    listings = sum(p.info()[0] for ps in osem.plans for p in ps.values() if p is not None)
+    log(f"tile plans in {plan_s:.2f}s, {listings / nev:.1f} listings per event, "
+        f"{torch.cuda.memory_allocated() / 2**30:.1f} GiB allocated")
     torch.cuda.synchronize()
     a, b = _timer()
     a.record()
@@ -391,4 +411,6 @@ def large_c5(args):
         args.steps, 0, ms,
         {"workload": "large_c5", "image": list(grid), "voxel_mm": [1.6, 1.6, 1.7],
          "events": nev, "subsets": K, "tof_fwhm_ps": 400, "attenuation": "mu 0.0096/mm body",
-         "step": "one OSEM iteration over all events"})), flush=True)
+         "step": "one OSEM iteration over all events", "tiled": True, "sorted": True,
+         "plan_build_s": round(plan_s, 3), "listings_per_event": round(listings / nev, 2),
+         "peak_gib": round(torch.cuda.max_memory_allocated() / 2**30, 1)})), flush=True)

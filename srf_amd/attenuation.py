@@ -52,18 +52,26 @@ def water_cylinder(grid, center, size, radius, half_height, mu=0.0096, device="c
 
 
 def attenuated_tor_effmap(scanner, model, mu, grid, center, size, device, chunk=4_000_000,
-                          max_pairs=None):
+                          max_pairs=None, tiled=True):
     """Sum over unordered crystal pairs of Maplors with the attenuation
-    weights as lors_value: sum_i w_ij exp(-int mu dl_i)."""
+    weights as lors_value: sum_i w_ij exp(-int mu dl_i).  Pairs are
+    enumerated on the device; tiled=True scatters through tile plans."""
     out = torch.zeros(tuple(grid), dtype=torch.float32, device=device)
     img = Image(out, center, size)
     total = scanner.nb_pairs() if max_pairs is None else min(max_pairs, scanner.nb_pairs())
     for start in range(0, total, chunk):
         stop = min(total, start + chunk)
-        pairs = torch.from_numpy(scanner.all_pairs((start, stop))).to(device)
+        pairs = scanner.all_pairs_device((start, stop), device)
         w = attenuation_weights(mu, pairs)
         groups, orders = map_process(pairs, return_order=True)
         data = split_from_groups(groups, values={a: w[orders[a]] for a in orders},
                                  without_tof=True)
-        model.maplors(data, img, out=out)
+        if tiled:
+            plans = model.make_plans(img, data, ncols=7)
+            model.maplors(data, img, out=out, plans=plans)
+            for p in plans.values():
+                if p is not None:
+                    p.close()
+        else:
+            model.maplors(data, img, out=out)
     return out

@@ -4,10 +4,14 @@
 (each a dict with ``image`` (Image), ``emap`` (Image of the inverse efficiency
 map) and ``data`` (ListModeDataSplit from preprocess.recon_process)) that share
 one SplitLoRsModel.  ``run_subset(k)`` advances every reconstruction by OSEM
-subset k in one persistent kernel launch (srfx_tor_batch_subset);
+subset k (srfx_tor_batch_subset: four kernels, two of them persistent);
 ``capture()`` records one full iteration (all subsets) into a HIP graph that
-``replay()`` reruns.  ``serial_iterate`` runs the same reconstructions one
-after another through the per-reconstruction OSEM driver, for comparison.
+``replay()`` reruns.  Every (reconstruction, axis group, subset) gets a tile
+plan at creation; a subset step is a persistent projection over the events of
+all reconstructions, a persistent tiled backprojection over the tile chunks of
+all plans and the batched update.  ``serial_iterate`` runs the same
+reconstructions one after another through the per-reconstruction OSEM driver
+(projection + tiled backprojection), for comparison.
 """
 import ctypes
 
@@ -121,11 +125,24 @@ class BatchedOSEM:
             pass
 
 
-def serial_iterate(recons, model, nb_subsets, nb_iterations=1, fused=True):
-    """The same reconstructions, one after another, through recon.OSEM."""
-    drivers = [OSEM(model, rc["data"], rc["emap"], nb_subsets=nb_subsets, fused=fused)
-               for rc in recons]
+def serial_drivers(recons, model, nb_subsets, fused=True, tiled=True):
+    """One recon.OSEM per reconstruction (tile plans built here, once)."""
+    drivers = [OSEM(model, rc["data"], rc["emap"], nb_subsets=nb_subsets, fused=fused,
+                    tiled=tiled) for rc in recons]
+    if tiled:
+        for d, rc in zip(drivers, recons):
+            d.plan(rc["image"])
+    return drivers
+
+
+def serial_iterate(recons, model, nb_subsets, nb_iterations=1, fused=True, tiled=True,
+                   drivers=None):
+    """The same reconstructions, one after another, through recon.OSEM (the
+    plain per-reconstruction path: projection + tiled backprojection)."""
+    if drivers is None:
+        drivers = serial_drivers(recons, model, nb_subsets, fused, tiled)
     for _ in range(nb_iterations):
         for d, rc in zip(drivers, recons):
+            d.subset = 0
             d.iterate(rc["image"], 1)
     return drivers

@@ -1,33 +1,45 @@
 // Batched TOF OSEM: many independent reconstructions (different image sizes
-// and event counts) advance one subset in one persistent launch.
+// and event counts) advance one subset together.
 //
-// A plan holds, in device memory, one TorParams per (reconstruction, axis
-// group) and, per subset, a table of work chunks of at most 256 events that
-// never straddle two groups.  A subset step is three kernels on one stream:
-//   batch_zero    zero every reconstruction's backprojection buffer and the
-//                 work counter;
-//   batch_osem    persistent fused projection + ratio backprojection: each
-//                 block takes the next chunk from the counter until none is
-//                 left (every block exits once the counter passes the table);
-//   batch_update  x <- x * E * bp for every reconstruction.
-// The step allocates nothing and issues no memset, so a caller can capture it
-// in a HIP graph.
+// At creation every (reconstruction, axis group, subset) gets a tile plan of
+// its events (tile_core.h).  A subset step is four kernels on one stream:
+//   batch_zero     zero every reconstruction's backprojection buffer and the
+//                  two work counters;
+//   batch_project  persistent projection: each block takes the next chunk of
+//                  <= 256 events (of any reconstruction and group) from a
+//                  counter and writes their projections into the plan's
+//                  buffer;
+//   batch_scatter  persistent tiled backprojection of 1/p: each block takes
+//                  the next tile chunk (of any plan), accumulates it in an
+//                  FP64 LDS tile and flushes it into that reconstruction's
+//                  backprojection buffer;
+//   batch_update   x <- x * E * bp for every reconstruction.
+// Every block of a persistent kernel exits once its counter passes the end of
+// the table.  The step allocates nothing and issues no memset, so a caller can
+// capture it in a HIP graph.
 #include <algorithm>
 #include <new>
 #include <vector>
 
-#include "tor_core.h"
+#include "tile_core.h"
 
 namespace srfx {
-namespace {
 
-constexpr int kChunk = 256;
+// One (reconstruction, axis group, subset): the subset's events and its plan.
+struct BatchJob {
+    TorParams p; // img: the reconstruction's image, out: its bp, val: proj
+    TileGeom g;
+    const int *list;
+    const float4 *rec;
+    float *proj;
+};
 
-struct Chunk {
-    int job;
-    int start;
-    int count;
-    int pad;
+struct EvChunk {
+    int job, start, count, pad;
+};
+
+struct TileWork {
+    int job, tile, start, count;
 };
 
 struct VoxRange {
@@ -37,6 +49,10 @@ struct VoxRange {
     long long n;
     long long off; // first voxel of this reconstruction in the concatenated range
 };
+
+namespace {
+
+constexpr int kChunk = 256;
 
 __device__ __forceinline__ int find_range(const VoxRange *r, int nrec, long long i)
 {
@@ -52,10 +68,10 @@ __device__ __forceinline__ int find_range(const VoxRange *r, int nrec, long long
 }
 
 __global__ __launch_bounds__(256) void batch_zero(const VoxRange *__restrict__ r, int nrec,
-                                                  long long total, unsigned *counter)
+                                                  long long total, unsigned *counters)
 {
-    if (blockIdx.x == 0 && threadIdx.x == 0)
-        *counter = 0u;
+    if (blockIdx.x == 0 && threadIdx.x < 2)
+        counters[threadIdx.x] = 0u;
     long long stride = (long long)gridDim.x * blockDim.x;
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += stride) {
         int k = find_range(r, nrec, i);
@@ -75,69 +91,76 @@ __global__ __launch_bounds__(256) void batch_update(const VoxRange *__restrict__
     }
 }
 
-__global__ __launch_bounds__(256) void batch_osem(const TorParams *__restrict__ jobs,
-                                                  const Chunk *__restrict__ chunks, int nchunks,
-                                                  unsigned *counter)
+// Next work item of a persistent block (the same value in every thread).
+__device__ __forceinline__ int next_item(unsigned *counter)
 {
     __shared__ int item;
+    if (threadIdx.x == 0)
+        item = (int)atomicAdd(counter, 1u);
+    __syncthreads();
+    const int c = __builtin_amdgcn_readfirstlane(item);
+    __syncthreads();
+    return c;
+}
+
+__global__ __launch_bounds__(256) void batch_project(const BatchJob *__restrict__ jobs,
+                                                     const EvChunk *__restrict__ chunks,
+                                                     int nchunks, unsigned *counter)
+{
     for (;;) {
-        if (threadIdx.x == 0)
-            item = (int)atomicAdd(counter, 1u);
-        __syncthreads();
-        const int c = __builtin_amdgcn_readfirstlane(item);
-        __syncthreads();
+        const int c = next_item(counter);
         if (c >= nchunks)
             break;
-        const Chunk ch = chunks[c];
-        const TorParams p = jobs[__builtin_amdgcn_readfirstlane(ch.job)];
+        const EvChunk ch = chunks[c];
+        const BatchJob &job = jobs[__builtin_amdgcn_readfirstlane(ch.job)];
+        TorParams p = job.p;
+        p.out = job.proj;
         const bool valid = (int)threadIdx.x < ch.count;
         const int64_t e = (int64_t)ch.start + threadIdx.x;
         switch (p.box) {
         case 3:
-            run_event<kOsem, 3, 1>(p, e, 0, valid);
+            run_event<kProj, 3, 1>(p, e, 0, valid);
             break;
         case 4:
-            run_event<kOsem, 4, 1>(p, e, 0, valid);
+            run_event<kProj, 4, 1>(p, e, 0, valid);
             break;
         default:
-            run_event<kOsem, 0, 1>(p, e, 0, valid);
+            run_event<kProj, 0, 1>(p, e, 0, valid);
             break;
         }
     }
 }
 
-} // namespace
-} // namespace srfx
-
-struct srfx_tor_batch {
-    int nrec = 0, nsub = 0, njobs = 0;
-    srfx::TorParams *d_jobs = nullptr;
-    std::vector<srfx::Chunk *> d_chunks;
-    std::vector<int> nchunks;
-    srfx::VoxRange *d_vox = nullptr;
-    long long total_vox = 0;
-    unsigned *d_counter = nullptr;
-    int osem_blocks = 0, elem_blocks = 0;
-};
-
-using namespace srfx;
-
-static void free_plan(srfx_tor_batch *b)
+__global__ __launch_bounds__(256) void batch_scatter(const BatchJob *__restrict__ jobs,
+                                                     const TileWork *__restrict__ work,
+                                                     int nwork, unsigned *counter)
 {
-    if (!b)
-        return;
-    (void)hipFree(b->d_jobs);
-    for (Chunk *c : b->d_chunks)
-        (void)hipFree(c);
-    (void)hipFree(b->d_vox);
-    (void)hipFree(b->d_counter);
-    delete b;
+    extern __shared__ double ldsd[];
+    for (;;) {
+        const int c = next_item(counter);
+        if (c >= nwork)
+            break;
+        const TileWork w = work[c];
+        const BatchJob &job = jobs[__builtin_amdgcn_readfirstlane(w.job)];
+        const TileChunk ch{w.tile, w.start, w.count, 0};
+        switch (job.p.box) {
+        case 3:
+            scatter_chunk<kBack, 3>(job.p, job.g, ch, job.list, job.rec, ldsd);
+            break;
+        case 4:
+            scatter_chunk<kBack, 4>(job.p, job.g, ch, job.list, job.rec, ldsd);
+            break;
+        default:
+            scatter_chunk<kBack, 0>(job.p, job.g, ch, job.list, job.rec, ldsd);
+            break;
+        }
+    }
 }
 
 // Op frame of axis group a for a C-order (nx, ny, nz) world image, as
 // physics/split_lors.py:54-85 derives it by transposing: op (x, y, z) =
 // world (y, z, x) for the x group, (x, z, y) for y, (x, y, z) for z.
-static srfx_grid group_grid(const srfx_recon &r, int a)
+srfx_grid group_grid(const srfx_recon &r, int a)
 {
     static const int perm[3][3] = {{1, 2, 0}, {0, 2, 1}, {0, 1, 2}};
     const int64_t st[3] = {(int64_t)r.shape[1] * r.shape[2], (int64_t)r.shape[2], 1};
@@ -152,6 +175,56 @@ static srfx_grid group_grid(const srfx_recon &r, int a)
     return g;
 }
 
+// Tile of the batch's plans: 8 slices x 32 x 16 voxels, <= 2048 events per
+// workgroup (the FP64 LDS tile is 32 KB).
+constexpr srfx_tile kBatchTile = {8, 32, 16, 2048, 256, 0};
+
+} // namespace
+} // namespace srfx
+
+struct srfx_tor_batch {
+    int nrec = 0, nsub = 0, njobs = 0;
+    std::vector<srfx_tor_plan> plans;
+    srfx::BatchJob *d_jobs = nullptr;
+    std::vector<srfx::EvChunk *> d_ev;
+    std::vector<int> nev;
+    std::vector<srfx::TileWork *> d_work;
+    std::vector<int> nwork;
+    srfx::VoxRange *d_vox = nullptr;
+    long long total_vox = 0;
+    unsigned *d_counters = nullptr;
+    unsigned scatter_lds = 0;
+    int proj_blocks = 0, scatter_blocks = 0, elem_blocks = 0;
+};
+
+using namespace srfx;
+
+static void free_plan(srfx_tor_batch *b)
+{
+    if (!b)
+        return;
+    (void)hipDeviceSynchronize();
+    for (srfx_tor_plan p : b->plans)
+        (void)srfx_tor_plan_destroy(p);
+    (void)hipFree(b->d_jobs);
+    for (EvChunk *c : b->d_ev)
+        (void)hipFree(c);
+    for (TileWork *w : b->d_work)
+        (void)hipFree(w);
+    (void)hipFree(b->d_vox);
+    (void)hipFree(b->d_counters);
+    delete b;
+}
+
+template <class T> static bool upload(T **dst, const std::vector<T> &v)
+{
+    *dst = nullptr;
+    if (v.empty())
+        return true;
+    return hipMalloc((void **)dst, sizeof(T) * v.size()) == hipSuccess &&
+           hipMemcpy(*dst, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice) == hipSuccess;
+}
+
 extern "C" int srfx_tor_batch_create(const srfx_recon *recons, int32_t nrecon,
                                      int32_t nb_subsets, srfx_tor_model m,
                                      srfx_tor_batch **out)
@@ -159,71 +232,106 @@ extern "C" int srfx_tor_batch_create(const srfx_recon *recons, int32_t nrecon,
     if (!out || !recons || nrecon <= 0 || nb_subsets <= 0)
         return SRFX_EINVAL;
     *out = nullptr;
-    std::vector<TorParams> jobs;
-    std::vector<VoxRange> vox;
-    long long off = 0;
     for (int r = 0; r < nrecon; r++) {
         const srfx_recon &rc = recons[r];
         if (!rc.image || !rc.emap || !rc.bp)
             return SRFX_EINVAL;
         for (int a = 0; a < 3; a++) {
             TorParams p;
-            srfx_grid g = group_grid(rc, a);
-            int st = setup(p, rc.lors[a], rc.n[a], rc.ld[a], g, m.kernel_width,
+            int st = setup(p, rc.lors[a], rc.n[a], rc.ld[a], group_grid(rc, a), m.kernel_width,
                            m.geo_sigma2_flag, m.tof_bin, m.tof_sigma2);
             if (st)
                 return st;
-            p.img = rc.image;
-            p.out = rc.bp;
-            jobs.push_back(p);
+        }
+    }
+    srfx_tor_batch *b = new (std::nothrow) srfx_tor_batch;
+    if (!b)
+        return SRFX_ENOMEM;
+    b->nrec = nrecon;
+    b->nsub = nb_subsets;
+    std::vector<BatchJob> jobs;
+    std::vector<std::vector<EvChunk>> ev(nb_subsets);
+    std::vector<std::vector<TileWork>> work(nb_subsets);
+    std::vector<VoxRange> vox;
+    long long off = 0;
+    int rc_status = SRFX_OK;
+    // Per subset: contiguous rows [k n / K, (k+1) n / K) of every group
+    // (graph/reconstruction/worker.py:79-85), each with its own plan.
+    for (int r = 0; r < nrecon && !rc_status; r++) {
+        const srfx_recon &rc = recons[r];
+        for (int a = 0; a < 3 && !rc_status; a++) {
+            const int64_t n = rc.n[a];
+            for (int k = 0; k < nb_subsets; k++) {
+                const int64_t lo = k * n / nb_subsets, hi = (k + 1) * n / nb_subsets;
+                if (hi <= lo)
+                    continue;
+                srfx_tor_plan plan = nullptr;
+                rc_status = srfx_tor_plan_create(rc.lors[a] + lo, hi - lo, rc.ld[a], 10,
+                                                 group_grid(rc, a), m, kBatchTile, &plan,
+                                                 nullptr);
+                if (rc_status)
+                    break;
+                b->plans.push_back(plan);
+                BatchJob job;
+                job.p = plan->p;
+                job.p.img = rc.image;
+                job.p.out = rc.bp;
+                job.p.val = plan->proj;
+                job.g = plan->g;
+                job.list = plan->list;
+                job.rec = plan->rec;
+                job.proj = plan->proj;
+                const int j = (int)jobs.size();
+                jobs.push_back(job);
+                for (int64_t s = 0; s < hi - lo; s += kChunk)
+                    ev[k].push_back(
+                        EvChunk{j, (int)s, (int)std::min<int64_t>(kChunk, hi - lo - s), 0});
+                std::vector<TileChunk> ch(plan->nchunks);
+                if (!ch.empty() &&
+                    hipMemcpy(ch.data(), plan->chunks, sizeof(TileChunk) * ch.size(),
+                              hipMemcpyDeviceToHost) != hipSuccess) {
+                    rc_status = SRFX_ELAUNCH;
+                    break;
+                }
+                for (const TileChunk &c : ch)
+                    work[k].push_back(TileWork{j, c.tile, c.start, c.count});
+            }
         }
         long long nv = (long long)rc.shape[0] * rc.shape[1] * rc.shape[2];
         vox.push_back(VoxRange{rc.image, rc.emap, rc.bp, nv, off});
         off += nv;
     }
-    srfx_tor_batch *b = new (std::nothrow) srfx_tor_batch;
-    if (!b)
-        return SRFX_ELAUNCH;
-    b->nrec = nrecon;
-    b->nsub = nb_subsets;
+    if (rc_status) {
+        free_plan(b);
+        return rc_status;
+    }
     b->njobs = (int)jobs.size();
     b->total_vox = off;
-    bool ok = hipMalloc(&b->d_jobs, sizeof(TorParams) * jobs.size()) == hipSuccess &&
-              hipMalloc(&b->d_vox, sizeof(VoxRange) * vox.size()) == hipSuccess &&
-              hipMalloc(&b->d_counter, sizeof(unsigned)) == hipSuccess &&
-              hipMemcpy(b->d_jobs, jobs.data(), sizeof(TorParams) * jobs.size(),
-                        hipMemcpyHostToDevice) == hipSuccess &&
-              hipMemcpy(b->d_vox, vox.data(), sizeof(VoxRange) * vox.size(),
-                        hipMemcpyHostToDevice) == hipSuccess;
-    // Per subset: contiguous rows [k n / K, (k+1) n / K) of every group
-    // (graph/reconstruction/worker.py:79-85), cut into chunks.
+    bool ok = upload(&b->d_jobs, jobs) && upload(&b->d_vox, vox) &&
+              hipMalloc(&b->d_counters, 2 * sizeof(unsigned)) == hipSuccess;
     for (int k = 0; ok && k < nb_subsets; k++) {
-        std::vector<Chunk> ch;
-        for (int j = 0; j < b->njobs; j++) {
-            int64_t n = jobs[j].n;
-            int64_t lo = k * n / nb_subsets, hi = (k + 1) * n / nb_subsets;
-            for (int64_t s = lo; s < hi; s += kChunk)
-                ch.push_back(Chunk{j, (int)s, (int)std::min<int64_t>(kChunk, hi - s), 0});
-        }
-        Chunk *d = nullptr;
-        if (!ch.empty()) {
-            ok = hipMalloc(&d, sizeof(Chunk) * ch.size()) == hipSuccess &&
-                 hipMemcpy(d, ch.data(), sizeof(Chunk) * ch.size(), hipMemcpyHostToDevice) ==
-                     hipSuccess;
-        }
-        b->d_chunks.push_back(d);
-        b->nchunks.push_back((int)ch.size());
+        EvChunk *e = nullptr;
+        TileWork *w = nullptr;
+        ok = upload(&e, ev[k]) && upload(&w, work[k]);
+        b->d_ev.push_back(e);
+        b->nev.push_back((int)ev[k].size());
+        b->d_work.push_back(w);
+        b->nwork.push_back((int)work[k].size());
     }
-    int dev = 0, cus = 0, per_cu = 0;
+    b->scatter_lds = (unsigned)(sizeof(double) * kBatchTile.slices * kBatchTile.wx * kBatchTile.wy);
+    int dev = 0, cus = 0, per_cu_p = 0, per_cu_s = 0;
     ok = ok && hipGetDevice(&dev) == hipSuccess &&
          hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
-         hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_osem, 256, 0) ==
-             hipSuccess;
+         hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu_p, batch_project, 256, 0) ==
+             hipSuccess &&
+         hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu_s, batch_scatter, 256,
+                                                      b->scatter_lds) == hipSuccess;
     if (!ok) {
         free_plan(b);
         return SRFX_ELAUNCH;
     }
-    b->osem_blocks = std::max(1, cus * std::max(1, per_cu));
+    b->proj_blocks = std::max(1, cus * std::max(1, per_cu_p));
+    b->scatter_blocks = std::max(1, cus * std::max(1, per_cu_s));
     b->elem_blocks = std::max(1, cus * 4);
     *out = b;
     return SRFX_OK;
@@ -235,10 +343,13 @@ extern "C" int srfx_tor_batch_subset(srfx_tor_batch *b, int32_t subset, void *st
         return SRFX_EINVAL;
     hipStream_t st = (hipStream_t)stream;
     hipLaunchKernelGGL(batch_zero, dim3(b->elem_blocks), dim3(256), 0, st, b->d_vox, b->nrec,
-                       b->total_vox, b->d_counter);
-    if (b->nchunks[subset] > 0)
-        hipLaunchKernelGGL(batch_osem, dim3(b->osem_blocks), dim3(256), 0, st, b->d_jobs,
-                           b->d_chunks[subset], b->nchunks[subset], b->d_counter);
+                       b->total_vox, b->d_counters);
+    if (b->nev[subset] > 0)
+        hipLaunchKernelGGL(batch_project, dim3(b->proj_blocks), dim3(256), 0, st, b->d_jobs,
+                           b->d_ev[subset], b->nev[subset], b->d_counters);
+    if (b->nwork[subset] > 0)
+        hipLaunchKernelGGL(batch_scatter, dim3(b->scatter_blocks), dim3(256), b->scatter_lds, st,
+                           b->d_jobs, b->d_work[subset], b->nwork[subset], b->d_counters + 1);
     hipLaunchKernelGGL(batch_update, dim3(b->elem_blocks), dim3(256), 0, st, b->d_vox, b->nrec,
                        b->total_vox);
     return launch_status();

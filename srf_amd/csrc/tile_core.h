@@ -1,0 +1,457 @@
+// Device side of the tiled TOR ops (tile.hip: plans and their entry points;
+// batch.hip: batched reconstructions over plans), and the plan object.
+//
+// A plan bins every event of a fixed set into the image tiles it can touch;
+// the kernels then gather from / scatter into tiles staged in LDS.
+//
+// A tile is a slab of S consecutive slices times a window of Wx x Wy voxels
+// of the op frame.  Along a TOR group's dominant axis an event moves by at
+// most one voxel pitch per slice, so within one slab its voxels stay inside a
+// small footprint and it touches few tiles.  The plan lists, for every tile,
+// the events whose footprint meets it ("listings"), cut into chunks of at most
+// `chunk` listings; one workgroup takes one chunk:
+//   scatter: zero the tile (FP64) in LDS, every lane walks one listing's
+//            voxels that lie in the tile and adds its weights with LDS
+//            atomics, then the tile is flushed with one coalesced FP32 global
+//            atomic per non-zero voxel (instead of one scattered global atomic
+//            per weight);
+//   gather:  load the tile into LDS, every lane sums its listing's weighted
+//            voxels from LDS into one partial per listing; a second pass sums
+//            each event's partials in the event's own listing order, so the
+//            projection is deterministic.
+// The voxels an event visits and their weights are those of the untiled
+// kernels (tor_core.h): the tile walk is the candidate-box (or chord) walk
+// clipped to the window.
+#pragma once
+
+#include "tor_core.h"
+
+namespace srfx {
+
+struct TileGeom {
+    int S, Wx, Wy;       // slices per slab, window width along op x and y (voxels)
+    int nwx, nwy, nslab; // windows per axis and number of slabs
+};
+
+
+struct TileChunk {
+    int tile, start, count, pad;
+};
+
+namespace {
+
+__device__ __forceinline__ int trunc_clamped(double u)
+{
+    return (int)fmin(fmax(u, -1.0e9), 1.0e9);
+}
+
+// Windows wx0..wx1, wy0..wy1 holding every voxel the event can visit in the
+// slices s0..s1.  The patch origin of slice s is (int)u(s) - half with u
+// affine in s (cross_at), or the reference's exact origin, which differs from
+// it by at most one voxel; the patch spans P voxels from its origin.
+__device__ __forceinline__ bool footprint(const TorParams &p, const TileGeom &g, const Track &t,
+                                          int s0, int s1, int &wx0, int &wx1, int &wy0, int &wy1)
+{
+    int a0 = trunc_clamped(t.ax + (double)s0 * t.bx) - p.half;
+    int a1 = trunc_clamped(t.ax + (double)s1 * t.bx) - p.half;
+    int b0 = trunc_clamped(t.ay + (double)s0 * t.by) - p.half;
+    int b1 = trunc_clamped(t.ay + (double)s1 * t.by) - p.half;
+    int x0 = max(min(a0, a1) - 1, 0), x1 = min(max(a0, a1) + p.patch, p.gx - 1);
+    int y0 = max(min(b0, b1) - 1, 0), y1 = min(max(b0, b1) + p.patch, p.gy - 1);
+    if (x0 > x1 || y0 > y1)
+        return false;
+    wx0 = x0 / g.Wx;
+    wx1 = x1 / g.Wx;
+    wy0 = y0 / g.Wy;
+    wy1 = y1 / g.Wy;
+    return true;
+}
+
+// Calls f(tile) for every tile the event can touch: slabs in slice order,
+// then windows row by row.  The count and fill passes enumerate identically.
+template <class F>
+__device__ __forceinline__ void for_each_tile(const TorParams &p, const TileGeom &g,
+                                              const Event &ev, const Track &t, F f)
+{
+    const int sl0 = ev.s_lo / g.S, sl1 = ev.s_hi / g.S;
+    for (int sl = sl0; sl <= sl1; sl++) {
+        int s0 = max(ev.s_lo, sl * g.S), s1 = min(ev.s_hi, sl * g.S + g.S - 1);
+        int wx0, wx1, wy0, wy1;
+        if (!footprint(p, g, t, s0, s1, wx0, wx1, wy0, wy1))
+            continue;
+        for (int wy = wy0; wy <= wy1; wy++)
+            for (int wx = wx0; wx <= wx1; wx++)
+                f((sl * g.nwy + wy) * g.nwx + wx);
+    }
+}
+
+struct PlanArgs {
+    int cols7, ntiles;
+    int *tile_cnt;        // count pass: listings per tile; fill pass: read
+    int *ev_cnt;          // count pass: listings per event (nullable)
+    float4 *rec;          // count pass writes, fill pass reads: 48-byte event records
+    const int *ev_off;    // fill pass: first listing of each event (nullable)
+    const int *tile_off;  // fill pass: first slot of each tile
+    int *tile_fill;       // fill pass: slots taken per tile
+    int *list;            // fill pass: event of each slot
+    int *ev_slot;         // fill pass: slot of each of an event's listings (nullable)
+    int *err;             // fill pass: count of listings that did not fit
+};
+
+__device__ __forceinline__ bool plan_init(const TorParams &p, int cols7, const RawEvent &r,
+                                          Event &ev)
+{
+    return cols7 ? init_event<kMap>(p, r, ev) : init_event<kProj>(p, r, ev);
+}
+
+// Count pass (FILL false): pack the event's record, count its tiles per
+// event and per tile.  Fill pass (FILL true): list the event in each of its
+// tiles.  With LDS, a workgroup first counts its own listings per tile in
+// LDS, then adds them to the global counters (count pass) or reserves a
+// contiguous range of slots per tile with one global atomic (fill pass), so
+// the global atomics are per (workgroup, tile) instead of per listing and a
+// workgroup's slots of one tile are contiguous.  Both passes enumerate the
+// tiles with the same code; the fill pass checks every slot against the
+// counted ranges and reports a mismatch in *err instead of writing out of
+// range.
+template <bool FILL, bool LDS>
+__global__ __launch_bounds__(256) void plan_pass(TorParams p, TileGeom g, PlanArgs a)
+{
+    extern __shared__ int sh[]; // LDS: counts [ntiles], FILL: + bases, ranks [ntiles] each
+    const int nt = a.ntiles;
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = e < p.n;
+    if (LDS) {
+        for (int i = threadIdx.x; i < (FILL ? 3 : 1) * nt; i += blockDim.x)
+            sh[i] = 0;
+        __syncthreads();
+    }
+    RawEvent r;
+    Event ev;
+    Track t;
+    bool ok = false;
+    if (valid) {
+        if (FILL) {
+            fetch_event_rec(a.rec, e, r);
+        } else {
+            if (a.cols7)
+                fetch_event<kMap>(p, e, r);
+            else
+                fetch_event<kProj>(p, e, r);
+            a.rec[3 * e] = make_float4(r.x1, r.y1, r.z1, r.x2);
+            a.rec[3 * e + 1] = make_float4(r.y2, r.z2, r.xc, r.yc);
+            a.rec[3 * e + 2] = make_float4(r.zc, r.f, 0.0f, 0.0f);
+        }
+        ok = plan_init(p, a.cols7, r, ev);
+        if (ok)
+            make_track(p, ev, t);
+    }
+    if (!FILL) {
+        int k = 0;
+        if (ok)
+            for_each_tile(p, g, ev, t, [&](int tile) {
+                atomicAdd(LDS ? &sh[tile] : &a.tile_cnt[tile], 1);
+                k++;
+            });
+        if (valid && a.ev_cnt)
+            a.ev_cnt[e] = k;
+        if (LDS) {
+            __syncthreads();
+            for (int i = threadIdx.x; i < nt; i += blockDim.x) {
+                int c = sh[i];
+                if (c)
+                    atomicAdd(&a.tile_cnt[i], c);
+            }
+        }
+        return;
+    }
+    if (LDS) {
+        if (ok)
+            for_each_tile(p, g, ev, t, [&](int tile) { atomicAdd(&sh[tile], 1); });
+        __syncthreads();
+        for (int i = threadIdx.x; i < nt; i += blockDim.x) {
+            int c = sh[i];
+            sh[nt + i] = c ? atomicAdd(&a.tile_fill[i], c) : 0;
+        }
+        __syncthreads();
+    }
+    const int k0 = (valid && a.ev_slot) ? a.ev_off[e] : 0;
+    const int k1 = (valid && a.ev_slot) ? a.ev_off[e + 1] : 0;
+    int k = 0;
+    if (ok)
+        for_each_tile(p, g, ev, t, [&](int tile) {
+            int rank;
+            bool fits;
+            if (LDS) {
+                int loc = atomicAdd(&sh[2 * nt + tile], 1);
+                rank = sh[nt + tile] + loc;
+                fits = loc < sh[tile];
+            } else {
+                rank = atomicAdd(&a.tile_fill[tile], 1);
+                fits = true;
+            }
+            fits = fits && rank < a.tile_cnt[tile] && (!a.ev_slot || k0 + k < k1);
+            if (fits) {
+                int slot = a.tile_off[tile] + rank;
+                a.list[slot] = (int)e;
+                if (a.ev_slot)
+                    a.ev_slot[k0 + k] = slot;
+            } else {
+                atomicAdd(a.err, 1);
+            }
+            k++;
+        });
+    if (valid && a.ev_slot && k0 + k != k1)
+        atomicAdd(a.err, 1);
+}
+
+__global__ __launch_bounds__(256) void fill_int(int *v, int64_t n, int value)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        v[i] = value;
+}
+
+// Tile bounds of a chunk: slices sa..sb, op-frame voxels x0..x1, y0..y1.
+} // namespace
+
+struct TileBox {
+    int sa, sb, x0, x1, y0, y1;
+};
+
+namespace {
+
+__device__ __forceinline__ TileBox tile_box(const TorParams &p, const TileGeom &g, int tile)
+{
+    TileBox b;
+    const int per = g.nwx * g.nwy;
+    const int sl = tile / per, r = tile - sl * per;
+    const int wy = r / g.nwx, wx = r - wy * g.nwx;
+    b.sa = sl * g.S;
+    b.sb = min(b.sa + g.S, p.gz) - 1;
+    b.x0 = wx * g.Wx;
+    b.x1 = min(b.x0 + g.Wx, p.gx) - 1;
+    b.y0 = wy * g.Wy;
+    b.y1 = min(b.y0 + g.Wy, p.gy) - 1;
+    return b;
+}
+
+// The chord walk of one slice (patch_walk) clipped to the tile's window.
+//   KIND 0: returns sum w * lds[voxel]; KIND 1: lds[voxel] += w * scale.
+template <int KIND, typename T>
+__device__ __forceinline__ float window_walk(const TorParams &p, const TileGeom &g,
+                                             const TileBox &b, const Event &ev, const Track &t,
+                                             const Cross &c, float amp, int s, float scale,
+                                             T *lds)
+{
+    const int P = p.patch;
+    int i_lo = max(0, b.x0 - c.ix0), i_hi = min(P - 1, b.x1 - c.ix0);
+    int j_lo = max(0, b.y0 - c.iy0), j_hi = min(P - 1, b.y1 - c.iy0);
+    const float eps = 1.0e-4f;
+    j_lo = max(j_lo, (int)ceilf((c.dy0 - t.dymax) * p.rfinter_y - eps));
+    j_hi = min(j_hi, (int)floorf((c.dy0 + t.dymax) * p.rfinter_y + eps));
+    const int row0 = ((s - b.sa) * g.Wy + (c.iy0 - b.y0)) * g.Wx + (c.ix0 - b.x0);
+    float acc = 0.0f;
+    for (int j = j_lo; j <= j_hi; j++) {
+        float dy = voxel_offset(c.dy0, j, p.finter_y);
+        float tt = t.tm - dy * dy * t.q;
+        if (tt <= 0.0f)
+            continue;
+        float hw = __builtin_amdgcn_sqrtf(tt * t.A);
+        float ctr = c.dx0 - t.cxy * dy;
+        int il = max(i_lo, (int)ceilf((ctr - hw) * p.rfinter_x - eps));
+        int ih = min(i_hi, (int)floorf((ctr + hw) * p.rfinter_x + eps));
+        for (int i = il; i <= ih; i++) {
+            float dx = voxel_offset(c.dx0, i, p.finter_x);
+            float d2 = tube_dist2(dx, dy, ev.ux, ev.uy);
+            bool in = d2 < ev.thr - kThrMargin;
+            if (!in && d2 < ev.thr + kThrMargin)
+                in = inside_exact(p, ev, s, c.ix0 + i, c.iy0 + j);
+            if (in) {
+                float w = amp * __builtin_amdgcn_exp2f(d2 * ev.cw);
+                int li = row0 + j * g.Wx + i;
+                if constexpr (KIND == 0)
+                    acc = fmaf(w, lds[li], acc);
+                else
+                    atomicAdd(&lds[li], (T)(w * scale));
+            }
+        }
+    }
+    return acc;
+}
+
+// One listing: the event's slices inside the tile, each through the NB x NB
+// candidate box clipped to the window when the event's box fits, else the
+// clipped chord walk.
+template <int KIND, int NB, typename T>
+__device__ __forceinline__ float tile_event(const TorParams &p, const TileGeom &g,
+                                            const TileBox &b, const Event &ev, bool tof,
+                                            float scale, T *lds)
+{
+    Track t;
+    make_track(p, ev, t);
+    const bool boxed = NB > 0 && t.nx <= NB && t.ny <= NB;
+    const int s0 = max(ev.s_lo, b.sa), s1 = min(ev.s_hi, b.sb);
+    const int P = p.patch;
+    float acc = 0.0f;
+    for (int s = s0; s <= s1; s++) {
+        Cross c;
+        cross_at(p, ev, t, s, c);
+        float amp = tof ? tof_factor(p, ev, t, s) : 1.0f;
+        if (NB > 0 && boxed) {
+            PatchIdx idx;
+            idx.base = ((s - b.sa) * g.Wy + (c.iy0 - b.y0)) * g.Wx + (c.ix0 - b.x0);
+            idx.si = 1;
+            idx.sj = g.Wx;
+            const float *src = nullptr;
+            if constexpr (KIND == 0)
+                src = lds;
+            acc += slice_box<KIND, (NB > 0 ? NB : 1), true>(
+                p, ev, t, c, amp, s, scale, max(0, b.x0 - c.ix0), min(P - 1, b.x1 - c.ix0),
+                max(0, b.y0 - c.iy0), min(P - 1, b.y1 - c.iy0), idx, src, lds);
+        } else {
+            acc += window_walk<KIND>(p, g, b, ev, t, c, amp, s, scale, lds);
+        }
+    }
+    return acc;
+}
+
+// Scatter of one chunk (modes kBack, kMap, kAdj) into p.out, through the
+// FP64 LDS tile ldsd (S * Wy * Wx doubles).  Every thread of the workgroup
+// calls it; it synchronises the workgroup before and after the flush.
+template <int MODE, int NB>
+__device__ __forceinline__ void scatter_chunk(const TorParams &p, const TileGeom &g,
+                                              const TileChunk &ch,
+                                              const int *__restrict__ list,
+                                              const float4 *__restrict__ rec, double *ldsd)
+{
+    const TileBox b = tile_box(p, g, ch.tile);
+    const int nv = g.S * g.Wy * g.Wx;
+    for (int i = threadIdx.x; i < nv; i += blockDim.x)
+        ldsd[i] = 0.0;
+    __syncthreads();
+    const bool tof = MODE != kMap && p.use_tof;
+    // The 64 lanes of a wave take listings `stride` apart in the chunk:
+    // neighbours in the list are neighbouring (often near-identical) events,
+    // whose LDS atomics would hit the same voxels in the same instruction.
+    const int stride = (ch.count + 63) >> 6;
+    for (int q = threadIdx.x; q < 64 * stride; q += blockDim.x) {
+        const int j = (q & 63) * stride + (q >> 6);
+        if (j >= ch.count)
+            continue;
+        const int64_t e = list[ch.start + j];
+        float scale;
+        bool go = true;
+        if (MODE == kBack) {
+            float v = p.val[e];
+            go = v > 1e-7;
+            scale = 1.0f / v;
+        } else if (MODE == kMap) {
+            scale = 1.0f / p.val[e];
+        } else {
+            scale = p.val[e];
+            go = scale != 0.0f;
+        }
+        if (go) {
+            RawEvent r;
+            Event ev;
+            fetch_event_rec(rec, e, r);
+            if (init_event<MODE>(p, r, ev))
+                tile_event<1, NB>(p, g, b, ev, tof, scale, ldsd);
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < nv; i += blockDim.x) {
+        float v = (float)ldsd[i];
+        if (v != 0.0f) {
+            int lx = i % g.Wx, r = i / g.Wx;
+            int ly = r % g.Wy, ls = r / g.Wy;
+            int ix = b.x0 + lx, iy = b.y0 + ly, s = b.sa + ls;
+            if (ix <= b.x1 && iy <= b.y1 && s <= b.sb)
+                unsafeAtomicAdd(p.out + (ix * p.ox + iy * p.oy + s * p.oz), v);
+        }
+    }
+    __syncthreads();
+}
+
+// One workgroup per chunk.  The LDS tile is FP64: LDS FP64 atomic adds run
+// at several lanes per cycle where FP32 ones are serialised lane by lane,
+// and the tile sums gain precision.
+template <int MODE, int NB>
+__global__ __launch_bounds__(256) void tile_scatter(TorParams p, TileGeom g,
+                                                    const TileChunk *__restrict__ chunks,
+                                                    const int *__restrict__ list,
+                                                    const float4 *__restrict__ rec)
+{
+    extern __shared__ double ldsd[];
+    scatter_chunk<MODE, NB>(p, g, chunks[blockIdx.x], list, rec, ldsd);
+}
+
+// Gather of one chunk: partial[slot] = the listing's projection over the tile.
+template <int NB>
+__global__ __launch_bounds__(256) void tile_gather(TorParams p, TileGeom g,
+                                                   const TileChunk *__restrict__ chunks,
+                                                   const int *__restrict__ list,
+                                                   const float4 *__restrict__ rec, float *partial)
+{
+    extern __shared__ float lds[];
+    const TileChunk ch = chunks[blockIdx.x];
+    const TileBox b = tile_box(p, g, ch.tile);
+    const int nv = g.S * g.Wy * g.Wx;
+    for (int i = threadIdx.x; i < nv; i += blockDim.x) {
+        int lx = i % g.Wx, r = i / g.Wx;
+        int ly = r % g.Wy, ls = r / g.Wy;
+        int ix = b.x0 + lx, iy = b.y0 + ly, s = b.sa + ls;
+        float v = 0.0f;
+        if (ix <= b.x1 && iy <= b.y1 && s <= b.sb)
+            v = p.img[ix * p.sx + iy * p.sy + s * p.sz];
+        lds[i] = v;
+    }
+    __syncthreads();
+    const bool tof = p.use_tof;
+    for (int j = threadIdx.x; j < ch.count; j += blockDim.x) {
+        const int64_t e = list[ch.start + j];
+        RawEvent r;
+        Event ev;
+        fetch_event_rec(rec, e, r);
+        float acc = 0.0f;
+        if (init_event<kProj>(p, r, ev))
+            acc = tile_event<0, NB>(p, g, b, ev, tof, 0.0f, lds);
+        partial[ch.start + j] = acc;
+    }
+}
+
+// proj[e] = sum of the event's partials in its listing order.
+__global__ __launch_bounds__(256) void tile_reduce(const float *__restrict__ partial,
+                                                   const int *__restrict__ ev_off,
+                                                   const int *__restrict__ ev_slot, int64_t n,
+                                                   float *proj)
+{
+    int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n)
+        return;
+    float v = 0.0f;
+    for (int k = ev_off[e]; k < ev_off[e + 1]; k++)
+        v += partial[ev_slot[k]];
+    proj[e] = v;
+}
+
+} // namespace
+} // namespace srfx
+
+// Plan object of srfx_tor_plan_create.
+struct srfx_tor_plan_s {
+    srfx::TorParams p; // geometry, model and events; image / value pointers set per call
+    srfx_grid grid;
+    srfx::TileGeom g;
+    int cols7;
+    int64_t nlist;
+    int nchunks;
+    int block;
+    int *list;             // event of each slot, tiles in order
+    float4 *rec;           // 48-byte record of each event
+    srfx::TileChunk *chunks;
+    float *proj;           // projection buffer of the OSEM step
+    int *ev_off, *ev_slot; // tiled projection only: each event's slots
+    float *partial;        // tiled projection only: per-slot partial sums
+};

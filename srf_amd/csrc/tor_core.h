@@ -33,6 +33,8 @@ enum TorMode { kProj = 0, kBack = 1, kMap = 2, kAdj = 3, kCount = 4, kOsem = 5 }
 constexpr float kLog2e = 1.4426950408889634f;
 constexpr float kThrMargin = 1.0e-4f; // mm^2 band around the cut-off decided in FP64
 
+} // namespace
+
 struct TorParams {
     const float *__restrict__ lors;
     int64_t n, ld;
@@ -60,6 +62,7 @@ struct TorParams {
     float tofk;    // tof_bin / sqrt(2 pi tof_sigma2)
 };
 
+
 struct Event {
     double x1, y1, z1, d0, d1, d2;
     double rd2;       // 1 / d2
@@ -72,6 +75,8 @@ struct Event {
     float ctof;       // per-slice TOF exponent factor (log2 domain)
     int s_lo, s_hi;   // strictly crossed slices
 };
+
+namespace {
 
 __device__ __forceinline__ double slice_z(const TorParams &p, int s)
 {
@@ -131,9 +136,13 @@ __device__ __forceinline__ bool slice_range(const TorParams &p, Event &ev)
 
 // Raw event record: end points, TOF centre (zero for Maplors rows) and the
 // sigma2 factor.
+} // namespace
+
 struct RawEvent {
     float x1, y1, z1, x2, y2, z2, xc, yc, zc, f;
 };
+
+namespace {
 
 template <int MODE>
 __device__ __forceinline__ void fetch_event(const TorParams &p, int64_t e, RawEvent &r)
@@ -233,6 +242,8 @@ __device__ __forceinline__ bool init_event(const TorParams &p, const RawEvent &r
 // in voxel units from the grid corner, is affine in s: u(s) = ax + s*bx,
 // w(s) = ay + s*by (FP64).  The vector from the crossing point to the TOF
 // centre is v0 - (s - s_lo)*dv (FP32).
+} // namespace
+
 struct Track {
     double ax, bx, ay, by;
     float v0x, v0y, v0z, dvx, dvy, dvz;
@@ -245,6 +256,8 @@ struct Track {
     float hx;    // sqrt(tm (A + cxy^2 / q)): half width of the ellipse
     int nx, ny;  // voxel columns / rows any slice's ellipse can span
 };
+
+namespace {
 
 __device__ __forceinline__ void make_track(const TorParams &p, const Event &ev, Track &t)
 {
@@ -277,10 +290,14 @@ __device__ __forceinline__ void make_track(const TorParams &p, const Event &ev, 
     t.ny = (int)fminf(floorf(2.0f * t.dymax * p.rfinter_y + 2.0e-4f), 1.0e6f) + 1;
 }
 
+} // namespace
+
 struct Cross {
     int ix0, iy0;
     float dx0, dy0; // crossing point minus centre of the first patch voxel, mm
 };
+
+namespace {
 
 __device__ __forceinline__ bool near_int(double u)
 {
@@ -461,10 +478,14 @@ __device__ __forceinline__ float patch_walk(const TorParams &p, const Event &ev,
 
 // Index of patch voxel (i, j) of one slice in the accessed array: the
 // strided image, or an LDS tile.
+} // namespace
+
 struct PatchIdx {
     int base, si, sj;
     __device__ __forceinline__ int operator()(int i, int j) const { return base + i * si + j * sj; }
 };
+
+namespace {
 
 // One slice through the event's candidate box: the NB x NB voxels (at most
 // t.nx columns, t.ny rows) starting at the first column / row whose centre can

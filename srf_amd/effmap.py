@@ -60,15 +60,24 @@ def initial_image(grid, center, size, inner_radius, axial_length, device, crop_r
 
 
 def compute_tor_effmap(scanner, model, grid, center, size, device, chunk=8_000_000,
-                       max_pairs=None):
-    """Sum over all unordered crystal pairs of Maplors (values 1), (nx, ny, nz)."""
+                       max_pairs=None, tiled=True):
+    """Sum over all unordered crystal pairs of Maplors (values 1), (nx, ny, nz).
+    Pairs are enumerated on the device; with tiled=True each chunk's map LORs
+    are binned into image tiles once and scattered through LDS tiles."""
     out = torch.zeros(tuple(grid), dtype=torch.float32, device=device)
     img = Image(out, center, size)
     total = scanner.nb_pairs() if max_pairs is None else min(max_pairs, scanner.nb_pairs())
     for start in range(0, total, chunk):
         stop = min(total, start + chunk)
-        pairs = torch.from_numpy(scanner.all_pairs((start, stop))).to(device)
+        pairs = scanner.all_pairs_device((start, stop), device)
         groups = map_process(pairs)
         data = split_from_groups(groups, without_tof=True)
-        model.maplors(data, img, out=out)
+        if tiled:
+            plans = model.make_plans(img, data, ncols=7)
+            model.maplors(data, img, out=out, plans=plans)
+            for p in plans.values():
+                if p is not None:
+                    p.close()
+        else:
+            model.maplors(data, img, out=out)
     return out

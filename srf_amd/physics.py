@@ -158,8 +158,10 @@ class SplitLoRsModel:
                       _ptr(out), _lib.SRFX_ACCUMULATE | self._flags("backprojection"), _stream())
         return Image(out, image.center, image.size)
 
-    def maplors(self, data, image, out=None):
-        """split_lors.py:116-133: sum over groups of Maplors(lors_value)."""
+    def maplors(self, data, image, out=None, plans=None):
+        """split_lors.py:116-133: sum over groups of Maplors(lors_value).
+        With ``plans`` (make_plans(..., ncols=7) of the same events) the tiled
+        kernels run."""
         x = _world(image)
         if out is None:
             out = torch.zeros_like(x)
@@ -169,6 +171,9 @@ class SplitLoRsModel:
             if n == 0:
                 continue
             v = _vec(lm.values, n, "lors_value")
+            if plans is not None:
+                self._planned(plans, a, n, out).maplors(v, out=out, accumulate=True)
+                continue
             _lib.call("srfx_tor_maplors", _ptr(lors), n, ld, _ptr(v),
                       self._grid(Image(out, image.center, image.size), a),
                       self.kernel_width, 1 if self.geo_sigma2_flag else 0, _ptr(out),

@@ -78,6 +78,30 @@ class CylindricalScanner:
         pb = self.crystal_xyz(b // self.nb_crystals, b % self.nb_crystals)
         return np.hstack([pa, pb]).astype(np.float32)
 
+    def crystal_table(self):
+        """(nb_rings * nb_crystals, 3) float32 centres, indexed by crystal id
+        k = ring * nb_crystals + crystal (the values all_pairs uses)."""
+        k = np.arange(self.nb_rings * self.nb_crystals)
+        return self.crystal_xyz(k // self.nb_crystals, k % self.nb_crystals).astype(np.float32)
+
+    def all_pairs_device(self, chunk, device):
+        """all_pairs(chunk) built on ``device`` (torch, float32 [N, 6]); the
+        same pairs in the same order and the same float32 values."""
+        import torch
+        nc = self.nb_rings * self.nb_crystals
+        start, stop = chunk
+        idx = torch.arange(start, stop, dtype=torch.int64, device=device)
+        q = (2 * nc - 1) ** 2 - 8.0 * idx.double()
+        a = torch.floor((2 * nc - 1 - torch.sqrt(q)) / 2).long()
+
+        def off(a):
+            return a * nc - a * (a + 1) // 2
+        a = torch.where(idx < off(a), a - 1, a)
+        a = torch.where(idx >= off(a) + (nc - 1 - a), a + 1, a)
+        b = idx - off(a) + a + 1
+        table = torch.from_numpy(self.crystal_table()).to(device)
+        return torch.cat([table[a], table[b]], dim=1)
+
     def nb_pairs(self):
         nc = self.nb_rings * self.nb_crystals
         return nc * (nc - 1) // 2

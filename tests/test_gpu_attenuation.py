@@ -54,3 +54,24 @@ def test_attenuated_effmap(cuda):
     inner = b > 0.01 * b.max()
     ratio = (a[inner] / b[inner])
     assert ratio.min() > np.exp(-0.0096 * 300 * 1.05) and ratio.max() <= 1.0 + 1e-5
+
+
+def test_tiled_effmap_equals_untiled(cuda):
+    """Maplors over crystal pairs through tile plans (the default) against the
+    untiled scatter, plain and attenuated."""
+    from srf_amd import attenuation as A, effmap, physics
+    from srf_amd.data import Image
+    from srf_amd.scanner import CylindricalScanner
+    sc = CylindricalScanner()
+    model = physics.SplitLoRsModel(4.0, False)
+    grid, size = (100, 100, 50), (400.0, 400.0, 110.0)
+    n = 400000
+    t = effmap.compute_tor_effmap(sc, model, grid, CENTER, size, cuda, max_pairs=n, chunk=150000)
+    u = effmap.compute_tor_effmap(sc, model, grid, CENTER, size, cuda, max_pairs=n, chunk=150000,
+                                  tiled=False)
+    assert float(u.abs().sum()) > 0
+    assert H.rel_l2(t.cpu().numpy(), u.cpu().numpy()) <= 1e-6
+    mu = Image(A.water_cylinder(grid, CENTER, size, 150.0, 54.4, device=cuda), CENTER, size)
+    at = A.attenuated_tor_effmap(sc, model, mu, grid, CENTER, size, cuda, max_pairs=n)
+    au = A.attenuated_tor_effmap(sc, model, mu, grid, CENTER, size, cuda, max_pairs=n, tiled=False)
+    assert H.rel_l2(at.cpu().numpy(), au.cpu().numpy()) <= 1e-6
