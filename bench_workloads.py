@@ -380,7 +380,12 @@ def large_c5(args):
                                                size, dev, max_pairs=args.effmap_pairs or None)
     E = effmap.finalize(summed, grid, center, size, sc.inner_radius, sc.axial_length)
     torch.cuda.synchronize()
-    log(f"attenuated efficiency map in {time.time() - t0:.1f}s")
+    eff_s = time.time() - t0
+    eff_stats = {"pairs": args.effmap_pairs or sc.nb_pairs(), "seconds": round(eff_s, 2),
+                 "sum": float(summed.double().sum()),
+                 "nonzero_voxels": int((summed > 0).sum()),
+                 "inverse_map_nonzero_voxels": int((E > 0).sum())}
+    log(f"attenuated efficiency map in {eff_s:.1f}s: {eff_stats}")
     x = effmap.initial_image(grid, center, size, sc.inner_radius, sc.axial_length, dev)
     K = 20
     t0 = time.time()
@@ -413,4 +418,5 @@ def large_c5(args):
          "events": nev, "subsets": K, "tof_fwhm_ps": 400, "attenuation": "mu 0.0096/mm body",
          "step": "one OSEM iteration over all events", "tiled": True, "sorted": True,
          "plan_build_s": round(plan_s, 3), "listings_per_event": round(listings / nev, 2),
-         "peak_gib": round(torch.cuda.max_memory_allocated() / 2**30, 1)})), flush=True)
+         "peak_gib": round(torch.cuda.max_memory_allocated() / 2**30, 1),
+         "effmap": eff_stats})), flush=True)

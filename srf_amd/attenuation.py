@@ -52,10 +52,11 @@ def water_cylinder(grid, center, size, radius, half_height, mu=0.0096, device="c
 
 
 def attenuated_tor_effmap(scanner, model, mu, grid, center, size, device, chunk=4_000_000,
-                          max_pairs=None, tiled=True):
+                          max_pairs=None, tiled=False):
     """Sum over unordered crystal pairs of Maplors with the attenuation
     weights as lors_value: sum_i w_ij exp(-int mu dl_i).  Pairs are
-    enumerated on the device; tiled=True scatters through tile plans."""
+    enumerated on the device; tiled=True scatters through tile plans (see
+    effmap.compute_tor_effmap for the default)."""
     out = torch.zeros(tuple(grid), dtype=torch.float32, device=device)
     img = Image(out, center, size)
     total = scanner.nb_pairs() if max_pairs is None else min(max_pairs, scanner.nb_pairs())

@@ -60,10 +60,12 @@ def initial_image(grid, center, size, inner_radius, axial_length, device, crop_r
 
 
 def compute_tor_effmap(scanner, model, grid, center, size, device, chunk=8_000_000,
-                       max_pairs=None, tiled=True):
+                       max_pairs=None, tiled=False):
     """Sum over all unordered crystal pairs of Maplors (values 1), (nx, ny, nz).
-    Pairs are enumerated on the device; with tiled=True each chunk's map LORs
-    are binned into image tiles once and scattered through LDS tiles."""
+    Pairs are enumerated on the device.  tiled=True bins each chunk's map LORs
+    into image tiles and scatters through LDS tiles; for the single pass of a
+    map the plan costs more than it saves (0.98 s against 0.77 s for the 75.5M
+    pairs at 200x200x100), so the default is the direct scatter."""
     out = torch.zeros(tuple(grid), dtype=torch.float32, device=device)
     img = Image(out, center, size)
     total = scanner.nb_pairs() if max_pairs is None else min(max_pairs, scanner.nb_pairs())

@@ -66,12 +66,13 @@ def test_tiled_effmap_equals_untiled(cuda):
     model = physics.SplitLoRsModel(4.0, False)
     grid, size = (100, 100, 50), (400.0, 400.0, 110.0)
     n = 400000
-    t = effmap.compute_tor_effmap(sc, model, grid, CENTER, size, cuda, max_pairs=n, chunk=150000)
+    t = effmap.compute_tor_effmap(sc, model, grid, CENTER, size, cuda, max_pairs=n, chunk=150000,
+                                  tiled=True)
     u = effmap.compute_tor_effmap(sc, model, grid, CENTER, size, cuda, max_pairs=n, chunk=150000,
                                   tiled=False)
     assert float(u.abs().sum()) > 0
     assert H.rel_l2(t.cpu().numpy(), u.cpu().numpy()) <= 1e-6
     mu = Image(A.water_cylinder(grid, CENTER, size, 150.0, 54.4, device=cuda), CENTER, size)
-    at = A.attenuated_tor_effmap(sc, model, mu, grid, CENTER, size, cuda, max_pairs=n)
+    at = A.attenuated_tor_effmap(sc, model, mu, grid, CENTER, size, cuda, max_pairs=n, tiled=True)
     au = A.attenuated_tor_effmap(sc, model, mu, grid, CENTER, size, cuda, max_pairs=n, tiled=False)
     assert H.rel_l2(at.cpu().numpy(), au.cpu().numpy()) <= 1e-6
