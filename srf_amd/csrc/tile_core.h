@@ -280,6 +280,35 @@ __device__ __forceinline__ float window_walk(const TorParams &p, const TileGeom 
     return acc;
 }
 
+// Narrow [s0, s1] to a superset of the slices whose patch can meet the
+// window columns x0..x1 and rows y0..y1.  The patch origin is trunc(u) - half
+// (or the exact origin, one voxel away) with u(s) = ax + s bx, so the patch
+// meets the columns only if x0 - P + half - 2 < u(s) < x1 + half + 2; the
+// bounds on s get one more slice of slack on each side.
+__device__ __forceinline__ void axis_slices(double a, double b, double lo, double hi, int &s0,
+                                            int &s1)
+{
+    if (fabs(b) < 1.0e-9) {
+        // u is constant to 1e-9 voxel per slice: test it at both ends
+        const double u0 = a + (double)s0 * b, u1 = a + (double)s1 * b;
+        if (fmax(u0, u1) <= lo - 1.0 || fmin(u0, u1) >= hi + 1.0)
+            s1 = s0 - 1;
+        return;
+    }
+    const double r = 1.0 / b;
+    const double t1 = (lo - a) * r, t2 = (hi - a) * r;
+    const double tl = fmax(fmin(t1, t2), -1.0e8), th = fmin(fmax(t1, t2), 1.0e8);
+    s0 = max(s0, (int)floor(tl) - 1);
+    s1 = min(s1, (int)ceil(th) + 1);
+}
+
+__device__ __forceinline__ void window_slices(const Track &t, int half, int P, int x0, int x1,
+                                              int y0, int y1, int &s0, int &s1)
+{
+    axis_slices(t.ax, t.bx, (double)(x0 - P + half - 2), (double)(x1 + half + 2), s0, s1);
+    axis_slices(t.ay, t.by, (double)(y0 - P + half - 2), (double)(y1 + half + 2), s0, s1);
+}
+
 // One listing: the event's slices inside the tile, each through the NB x NB
 // candidate box clipped to the window when the event's box fits, else the
 // clipped chord walk.
@@ -290,15 +319,17 @@ __device__ __forceinline__ float tile_event(const TorParams &p, const TileGeom &
 {
     Track t;
     make_track(p, ev, t);
-    const bool boxed = NB > 0 && t.nx <= NB && t.ny <= NB;
-    const int s0 = max(ev.s_lo, b.sa), s1 = min(ev.s_hi, b.sb);
+    constexpr int NBB = NB > 1 ? NB : 2;
+    const int shape = box_shape<NB>(t);
+    int s0 = max(ev.s_lo, b.sa), s1 = min(ev.s_hi, b.sb);
     const int P = p.patch;
+    window_slices(t, p.half, P, b.x0, b.x1, b.y0, b.y1, s0, s1);
     float acc = 0.0f;
     for (int s = s0; s <= s1; s++) {
         Cross c;
         cross_at(p, ev, t, s, c);
         float amp = tof ? tof_factor(p, ev, t, s) : 1.0f;
-        if (NB > 0 && boxed) {
+        if (NB > 0 && shape) {
             PatchIdx idx;
             idx.base = ((s - b.sa) * g.Wy + (c.iy0 - b.y0)) * g.Wx + (c.ix0 - b.x0);
             idx.si = 1;
@@ -306,9 +337,14 @@ __device__ __forceinline__ float tile_event(const TorParams &p, const TileGeom &
             const float *src = nullptr;
             if constexpr (KIND == 0)
                 src = lds;
-            acc += slice_box<KIND, (NB > 0 ? NB : 1), true>(
-                p, ev, t, c, amp, s, scale, max(0, b.x0 - c.ix0), min(P - 1, b.x1 - c.ix0),
-                max(0, b.y0 - c.iy0), min(P - 1, b.y1 - c.iy0), idx, src, lds);
+            const int il = max(0, b.x0 - c.ix0), ih = min(P - 1, b.x1 - c.ix0);
+            const int jl = max(0, b.y0 - c.iy0), jh = min(P - 1, b.y1 - c.iy0);
+            if (shape == 1)
+                acc += slice_box<KIND, NBB, NBB - 1, true>(p, ev, t, c, amp, s, scale, il, ih, jl,
+                                                           jh, idx, src, lds);
+            else
+                acc += slice_box<KIND, NBB, NBB, true>(p, ev, t, c, amp, s, scale, il, ih, jl, jh,
+                                                       idx, src, lds);
         } else {
             acc += window_walk<KIND>(p, g, b, ev, t, c, amp, s, scale, lds);
         }

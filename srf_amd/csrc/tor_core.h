@@ -498,34 +498,38 @@ namespace {
 // arithmetic as patch_walk.
 //   KIND 0: returns sum w * src[idx]; 1: dst[idx] += w * scale (LDS: LDS
 //   atomics in dst's type, else FP32 global atomics); 2: returns the count.
-template <int KIND, int NB, bool LDS, typename D>
+template <int KIND, int BX, int BY, bool LDS, typename D>
 __device__ __forceinline__ float slice_box(const TorParams &p, const Event &ev, const Track &t,
                                            const Cross &c, float amp, int s, float scale, int i_lo,
                                            int i_hi, int j_lo, int j_hi, const PatchIdx &idx,
                                            const float *src, D *dst)
 {
+    constexpr int NC = BX * BY;
     const float eps = 1.0e-4f;
     const int ib0 = (int)ceilf((c.dx0 - t.hx) * p.rfinter_x - eps);
     const int jb0 = (int)ceilf((c.dy0 - t.dymax) * p.rfinter_y - eps);
     const int ib = max(i_lo, ib0), ie = min(i_hi, ib0 + t.nx - 1);
     const int jb = max(j_lo, jb0), je = min(j_hi, jb0 + t.ny - 1);
-    float cx2[NB], cux[NB], ry2[NB], ruy[NB];
+    float cx2[BX], cux[BX], ry2[BY], ruy[BY];
 #pragma unroll
-    for (int k = 0; k < NB; k++) {
+    for (int k = 0; k < BX; k++) {
         float dx = voxel_offset(c.dx0, ib + k, p.finter_x);
-        float dy = voxel_offset(c.dy0, jb + k, p.finter_y);
         cx2[k] = dx * dx;
         cux[k] = ev.ux * dx;
+    }
+#pragma unroll
+    for (int k = 0; k < BY; k++) {
+        float dy = voxel_offset(c.dy0, jb + k, p.finter_y);
         ry2[k] = dy * dy;
         ruy[k] = ev.uy * dy;
     }
-    float d2[NB * NB];
+    float d2[NC];
     uint32_t in = 0, near = 0;
 #pragma unroll
-    for (int jj = 0; jj < NB; jj++) {
+    for (int jj = 0; jj < BY; jj++) {
 #pragma unroll
-        for (int ii = 0; ii < NB; ii++) {
-            const int k = jj * NB + ii;
+        for (int ii = 0; ii < BX; ii++) {
+            const int k = jj * BX + ii;
             const bool ok = ib + ii <= ie && jb + jj <= je;
             d2[k] = tube_d2(cx2[ii], cux[ii], ry2[jj], ruy[jj]);
             const bool a = ok && d2[k] < ev.thr - kThrMargin;
@@ -538,7 +542,7 @@ __device__ __forceinline__ float slice_box(const TorParams &p, const Event &ev, 
         // rare: the reference's FP64 decision, one voxel at a time
         for (uint32_t m = (uint32_t)opaque((int)near); m; m &= m - 1) {
             const int k = __builtin_ctz(m);
-            const int ii = k % NB, jj = k / NB;
+            const int ii = k % BX, jj = k / BX;
             if (inside_exact(p, ev, s, c.ix0 + ib + ii, c.iy0 + jb + jj))
                 in |= 1u << k;
         }
@@ -547,20 +551,20 @@ __device__ __forceinline__ float slice_box(const TorParams &p, const Event &ev, 
         return (float)__builtin_popcount(in);
     float acc = 0.0f;
     if (KIND == 0) {
-        float v[NB * NB];
+        float v[NC];
 #pragma unroll
-        for (int k = 0; k < NB * NB; k++)
-            v[k] = ((in >> k) & 1u) ? src[idx(ib + k % NB, jb + k / NB)] : 0.0f;
+        for (int k = 0; k < NC; k++)
+            v[k] = ((in >> k) & 1u) ? src[idx(ib + k % BX, jb + k / BX)] : 0.0f;
 #pragma unroll
-        for (int k = 0; k < NB * NB; k++)
+        for (int k = 0; k < NC; k++)
             if ((in >> k) & 1u)
                 acc = fmaf(amp * __builtin_amdgcn_exp2f(d2[k] * ev.cw), v[k], acc);
     } else {
 #pragma unroll
-        for (int k = 0; k < NB * NB; k++) {
+        for (int k = 0; k < NC; k++) {
             if ((in >> k) & 1u) {
                 float w = amp * __builtin_amdgcn_exp2f(d2[k] * ev.cw);
-                D *q = dst + idx(ib + k % NB, jb + k / NB);
+                D *q = dst + idx(ib + k % BX, jb + k / BX);
                 if constexpr (LDS)
                     atomicAdd(q, (D)(w * scale));
                 else
@@ -571,12 +575,23 @@ __device__ __forceinline__ float slice_box(const TorParams &p, const Event &ev, 
     return acc;
 }
 
+// Box shape of an event for a compiled box size NB: 1 = NB x (NB - 1)
+// (enough for every event whose ellipse spans at most NB - 1 rows, e.g. all
+// C2 events), 2 = NB x NB, 0 = larger (chord walk).
+template <int NB>
+__device__ __forceinline__ int box_shape(const Track &t)
+{
+    if (NB <= 0 || t.nx > NB || t.ny > NB)
+        return 0;
+    return t.ny <= NB - 1 ? 1 : 2;
+}
+
 // The box walk of one slice over the strided image (patch voxels inside the
 // grid).
 template <int KIND, int NB>
 __device__ __forceinline__ float slice_box_image(const TorParams &p, const Event &ev,
                                                  const Track &t, const Cross &c, float amp, int s,
-                                                 float scale)
+                                                 float scale, int shape)
 {
     const int P = p.patch;
     if (c.ix0 <= -P || c.ix0 >= p.gx || c.iy0 <= -P || c.iy0 >= p.gy)
@@ -595,8 +610,11 @@ __device__ __forceinline__ float slice_box_image(const TorParams &p, const Event
         idx.si = p.ox;
         idx.sj = p.oy;
     }
-    return slice_box<KIND, NB, false>(p, ev, t, c, amp, s, scale, i_lo, i_hi, j_lo, j_hi, idx,
-                                      p.img, p.out);
+    if (shape == 1)
+        return slice_box<KIND, NB, NB - 1, false>(p, ev, t, c, amp, s, scale, i_lo, i_hi, j_lo,
+                                                  j_hi, idx, p.img, p.out);
+    return slice_box<KIND, NB, NB, false>(p, ev, t, c, amp, s, scale, i_lo, i_hi, j_lo, j_hi,
+                                          idx, p.img, p.out);
 }
 
 // Walk the crossed slices s_first, s_first + rstep, ... <= s_last of an event
@@ -608,14 +626,15 @@ __device__ __forceinline__ float walk_event(const TorParams &p, const Event &ev,
 {
     Track t;
     make_track(p, ev, t);
-    const bool boxed = NB > 0 && !p.full_patch && t.nx <= NB && t.ny <= NB;
+    constexpr int NBB = NB > 1 ? NB : 2;
+    const int shape = p.full_patch ? 0 : box_shape<NB>(t);
     float acc = 0.0f;
     for (int s = s_first; s <= s_last; s += rstep) {
         Cross c;
         cross_at(p, ev, t, s, c);
         float amp = tof ? tof_factor(p, ev, t, s) : 1.0f;
-        if (NB > 0 && boxed)
-            acc += slice_box_image<KIND, (NB > 0 ? NB : 1)>(p, ev, t, c, amp, s, scale);
+        if (NB > 0 && shape)
+            acc += slice_box_image<KIND, NBB>(p, ev, t, c, amp, s, scale, shape);
         else if (p.full_patch)
             acc += patch_walk<KIND, true>(p, ev, t, c, amp, s, scale);
         else
