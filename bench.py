@@ -55,6 +55,24 @@ def parse():
     return ap.parse_args()
 
 
+def _measured_traffic(workload):
+    """HBM bytes per launch of each kernel of ``workload`` from the committed
+    rocprofv3 counter summary (FETCH_SIZE and WRITE_SIZE scaled by a plain
+    copy; see profiles/), keyed by op name, or {} if there is none."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                        f"hbm_traffic_{workload}.json")
+    try:
+        with open(path) as f:
+            d = json.load(f)
+    except (OSError, ValueError):
+        return {}
+    out = {}
+    for name, k in d.get("kernels", {}).items():
+        out[name] = {"bytes_per_launch": int(k["bytes_per_launch"]),
+                     "source": f"profiles/hbm_traffic_{workload}.json ({k['kernel']})"}
+    return out
+
+
 def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
@@ -253,10 +271,18 @@ def main():
                              "algorithmic_bytes_per_launch": int(byts),
                              "achieved_GBps": round(ach, 1),
                              "frac": round(ach / HBM_PEAK_GBS, 4)}
+        traffic = _measured_traffic("tof_tor_c2")
+        for name in kernels:
+            t = traffic.get(name)
+            if t:
+                kernels[name]["traffic_bytes_per_launch"] = t["bytes_per_launch"]
         dom = max(kernels, key=lambda k: kernels[k]["mean_launch_ms"])
         k = kernels[dom]
+        tr = traffic.get(dom)
         roof = {"bound": "hbm", "achieved": k["achieved_GBps"], "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": k["frac"], "traffic": None, "kernel": k["kernel"],
+                "unit": "GB/s", "frac": k["frac"],
+                "traffic": tr["bytes_per_launch"] if tr else None,
+                "traffic_source": tr["source"] if tr else None, "kernel": k["kernel"],
                 "group": a, "mean_launch_ms": k["mean_launch_ms"],
                 "algorithmic_bytes_per_launch": k["algorithmic_bytes_per_launch"],
                 "bytes_formula": "sum_i (40 + 4 + 4*nnz_i) + 4*N_vox per op"}
