@@ -7,8 +7,8 @@
 //                  two work counters;
 //   batch_project  persistent projection: each block takes the next chunk of
 //                  <= 256 events (of any reconstruction and group) from a
-//                  counter and writes their projections into the plan's
-//                  buffer;
+//                  counter and writes their projections into the spare float
+//                  of the plan's event records;
 //   batch_scatter  persistent tiled backprojection of 1/p: each block takes
 //                  the next tile chunk (of any plan), accumulates it in an
 //                  FP64 LDS tile and flushes it into that reconstruction's
@@ -114,7 +114,8 @@ __global__ __launch_bounds__(256) void batch_project(const BatchJob *__restrict_
         const EvChunk ch = chunks[c];
         const BatchJob &job = jobs[__builtin_amdgcn_readfirstlane(ch.job)];
         TorParams p = job.p;
-        p.out = job.proj;
+        p.out = const_cast<float *>(reinterpret_cast<const float *>(job.rec)) + 10;
+        p.out_step = 12;
         const bool valid = (int)threadIdx.x < ch.count;
         const int64_t e = (int64_t)ch.start + threadIdx.x;
         switch (p.box) {
@@ -145,13 +146,13 @@ __global__ __launch_bounds__(256) void batch_scatter(const BatchJob *__restrict_
         const TileChunk ch{w.tile, w.start, w.count, 0};
         switch (job.p.box) {
         case 3:
-            scatter_chunk<kBack, 3>(job.p, job.g, ch, job.list, job.rec, ldsd);
+            scatter_chunk<kBack, 3, true>(job.p, job.g, ch, job.list, job.rec, ldsd);
             break;
         case 4:
-            scatter_chunk<kBack, 4>(job.p, job.g, ch, job.list, job.rec, ldsd);
+            scatter_chunk<kBack, 4, true>(job.p, job.g, ch, job.list, job.rec, ldsd);
             break;
         default:
-            scatter_chunk<kBack, 0>(job.p, job.g, ch, job.list, job.rec, ldsd);
+            scatter_chunk<kBack, 0, true>(job.p, job.g, ch, job.list, job.rec, ldsd);
             break;
         }
     }

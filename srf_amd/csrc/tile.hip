@@ -182,22 +182,22 @@ done:
     return rc;
 }
 
-template <int MODE>
+template <int MODE, bool VREC>
 void launch_scatter(const TorParams &p, const srfx_tor_plan_s *pl, dim3 grid, dim3 block,
                     unsigned sh, hipStream_t st)
 {
     switch (p.box) {
     case 3:
-        hipLaunchKernelGGL((tile_scatter<MODE, 3>), grid, block, sh, st, p, pl->g, pl->chunks,
-                           pl->list, pl->rec);
+        hipLaunchKernelGGL((tile_scatter<MODE, 3, VREC>), grid, block, sh, st, p, pl->g,
+                           pl->chunks, pl->list, pl->rec);
         break;
     case 4:
-        hipLaunchKernelGGL((tile_scatter<MODE, 4>), grid, block, sh, st, p, pl->g, pl->chunks,
-                           pl->list, pl->rec);
+        hipLaunchKernelGGL((tile_scatter<MODE, 4, VREC>), grid, block, sh, st, p, pl->g,
+                           pl->chunks, pl->list, pl->rec);
         break;
     default:
-        hipLaunchKernelGGL((tile_scatter<MODE, 0>), grid, block, sh, st, p, pl->g, pl->chunks,
-                           pl->list, pl->rec);
+        hipLaunchKernelGGL((tile_scatter<MODE, 0, VREC>), grid, block, sh, st, p, pl->g,
+                           pl->chunks, pl->list, pl->rec);
         break;
     }
 }
@@ -220,13 +220,16 @@ int plan_scatter(srfx_tor_plan_s *pl, int mode, const float *vals, float *image,
     const unsigned sh = lds_bytes_scatter(pl->g);
     switch (mode) {
     case kBack:
-        launch_scatter<kBack>(p, pl, grid, block, sh, st);
+        if (vals)
+            launch_scatter<kBack, false>(p, pl, grid, block, sh, st);
+        else
+            launch_scatter<kBack, true>(p, pl, grid, block, sh, st);
         break;
     case kAdj:
-        launch_scatter<kAdj>(p, pl, grid, block, sh, st);
+        launch_scatter<kAdj, false>(p, pl, grid, block, sh, st);
         break;
     default:
-        launch_scatter<kMap>(p, pl, grid, block, sh, st);
+        launch_scatter<kMap, false>(p, pl, grid, block, sh, st);
         break;
     }
     (void)flags;
@@ -393,13 +396,18 @@ extern "C" int srfx_tor_plan_osem_subset(srfx_tor_plan plan, const float *image,
     // tiled scatter of 1 / p over events with p > 1e-7.
     if (flags & SRFX_TILED_PROJECTION) {
         rc = plan_project(plan, image, plan->proj, st);
-    } else {
-        TorParams q = p;
-        q.img = image;
-        q.out = plan->proj;
-        rc = launch<kProj>(q, flags, st);
+        if (rc)
+            return rc;
+        return plan_scatter(plan, kBack, plan->proj, bp, &bp_grid, flags, st);
     }
+    // The projection goes straight into the spare float of each event
+    // record, which the scatter loads anyway (no separate value gather).
+    TorParams q = p;
+    q.img = image;
+    q.out = reinterpret_cast<float *>(plan->rec) + 10;
+    q.out_step = 12;
+    rc = launch<kProj>(q, flags, st);
     if (rc)
         return rc;
-    return plan_scatter(plan, kBack, plan->proj, bp, &bp_grid, flags, st);
+    return plan_scatter(plan, kBack, nullptr, bp, &bp_grid, flags, st);
 }

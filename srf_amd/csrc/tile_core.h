@@ -355,7 +355,9 @@ __device__ __forceinline__ float tile_event(const TorParams &p, const TileGeom &
 // Scatter of one chunk (modes kBack, kMap, kAdj) into p.out, through the
 // FP64 LDS tile ldsd (S * Wy * Wx doubles).  Every thread of the workgroup
 // calls it; it synchronises the workgroup before and after the flush.
-template <int MODE, int NB>
+// VREC: the values are the projections stored in the event records (the OSEM
+// step of a plan), not p.val.
+template <int MODE, int NB, bool VREC = false>
 __device__ __forceinline__ void scatter_chunk(const TorParams &p, const TileGeom &g,
                                               const TileChunk &ch,
                                               const int *__restrict__ list,
@@ -376,22 +378,22 @@ __device__ __forceinline__ void scatter_chunk(const TorParams &p, const TileGeom
         if (j >= ch.count)
             continue;
         const int64_t e = list[ch.start + j];
+        RawEvent r;
+        fetch_event_rec(rec, e, r);
+        const float val = VREC ? r.v : p.val[e];
         float scale;
         bool go = true;
         if (MODE == kBack) {
-            float v = p.val[e];
-            go = v > 1e-7;
-            scale = 1.0f / v;
+            go = val > 1e-7;
+            scale = 1.0f / val;
         } else if (MODE == kMap) {
-            scale = 1.0f / p.val[e];
+            scale = 1.0f / val;
         } else {
-            scale = p.val[e];
+            scale = val;
             go = scale != 0.0f;
         }
         if (go) {
-            RawEvent r;
             Event ev;
-            fetch_event_rec(rec, e, r);
             if (init_event<MODE>(p, r, ev))
                 tile_event<1, NB>(p, g, b, ev, tof, scale, ldsd);
         }
@@ -413,14 +415,14 @@ __device__ __forceinline__ void scatter_chunk(const TorParams &p, const TileGeom
 // One workgroup per chunk.  The LDS tile is FP64: LDS FP64 atomic adds run
 // at several lanes per cycle where FP32 ones are serialised lane by lane,
 // and the tile sums gain precision.
-template <int MODE, int NB>
+template <int MODE, int NB, bool VREC>
 __global__ __launch_bounds__(256) void tile_scatter(TorParams p, TileGeom g,
                                                     const TileChunk *__restrict__ chunks,
                                                     const int *__restrict__ list,
                                                     const float4 *__restrict__ rec)
 {
     extern __shared__ double ldsd[];
-    scatter_chunk<MODE, NB>(p, g, chunks[blockIdx.x], list, rec, ldsd);
+    scatter_chunk<MODE, NB, VREC>(p, g, chunks[blockIdx.x], list, rec, ldsd);
 }
 
 // Gather of one chunk: partial[slot] = the listing's projection over the tile.

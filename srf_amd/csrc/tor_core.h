@@ -41,6 +41,7 @@ struct TorParams {
     const float *__restrict__ val; // lors_value (bp, map) or g (adjoint)
     const float *__restrict__ img; // image read by proj / osem
     float *out;                    // proj[n] (proj) or image (bp, map, adj, osem)
+    int out_step;                  // proj: element step of out (1, or 12 into event records)
     int32_t *nnz;                  // count: per-event counts (nullable)
     unsigned long long *nnz_total; // count: running total (nullable)
     int gx, gy, gz;
@@ -140,6 +141,7 @@ __device__ __forceinline__ bool slice_range(const TorParams &p, Event &ev)
 
 struct RawEvent {
     float x1, y1, z1, x2, y2, z2, xc, yc, zc, f;
+    float v; // record only: the event's current projection (OSEM step)
 };
 
 namespace {
@@ -167,7 +169,8 @@ __device__ __forceinline__ void fetch_event(const TorParams &p, int64_t e, RawEv
 }
 
 // The same record from an array of 48-byte records
-// [x1 y1 z1 x2][y2 z2 xc yc][zc f - -] (three 16-byte loads).
+// [x1 y1 z1 x2][y2 z2 xc yc][zc f v -] (three 16-byte loads); v is written by
+// the projection of a plan's OSEM step.
 __device__ __forceinline__ void fetch_event_rec(const float4 *__restrict__ rec, int64_t e,
                                                 RawEvent &r)
 {
@@ -182,6 +185,7 @@ __device__ __forceinline__ void fetch_event_rec(const float4 *__restrict__ rec, 
     r.yc = b.w;
     r.zc = c.x;
     r.f = c.y;
+    r.v = c.z;
 }
 
 template <int MODE>
@@ -670,7 +674,7 @@ __device__ __forceinline__ int run_event(const TorParams &p, int64_t e, int sub,
         float part = any ? walk_event<0, NB>(p, ev, tof, 0.0f, ev.s_lo + sub, ev.s_hi, LPE) : 0.0f;
         float pv = group_sum<LPE>(part);
         if (valid && sub == 0)
-            p.out[e] = pv;
+            p.out[e * p.out_step] = pv;
     } else if (MODE == kBack) {
         float v = valid ? p.val[e] : 0.0f;
         if (any && v > 1e-7)
@@ -813,6 +817,7 @@ int setup(TorParams &p, const float *lors, int64_t n, int64_t ld, const srfx_gri
     if (!(kernel_width > 0.0f) || !(kernel_width < 1.0e6f))
         return SRFX_EINVAL;
     p = TorParams{};
+    p.out_step = 1;
     p.lors = lors;
     p.n = n;
     p.ld = ld;
