@@ -44,8 +44,9 @@ def parse():
     ap.add_argument("--workload", default="tof_tor_c2",
                     choices=["tof_tor_c2", "siddon_c1", "autograd_c3", "batched_c4",
                              "large_c5", "hbm_copy"])
-    ap.add_argument("--autotune", action="store_true",
-                    help="tune the launch schedule on a sample before timing")
+    ap.add_argument("--no-autotune", dest="autotune", action="store_false",
+                    help="skip tuning the projection schedule and the tile shape on a sample "
+                         "of the events before timing (default: tune)")
     ap.add_argument("--no-sort", dest="sort", action="store_false",
                     help="keep the generated event order inside each subset (default: "
                          "reorder events inside each subset for locality)")
@@ -128,7 +129,7 @@ def main():
     import torch
     from srf_amd import effmap, physics, synth, preprocess
     from srf_amd.data import Image, split_from_groups
-    from srf_amd.plan import DEFAULT_TILE
+    from srf_amd.plan import DEFAULT_CHUNK, DEFAULT_TILE
     from srf_amd.recon import OSEM, subset_view
     from srf_amd.scanner import CylindricalScanner
 
@@ -162,11 +163,15 @@ def main():
     if args.sort:
         groups, _ = preprocess.reorder_groups(groups, args.subsets)
     data = split_from_groups(groups)
-    tuned = None
+    tuned, tuned_tiles = None, None
     if args.autotune:
         t0 = time.time()
-        tuned = model.autotune(image, data, ops_=("osem_subset",))
-        log(f"autotuned osem_subset schedule {model.schedule['osem_subset']} in "
+        if args.untiled:
+            tuned = model.autotune(image, data, ops_=("osem_subset",))
+        else:
+            tuned = model.autotune(image, data, ops_=("projection",))
+            tuned_tiles = model.autotune_tiles(image, data)
+        log(f"autotuned schedules {model.schedule}, tile {model.tile}, chunk {model.chunk} in "
             f"{time.time() - t0:.1f}s")
     osem = OSEM(model, data, emap, nb_subsets=args.subsets, fused=not args.unfused,
                 tiled=not args.untiled)
@@ -199,14 +204,18 @@ def main():
 
     def timed_tiled(img, sub, out=None, plans=None):
         # the two kernels of srfx_tor_plan_osem_subset, launched one by one
+        # (projection into the plan's event records, then the tiled scatter)
         for a in "xyz":
             if len(sub[a]) == 0:
                 continue
             e0, e1, e2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
+            sch = model.schedule.get("projection")
             e0.record()
-            p = model.project_group(img, sub[a], a)
+            plans[a].osem_subset(img.data, out=out, accumulate=True, schedule=sch,
+                                 stage="project")
             e1.record()
-            plans[a].backprojection(p, out=out, accumulate=True)
+            plans[a].osem_subset(img.data, out=out, accumulate=True, schedule=sch,
+                                 stage="scatter")
             e2.record()
             timings.setdefault(a, []).append((e0, e1, e2))
         return Image(out, img.center, img.size)
@@ -262,9 +271,9 @@ def main():
         byts = float(np.mean([n * (LOR_BYTES + 4) + 4 * z + 4 * nvox for n, z in launches]))
         kernels = {}
         for name, sel, kname in (("projection", lambda t: t[0].elapsed_time(t[1]),
-                                  "tor_kernel<kProj> (chord walk, batched gather)"),
+                                  "tor_kernel<kProj> (candidate-box walk, values into the event records)"),
                                  ("backprojection", lambda t: t[1].elapsed_time(t[2]),
-                                  "tile_scatter<kBack> (LDS tile accumulation)")):
+                                  "tile_scatter<kBack> (FP64 LDS tile accumulation, coalesced flush)")):
             ms = float(np.mean([sel(t) for t in timings[a]]))
             ach = byts / (ms / 1e3) / 1e9
             kernels[name] = {"kernel": kname, "group": a, "mean_launch_ms": round(ms, 4),
@@ -327,15 +336,19 @@ def main():
                    "fused": not args.unfused, "sorted": bool(args.sort),
                    "tiled": osem.tiled,
                    "tile": list(model.tile or DEFAULT_TILE) if osem.tiled else None,
+                   "chunk_default": None if model.chunk else DEFAULT_CHUNK,
                    "plan_build_s": round(plan_s, 3) if plan_s is not None else None,
                    "listings_per_event": (round(listings / args.events, 2)
                                           if listings is not None else None),
                    "value_incl_plan_build": (round(args.events * args.steps /
                                                    (gpu_ms / 1e3 + plan_s) / 1e6, 2)
                                              if plan_s is not None else None),
-                   "schedule": model.schedule["osem_subset"],
-                   "autotune_ms": ({str(k): round(v, 3) for k, v in tuned["osem_subset"].items()}
-                                   if tuned else None)},
+                   "schedule": model.schedule["osem_subset" if args.untiled else "projection"],
+                   "chunk": model.chunk,
+                   "autotune_ms": ({op: {str(k): round(v, 3) for k, v in t.items()}
+                                    for op, t in tuned.items()} if tuned else None),
+                   "autotune_tiles_ms": ({str(k): round(v, 3) for k, v in tuned_tiles.items()}
+                                         if tuned_tiles else None)},
         "roofline": roof,
         "kernels": kernels,
         "cpu_baseline": cpu,

@@ -83,6 +83,12 @@ enum srfx_status {
 /* srfx_tor_plan_osem_subset: compute the projection half with the tiled
  * gather instead of the untiled kernel (selected by the schedule bits). */
 #define SRFX_TILED_PROJECTION (1u << 10)
+/* srfx_tor_plan_osem_subset in two calls (e.g. to time the two kernels):
+ * SRFX_OSEM_PROJECT_ONLY runs only the projection into the plan's event
+ * records, SRFX_OSEM_SCATTER_ONLY only the tiled backprojection of 1/p from
+ * them (zeroing bp unless SRFX_ACCUMULATE).  Not with SRFX_TILED_PROJECTION. */
+#define SRFX_OSEM_PROJECT_ONLY (1u << 11)
+#define SRFX_OSEM_SCATTER_ONLY (1u << 12)
 
 typedef struct srfx_grid {
     int32_t shape[3];  /* gx, gy, gz: voxels along the op's x, y, z */

@@ -23,6 +23,8 @@ _LPE_CODE = {1: 1, 4: 2, 16: 3}
 _BLOCK_CODE = {64: 1, 128: 2, 256: 3}
 SRFX_SLAB_SHIFT = 8
 SRFX_TILED_PROJECTION = 1 << 10
+SRFX_OSEM_PROJECT_ONLY = 1 << 11
+SRFX_OSEM_SCATTER_ONLY = 1 << 12
 _SLAB_CODE = {None: 0, "xcd8": 1, "blocked8": 2, "xcd16": 3}
 
 

@@ -1,5 +1,9 @@
 """Launch-schedule autotuner for the TOR kernels.
 
+Tile plans are tuned separately (SplitLoRsModel.autotune_tiles): tile shape
+(slices x wx x wy, TILES) and events per workgroup (CHUNKS), timed on the
+OSEM step of a sample of the caller's own events.
+
 A schedule is (lanes_per_event, block): how many lanes share one event (its
 crossed slices interleaved between them: 1, 4 or 16) and the LOR tile, i.e.
 threads per block (64, 128 or 256).  ``tune`` times every schedule of an op on
@@ -19,6 +23,10 @@ from . import ops
 LANES = (1, 4, 16)
 BLOCKS = (64, 128, 256)
 SCHEDULES = tuple((l, b) for l in LANES for b in BLOCKS)
+# Tile plans: (slices, wx, wy) and events per workgroup chunk.
+TILES = ((8, 32, 16), (8, 16, 32), (4, 32, 32), (8, 16, 16), (16, 16, 16), (4, 32, 16),
+         (8, 32, 32))
+CHUNKS = (1024, 2048, 4096)
 
 _cache = {}
 

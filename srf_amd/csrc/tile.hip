@@ -386,8 +386,12 @@ extern "C" int srfx_tor_plan_osem_subset(srfx_tor_plan plan, const float *image,
     const TorParams &p = plan->p;
     if (bp_grid.shape[0] != p.gx || bp_grid.shape[1] != p.gy || bp_grid.shape[2] != p.gz)
         return SRFX_EINVAL;
+    const bool proj_only = flags & SRFX_OSEM_PROJECT_ONLY;
+    const bool scat_only = flags & SRFX_OSEM_SCATTER_ONLY;
+    if ((proj_only && scat_only) || ((proj_only || scat_only) && (flags & SRFX_TILED_PROJECTION)))
+        return SRFX_EINVAL;
     hipStream_t st = (hipStream_t)stream;
-    int rc = zero_grid(bp, bp_grid, flags, st);
+    int rc = proj_only ? SRFX_OK : zero_grid(bp, bp_grid, flags, st);
     if (rc)
         return rc;
     if (p.n == 0)
@@ -402,12 +406,14 @@ extern "C" int srfx_tor_plan_osem_subset(srfx_tor_plan plan, const float *image,
     }
     // The projection goes straight into the spare float of each event
     // record, which the scatter loads anyway (no separate value gather).
-    TorParams q = p;
-    q.img = image;
-    q.out = reinterpret_cast<float *>(plan->rec) + 10;
-    q.out_step = 12;
-    rc = launch<kProj>(q, flags, st);
-    if (rc)
-        return rc;
+    if (!scat_only) {
+        TorParams q = p;
+        q.img = image;
+        q.out = reinterpret_cast<float *>(plan->rec) + 10;
+        q.out_step = 12;
+        rc = launch<kProj>(q, flags, st);
+        if (rc || proj_only)
+            return rc;
+    }
     return plan_scatter(plan, kBack, nullptr, bp, &bp_grid, flags, st);
 }

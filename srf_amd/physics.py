@@ -87,6 +87,33 @@ class SplitLoRsModel:
             report[op] = times
         return report
 
+    def autotune_tiles(self, image, data, sample=1 << 20, reps=3, tiles=None, chunks=None):
+        """Time the tiled OSEM step (projection + tiled backprojection) of the
+        first ``sample`` events of the largest group of ``data`` for every
+        tile shape x chunk size, on images laid out like ``image``, and keep
+        the fastest in self.tile / self.chunk.  Returns {(tile, chunk): ms}."""
+        from .autotune import CHUNKS, TILES, _time
+        from .plan import TorPlan
+        x = _world(image)
+        a = max(self.AXIS, key=lambda k: len(data[k]))
+        lm = data[a]
+        soa = lm.soa[:, :min(sample, len(lm))]
+        g, c, sz, st = group_frame(x.shape, image.center, image.size, x.stride(), a)
+        out = torch.zeros_like(x)
+        times = {}
+        for t in tiles or TILES:
+            for ch in chunks or CHUNKS:
+                plan = TorPlan(soa, g, c, sz, self.kernel_width, self.geo_sigma2_flag,
+                               self.tof_bin, self.tof_sigma2, strides=st, tile=t, chunk=ch,
+                               axis=a)
+                times[(tuple(t), int(ch))] = _time(
+                    lambda: plan.osem_subset(x, out=out, accumulate=True,
+                                             schedule=self.schedule.get("projection")), reps)
+                plan.close()
+        best = min(times, key=times.get)
+        self.tile, self.chunk = best
+        return times
+
     def _model(self):
         return _lib.make_tor_model(self.kernel_width, self.geo_sigma2_flag, self.tof_bin,
                                    self.tof_sigma2)
@@ -205,7 +232,7 @@ class SplitLoRsModel:
                 plan = self._planned(plans, a, n, out)
                 self._check_plan_layout(plan, x)
                 plan.osem_subset(x, out=out, accumulate=True,
-                                 schedule=self.schedule.get("osem_subset"))
+                                 schedule=self.schedule.get("projection"))
                 continue
             g_in = self._grid(image, a)
             g_out = self._grid(Image(out, image.center, image.size), a)

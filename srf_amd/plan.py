@@ -20,7 +20,7 @@ import torch
 from . import _lib
 from .ops import _check_dev, _geom, _ptr, _soa, _stream, _vec
 
-DEFAULT_TILE = (8, 32, 16)
+DEFAULT_TILE = (8, 16, 32)
 DEFAULT_CHUNK = 2048
 
 
@@ -92,15 +92,19 @@ class TorPlan:
         return self._scatter("srfx_tor_plan_maplors", lors_value, out, accumulate, None)
 
     def osem_subset(self, image, out=None, accumulate=False, tiled_projection=False,
-                    schedule=None, out_strides=None):
-        """Backprojection of 1/p (p > 1e-7) with p the projection of image."""
+                    schedule=None, out_strides=None, stage=None):
+        """Backprojection of 1/p (p > 1e-7) with p the projection of image.
+        stage "project" / "scatter" runs only that half (the projection is
+        kept in the plan's event records in between)."""
         image = self._image(image)
         out = self._out(out, image)
         og = self._cgrid if out_strides is None else _lib.make_grid(
             self.grid, self.center, self.size, out_strides)
+        st = {None: 0, "project": _lib.SRFX_OSEM_PROJECT_ONLY,
+              "scatter": _lib.SRFX_OSEM_SCATTER_ONLY}[stage]
         flags = ((_lib.SRFX_ACCUMULATE if accumulate else 0)
                  | (_lib.SRFX_TILED_PROJECTION if tiled_projection else 0)
-                 | _lib.schedule_flags(schedule))
+                 | _lib.schedule_flags(schedule) | st)
         _lib.call("srfx_tor_plan_osem_subset", self._h, _ptr(image), og, _ptr(out), flags,
                   _stream())
         return out

@@ -143,3 +143,22 @@ def test_plan_without_projection_tables(cuda, case):
     b1 = plan.backprojection(p)
     assert H.rel_l2(b1.cpu().numpy(), b0.cpu().numpy()) <= 1e-6
     plan.close()
+
+
+def test_plan_osem_stages(cuda, case):
+    """The OSEM step run as its two stages (projection into the plan's event
+    records, then the tiled scatter) equals the single call."""
+    import torch
+    lors, img = case
+    L, X = _t(lors, cuda), _t(img, cuda)
+    plan = _plan(L, False, True, (8, 16, 16), 512, tiled_projection=False)
+    full = plan.osem_subset(X)
+    out = torch.full_like(X, 0.25)
+    plan.osem_subset(X, out=out, accumulate=True, stage="project")
+    assert float((out - 0.25).abs().max()) == 0.0  # the projection stage writes no image
+    plan.osem_subset(X, out=out, accumulate=True, stage="scatter")
+    assert H.rel_l2((out - 0.25).cpu().numpy(), full.cpu().numpy()) <= 1e-6
+    from srf_amd._lib import SrfxError
+    with pytest.raises(SrfxError):
+        plan.osem_subset(X, stage="project", tiled_projection=True)
+    plan.close()
