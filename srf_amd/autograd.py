@@ -1,10 +1,11 @@
 """Differentiable TOR projector.
 
-``project(image, data, model)`` returns the projections of every event of
-``data`` (groups x, y, z concatenated) and is differentiable with respect to
-``image``.  The backward pass is the hand-written plain adjoint kernel
-(srfx_tor_backproject_adjoint, sum_i w_ij g_i): the system-matrix weights are
-recomputed from the LORs, no per-event voxel lists are stored.  Only the
+``project(image, data, model, plans=None)`` returns the projections of every
+event of ``data`` (groups x, y, z concatenated) and is differentiable with
+respect to ``image``.  The backward pass is the hand-written plain adjoint
+(sum_i w_ij g_i; srfx_tor_backproject_adjoint, or the tiled
+srfx_tor_plan_backproject_adjoint with ``plans``): the system-matrix weights
+are recomputed from the LORs, no per-event voxel lists are stored.  Only the
 image's shape and the LOR tensors (already resident) are kept for backward.
 TOF sigma and kernel width are attributes, not differentiated.
 
@@ -29,7 +30,7 @@ def _grid(model, image, axis):
 
 class _TorProject(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, x, model, center, size, soas):
+    def forward(ctx, x, model, center, size, soas, plans):
         img = Image(x.contiguous(), center, size)
         outs = []
         for a, soa in zip(("x", "y", "z"), soas):
@@ -40,6 +41,7 @@ class _TorProject(torch.autograd.Function):
                           _grid(model, img, a), model._model(), _ptr(p), 0, _stream())
             outs.append(p)
         ctx.model = model
+        ctx.plans = plans
         ctx.center, ctx.size = center, size
         ctx.soas = soas
         ctx.shape = x.shape
@@ -54,20 +56,27 @@ class _TorProject(torch.autograd.Function):
         off = 0
         for a, soa, n in zip(("x", "y", "z"), ctx.soas, ctx.counts):
             if n:
-                lors, n_, ld = _soa(soa, 10)
                 gi = g[off:off + n]
-                _lib.call("srfx_tor_backproject_adjoint", _ptr(lors), n_, ld, _ptr(gi),
-                          _grid(ctx.model, img, a), ctx.model._model(), _ptr(out),
-                          _lib.SRFX_ACCUMULATE, _stream())
+                if ctx.plans is not None:
+                    ctx.model._planned(ctx.plans, a, n, out).backprojection_adjoint(
+                        gi, out=out, accumulate=True)
+                else:
+                    lors, n_, ld = _soa(soa, 10)
+                    _lib.call("srfx_tor_backproject_adjoint", _ptr(lors), n_, ld, _ptr(gi),
+                              _grid(ctx.model, img, a), ctx.model._model(), _ptr(out),
+                              _lib.SRFX_ACCUMULATE, _stream())
             off += n
-        return out, None, None, None, None
+        return out, None, None, None, None, None
 
 
-def project(image, data, model):
+def project(image, data, model, plans=None):
     """Differentiable projection of Image ``image`` (data (nx, ny, nz),
-    requires_grad as needed) for ListModeDataSplit ``data``."""
+    requires_grad as needed) for ListModeDataSplit ``data``.  With ``plans``
+    (model.make_plans of the same events, for images laid out like ``image``)
+    the backward pass scatters through the tile plans."""
     soas = tuple(data[a].soa for a in ("x", "y", "z"))
-    return _TorProject.apply(image.data, model, list(image.center), list(image.size), soas)
+    return _TorProject.apply(image.data, model, list(image.center), list(image.size), soas,
+                             plans)
 
 
 def poisson_nll(p, x, sensitivity):

@@ -50,6 +50,7 @@ struct TorParams {
     double inter_x, inter_y, inter_z;
     double rinter_x, rinter_y, rinter_z;
     double l_bound, b_bound;
+    double dx_half, dy_half; // (half - 0.5) * inter_x, (half - 0.5) * inter_y
     double zbase; // center_z - (lz - inter_z)/2: slice s is at zbase + s*inter_z
     double tor_sigma2;
     float finter_x, finter_y;
@@ -303,12 +304,6 @@ struct Cross {
 
 namespace {
 
-__device__ __forceinline__ bool near_int(double u)
-{
-    double fr = u - floor(u);
-    return fr < 1.0e-9 || fr > 1.0 - 1.0e-9;
-}
-
 // The reference's crossing point of slice s (tor.cu.cc:30-36) in FP64 with
 // its operation order.
 __device__ __forceinline__ void cross_point_exact(const TorParams &p, const Event &ev, int s,
@@ -335,7 +330,13 @@ __device__ __forceinline__ void cross_at(const TorParams &p, const Event &ev, co
 {
     double u = t.ax + (double)s * t.bx;
     double w = t.ay + (double)s * t.by;
-    if (near_int(u) || near_int(w)) {
+    // trunc as the reference's (int) cast; the fraction u - trunc(u) lies in
+    // (-1, 1) and is within 1e-9 of 0 or +-1 exactly when u is within 1e-9 of
+    // an integer
+    const int iu = (int)u, iw = (int)w;
+    const double fu = u - (double)iu, fw = w - (double)iw;
+    const double au = fabs(fu), aw = fabs(fw);
+    if (au < 1.0e-9 || au > 1.0 - 1.0e-9 || aw < 1.0e-9 || aw > 1.0 - 1.0e-9) {
         double cx, cy;
         cross_point_exact(p, ev, s, cx, cy);
         c.ix0 = (int)__ddiv_rn(__dsub_rn(cx, p.l_bound), p.inter_x) - p.half;
@@ -343,10 +344,11 @@ __device__ __forceinline__ void cross_at(const TorParams &p, const Event &ev, co
         c.dx0 = (float)(cx - mesh_x(p, c.ix0));
         c.dy0 = (float)(cy - mesh_y(p, c.iy0));
     } else {
-        c.ix0 = (int)u - p.half;
-        c.iy0 = (int)w - p.half;
-        c.dx0 = (float)((u - ((double)c.ix0 + 0.5)) * p.inter_x);
-        c.dy0 = (float)((w - ((double)c.iy0 + 0.5)) * p.inter_y);
+        c.ix0 = iu - p.half;
+        c.iy0 = iw - p.half;
+        // (u - (ix0 + 0.5)) * inter = fu * inter + (half - 0.5) * inter
+        c.dx0 = (float)fma(fu, p.inter_x, p.dx_half);
+        c.dy0 = (float)fma(fw, p.inter_y, p.dy_half);
     }
 }
 
@@ -845,6 +847,8 @@ int setup(TorParams &p, const float *lors, int64_t n, int64_t ld, const srfx_gri
         return SRFX_EUNSUPPORTED;
     p.patch = (int)ps;
     p.half = (int)((unsigned)p.patch / 2);
+    p.dx_half = ((double)p.half - 0.5) * p.inter_x;
+    p.dy_half = ((double)p.half - 0.5) * p.inter_y;
     p.geo = geo ? 1 : 0;
     p.use_tof = tof_sigma2 < kTofThreshold ? 1 : 0;
     if (p.use_tof) {

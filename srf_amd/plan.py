@@ -53,6 +53,14 @@ class TorPlan:
         _lib.call("srfx_tor_plan_info", self._h, ctypes.byref(a), ctypes.byref(b))
         return int(a.value), int(b.value)
 
+    def device_bytes(self):
+        """Device memory held by the plan: 4 bytes per listing (its event),
+        52 per event (48-byte record, projection buffer), 16 per chunk, and 8
+        more per listing with the tiled-projection tables."""
+        nl, nc = self.info()
+        return 4 * nl + 52 * self.n + 16 * nc + (12 * nl + 4 * self.n if self.tiled_projection
+                                                 else 0)
+
     def _image(self, image, name="image"):
         _check_dev(image, name)
         g = self.grid
