@@ -267,7 +267,7 @@ def autograd_c3(args):
         memory={"recompute_peak_bytes": int(peak_ag), "stored_voxel_list_bytes": int(stored),
                 "tile_plan_bytes": int(plan_bytes), "tile_plan_build_s": round(plan_s, 3),
                 "note": "backward recomputes the weights, scattering through tile plans "
-                        "(4 bytes per (event, tile) listing + 52 per event); a stored-list "
+                        "(4 bytes per (event, tile) listing + 148 per event); a stored-list "
                         "backward keeps an int32 index and a float32 weight per non-zero "
                         "entry"})), flush=True)
 

@@ -7,8 +7,8 @@
 //                  two work counters;
 //   batch_project  persistent projection: each block takes the next chunk of
 //                  <= 256 events (of any reconstruction and group) from a
-//                  counter and writes their projections into the spare float
-//                  of the plan's event records;
+//                  counter and writes their projections into the value slot
+//                  of the plan's walk records;
 //   batch_scatter  persistent tiled backprojection of 1/p: each block takes
 //                  the next tile chunk (of any plan), accumulates it in an
 //                  FP64 LDS tile and flushes it into that reconstruction's
@@ -114,8 +114,8 @@ __global__ __launch_bounds__(256) void batch_project(const BatchJob *__restrict_
         const EvChunk ch = chunks[c];
         const BatchJob &job = jobs[__builtin_amdgcn_readfirstlane(ch.job)];
         TorParams p = job.p;
-        p.out = const_cast<float *>(reinterpret_cast<const float *>(job.rec)) + 10;
-        p.out_step = 12;
+        p.out = const_cast<float *>(reinterpret_cast<const float *>(job.rec)) + kWalkValue;
+        p.out_step = 4 * kWalkF4;
         const bool valid = (int)threadIdx.x < ch.count;
         const int64_t e = (int64_t)ch.start + threadIdx.x;
         switch (p.box) {

@@ -81,7 +81,8 @@ int plan_build(srfx_tor_plan_s *pl, hipStream_t st, int chunk, bool keep_slots)
     int last_cnt = 0, last_off = 0, mismatch = 0, total32 = 0;
     PlanArgs a{};
     // cnt_d: tile counts [ntiles], tile fill [ntiles], error count [1]
-    if ((rc = dev_alloc(&cnt_d, 2 * (size_t)ntiles + 1)) || (rc = dev_alloc(&pl->rec, 3 * n)) ||
+    if ((rc = dev_alloc(&cnt_d, 2 * (size_t)ntiles + 1)) ||
+        (rc = dev_alloc(&pl->rec, (size_t)kWalkF4 * n)) ||
         (rc = dev_alloc(&tile_off_d, ntiles)) || (rc = dev_alloc(&pl->proj, n)))
         goto done;
     if (keep_slots && ((rc = dev_alloc(&ev_cnt, n)) || (rc = dev_alloc(&pl->ev_off, n + 1))))
@@ -404,13 +405,13 @@ extern "C" int srfx_tor_plan_osem_subset(srfx_tor_plan plan, const float *image,
             return rc;
         return plan_scatter(plan, kBack, plan->proj, bp, &bp_grid, flags, st);
     }
-    // The projection goes straight into the spare float of each event
+    // The projection goes straight into the value slot of each event's walk
     // record, which the scatter loads anyway (no separate value gather).
     if (!scat_only) {
         TorParams q = p;
         q.img = image;
-        q.out = reinterpret_cast<float *>(plan->rec) + 10;
-        q.out_step = 12;
+        q.out = reinterpret_cast<float *>(plan->rec) + kWalkValue;
+        q.out_step = 4 * kWalkF4;
         rc = launch<kProj>(q, flags, st);
         if (rc || proj_only)
             return rc;

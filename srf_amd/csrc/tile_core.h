@@ -10,6 +10,8 @@
 // small footprint and it touches few tiles.  The plan lists, for every tile,
 // the events whose footprint meets it ("listings"), cut into chunks of at most
 // `chunk` listings; one workgroup takes one chunk:
+// Each event's walk record (tor_core.h) is computed once when the plan is
+// built; a listing unpacks it instead of redoing the event's set-up.
 //   scatter: zero the tile (FP64) in LDS, every lane walks one listing's
 //            voxels that lie in the tile and adds its weights with LDS
 //            atomics, then the tile is flushed with one coalesced FP32 global
@@ -89,7 +91,7 @@ struct PlanArgs {
     int cols7, ntiles;
     int *tile_cnt;        // count pass: listings per tile; fill pass: read
     int *ev_cnt;          // count pass: listings per event (nullable)
-    float4 *rec;          // count pass writes, fill pass reads: 48-byte event records
+    float4 *rec;          // count pass writes, fill pass reads: walk records
     const int *ev_off;    // fill pass: first listing of each event (nullable)
     const int *tile_off;  // fill pass: first slot of each tile
     int *tile_fill;       // fill pass: slots taken per tile
@@ -126,25 +128,25 @@ __global__ __launch_bounds__(256) void plan_pass(TorParams p, TileGeom g, PlanAr
             sh[i] = 0;
         __syncthreads();
     }
-    RawEvent r;
-    Event ev;
-    Track t;
+    Event ev{};
+    Track t{};
     bool ok = false;
     if (valid) {
         if (FILL) {
-            fetch_event_rec(a.rec, e, r);
+            // the walk record the count pass wrote
+            unpack_walk(p, a.rec + kWalkF4 * e, ev, t);
+            ok = ev.s_lo <= ev.s_hi;
         } else {
+            RawEvent r;
             if (a.cols7)
                 fetch_event<kMap>(p, e, r);
             else
                 fetch_event<kProj>(p, e, r);
-            a.rec[3 * e] = make_float4(r.x1, r.y1, r.z1, r.x2);
-            a.rec[3 * e + 1] = make_float4(r.y2, r.z2, r.xc, r.yc);
-            a.rec[3 * e + 2] = make_float4(r.zc, r.f, 0.0f, 0.0f);
+            ok = plan_init(p, a.cols7, r, ev);
+            if (ok)
+                make_track(p, ev, t);
+            pack_walk(r, ev, t, ok, a.rec + kWalkF4 * e);
         }
-        ok = plan_init(p, a.cols7, r, ev);
-        if (ok)
-            make_track(p, ev, t);
     }
     if (!FILL) {
         int k = 0;
@@ -314,11 +316,9 @@ __device__ __forceinline__ void window_slices(const Track &t, int half, int P, i
 // clipped chord walk.
 template <int KIND, int NB, typename T>
 __device__ __forceinline__ float tile_event(const TorParams &p, const TileGeom &g,
-                                            const TileBox &b, const Event &ev, bool tof,
-                                            float scale, T *lds)
+                                            const TileBox &b, const Event &ev, const Track &t,
+                                            bool tof, float scale, T *lds)
 {
-    Track t;
-    make_track(p, ev, t);
     constexpr int NBB = NB > 1 ? NB : 2;
     const int shape = box_shape<NB>(t);
     int s0 = max(ev.s_lo, b.sa), s1 = min(ev.s_hi, b.sb);
@@ -378,9 +378,10 @@ __device__ __forceinline__ void scatter_chunk(const TorParams &p, const TileGeom
         if (j >= ch.count)
             continue;
         const int64_t e = list[ch.start + j];
-        RawEvent r;
-        fetch_event_rec(rec, e, r);
-        const float val = VREC ? r.v : p.val[e];
+        Event ev;
+        Track t;
+        const float rv = unpack_walk(p, rec + kWalkF4 * e, ev, t);
+        const float val = VREC ? rv : p.val[e];
         float scale;
         bool go = true;
         if (MODE == kBack) {
@@ -392,11 +393,8 @@ __device__ __forceinline__ void scatter_chunk(const TorParams &p, const TileGeom
             scale = val;
             go = scale != 0.0f;
         }
-        if (go) {
-            Event ev;
-            if (init_event<MODE>(p, r, ev))
-                tile_event<1, NB>(p, g, b, ev, tof, scale, ldsd);
-        }
+        if (go)
+            tile_event<1, NB>(p, g, b, ev, t, tof, scale, ldsd);
     }
     __syncthreads();
     for (int i = threadIdx.x; i < nv; i += blockDim.x) {
@@ -449,12 +447,10 @@ __global__ __launch_bounds__(256) void tile_gather(TorParams p, TileGeom g,
     const bool tof = p.use_tof;
     for (int j = threadIdx.x; j < ch.count; j += blockDim.x) {
         const int64_t e = list[ch.start + j];
-        RawEvent r;
         Event ev;
-        fetch_event_rec(rec, e, r);
-        float acc = 0.0f;
-        if (init_event<kProj>(p, r, ev))
-            acc = tile_event<0, NB>(p, g, b, ev, tof, 0.0f, lds);
+        Track t;
+        unpack_walk(p, rec + kWalkF4 * e, ev, t);
+        const float acc = tile_event<0, NB>(p, g, b, ev, t, tof, 0.0f, lds);
         partial[ch.start + j] = acc;
     }
 }
@@ -487,7 +483,7 @@ struct srfx_tor_plan_s {
     int nchunks;
     int block;
     int *list;             // event of each slot, tiles in order
-    float4 *rec;           // 48-byte record of each event
+    float4 *rec;           // walk record of each event (kWalkF4 float4)
     srfx::TileChunk *chunks;
     float *proj;           // projection buffer of the OSEM step
     int *ev_off, *ev_slot; // tiled projection only: each event's slots

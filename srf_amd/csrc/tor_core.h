@@ -142,7 +142,6 @@ __device__ __forceinline__ bool slice_range(const TorParams &p, Event &ev)
 
 struct RawEvent {
     float x1, y1, z1, x2, y2, z2, xc, yc, zc, f;
-    float v; // record only: the event's current projection (OSEM step)
 };
 
 namespace {
@@ -167,26 +166,6 @@ __device__ __forceinline__ void fetch_event(const TorParams &p, int64_t e, RawEv
         r.zc = __builtin_nontemporal_load(L + 8 * ld + e);
         r.f = __builtin_nontemporal_load(L + 9 * ld + e);
     }
-}
-
-// The same record from an array of 48-byte records
-// [x1 y1 z1 x2][y2 z2 xc yc][zc f v -] (three 16-byte loads); v is written by
-// the projection of a plan's OSEM step.
-__device__ __forceinline__ void fetch_event_rec(const float4 *__restrict__ rec, int64_t e,
-                                                RawEvent &r)
-{
-    const float4 a = rec[3 * e], b = rec[3 * e + 1], c = rec[3 * e + 2];
-    r.x1 = a.x;
-    r.y1 = a.y;
-    r.z1 = a.z;
-    r.x2 = a.w;
-    r.y2 = b.x;
-    r.z2 = b.y;
-    r.xc = b.z;
-    r.yc = b.w;
-    r.zc = c.x;
-    r.f = c.y;
-    r.v = c.z;
 }
 
 template <int MODE>
@@ -301,6 +280,94 @@ struct Cross {
     int ix0, iy0;
     float dx0, dy0; // crossing point minus centre of the first patch voxel, mm
 };
+
+namespace {
+// Walk record of one event: everything a walk needs, produced once by
+// init_event and make_track (a plan's count pass) and unpacked by each of the
+// event's tile listings instead of recomputing it: 9 float4 (144 bytes).
+//   [ax bx] [ay by] (FP64 pairs) [v0x v0y v0z dvx] [dvy dvz A cxy]
+//   [q tm dymax hx] [ux uy thr cw] [ctof s_lo s_hi nx|ny<<16]
+//   [x1 y1 z1 x2] [y2 z2 f value]
+// value is the event's projection in a plan's OSEM step.  An event without a
+// crossed slice (or otherwise invalid) has s_lo = 1, s_hi = 0.
+constexpr int kWalkF4 = 9;
+constexpr int kWalkValue = 35; // float index of the value in a record
+
+__device__ __forceinline__ float4 dpair(double a, double b)
+{
+    return make_float4(__int_as_float(__double2loint(a)), __int_as_float(__double2hiint(a)),
+                       __int_as_float(__double2loint(b)), __int_as_float(__double2hiint(b)));
+}
+
+__device__ __forceinline__ void undpair(const float4 &f, double &a, double &b)
+{
+    a = __hiloint2double(__float_as_int(f.y), __float_as_int(f.x));
+    b = __hiloint2double(__float_as_int(f.w), __float_as_int(f.z));
+}
+
+__device__ __forceinline__ void pack_walk(const RawEvent &r, const Event &ev, const Track &t,
+                                          bool valid, float4 *w)
+{
+    w[0] = dpair(t.ax, t.bx);
+    w[1] = dpair(t.ay, t.by);
+    w[2] = make_float4(t.v0x, t.v0y, t.v0z, t.dvx);
+    w[3] = make_float4(t.dvy, t.dvz, t.A, t.cxy);
+    w[4] = make_float4(t.q, t.tm, t.dymax, t.hx);
+    w[5] = make_float4(ev.ux, ev.uy, ev.thr, ev.cw);
+    w[6] = make_float4(ev.ctof, __int_as_float(valid ? ev.s_lo : 1),
+                       __int_as_float(valid ? ev.s_hi : 0),
+                       __int_as_float((t.nx & 0xffff) | (t.ny << 16)));
+    w[7] = make_float4(r.x1, r.y1, r.z1, r.x2);
+    w[8] = make_float4(r.y2, r.z2, r.f, 0.0f);
+}
+
+// Inverse of pack_walk: the Event and Track fields the walks read, the FP64
+// differences recomputed exactly from the end points as init_event forms
+// them.  Returns the record's value.
+__device__ __forceinline__ float unpack_walk(const TorParams &p, const float4 *__restrict__ w,
+                                             Event &ev, Track &t)
+{
+    const float4 a = w[0], b = w[1], c = w[2], d = w[3], e = w[4], f = w[5], g = w[6],
+                 h = w[7], k = w[8];
+    undpair(a, t.ax, t.bx);
+    undpair(b, t.ay, t.by);
+    t.v0x = c.x;
+    t.v0y = c.y;
+    t.v0z = c.z;
+    t.dvx = c.w;
+    t.dvy = d.x;
+    t.dvz = d.y;
+    t.A = d.z;
+    t.cxy = d.w;
+    t.q = e.x;
+    t.tm = e.y;
+    t.dymax = e.z;
+    t.hx = e.w;
+    ev.ux = f.x;
+    ev.uy = f.y;
+    ev.thr = f.z;
+    ev.cw = f.w;
+    ev.ctof = g.x;
+    ev.s_lo = __float_as_int(g.y);
+    ev.s_hi = __float_as_int(g.z);
+    const int nxy = __float_as_int(g.w);
+    t.nx = nxy & 0xffff;
+    t.ny = nxy >> 16;
+    ev.x1 = h.x;
+    ev.y1 = h.y;
+    ev.z1 = h.z;
+    ev.d0 = (double)h.w - (double)h.x;
+    ev.d1 = (double)k.x - (double)h.y;
+    ev.d2 = (double)k.y - (double)h.z;
+    ev.dis2 = __dadd_rn(__dadd_rn(__dmul_rn(ev.d0, ev.d0), __dmul_rn(ev.d1, ev.d1)),
+                        __dmul_rn(ev.d2, ev.d2));
+    ev.rd2 = 0.0; // only make_track reads it
+    ev.sc_d = p.geo ? p.tor_sigma2 * (double)k.z : p.tor_sigma2;
+    ev.xc = ev.yc = ev.zc = 0.0f; // folded into the track
+    return k.w;
+}
+
+} // namespace
 
 namespace {
 

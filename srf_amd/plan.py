@@ -55,11 +55,12 @@ class TorPlan:
 
     def device_bytes(self):
         """Device memory held by the plan: 4 bytes per listing (its event),
-        52 per event (48-byte record, projection buffer), 16 per chunk, and 8
-        more per listing with the tiled-projection tables."""
+        148 per event (144-byte walk record, projection buffer), 16 per chunk,
+        and 12 more per listing and 4 per event with the tiled-projection
+        tables."""
         nl, nc = self.info()
-        return 4 * nl + 52 * self.n + 16 * nc + (12 * nl + 4 * self.n if self.tiled_projection
-                                                 else 0)
+        return 4 * nl + 148 * self.n + 16 * nc + (12 * nl + 4 * self.n if self.tiled_projection
+                                                  else 0)
 
     def _image(self, image, name="image"):
         _check_dev(image, name)

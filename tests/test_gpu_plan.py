@@ -151,8 +151,13 @@ def test_plan_osem_stages(cuda, case):
     import torch
     lors, img = case
     L, X = _t(lors, cuda), _t(img, cuda)
+    from srf_amd import ops
     plan = _plan(L, False, True, (8, 16, 16), 512, tiled_projection=False)
     full = plan.osem_subset(X)
+    ref = ops.osem_subset(L, X, H.TOR_GRID, H.TOR_CENTER, H.TOR_SIZE, H.KW, False, H.TOF_BIN,
+                          H.TOF_SIGMA2)
+    assert float(ref.abs().sum()) > 0
+    assert H.rel_l2(full.cpu().numpy(), ref.cpu().numpy()) <= 1e-6
     out = torch.full_like(X, 0.25)
     plan.osem_subset(X, out=out, accumulate=True, stage="project")
     assert float((out - 0.25).abs().max()) == 0.0  # the projection stage writes no image
