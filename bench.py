@@ -44,9 +44,11 @@ def parse():
     ap.add_argument("--workload", default="tof_tor_c2",
                     choices=["tof_tor_c2", "siddon_c1", "autograd_c3", "batched_c4",
                              "large_c5", "hbm_copy"])
-    ap.add_argument("--no-autotune", dest="autotune", action="store_false",
-                    help="skip tuning the projection schedule and the tile shape on a sample "
-                         "of the events before timing (default: tune)")
+    ap.add_argument("--autotune", action="store_true",
+                    help="tune the projection schedule and the tile shape on a sample of the "
+                         "events before timing (default: the fixed defaults, tile 8x16x32, "
+                         "chunk 2048; the tuner's candidates lie within a few percent of "
+                         "each other, inside its timing noise)")
     ap.add_argument("--no-sort", dest="sort", action="store_false",
                     help="keep the generated event order inside each subset (default: "
                          "reorder events inside each subset for locality)")
@@ -173,6 +175,10 @@ def main():
             tuned_tiles = model.autotune_tiles(image, data)
         log(f"autotuned schedules {model.schedule}, tile {model.tile}, chunk {model.chunk} in "
             f"{time.time() - t0:.1f}s")
+    elif not args.untiled:
+        # the tiled step's projection: 4 lanes per event, 256 threads (0.77 ms per
+        # 1M events at this workload against 0.92 ms at 1 lane per event)
+        model.schedule["projection"] = (4, 256)
     osem = OSEM(model, data, emap, nb_subsets=args.subsets, fused=not args.unfused,
                 tiled=not args.untiled)
     plan_s, listings = None, None

@@ -571,6 +571,7 @@ namespace {
 // arithmetic as patch_walk.
 //   KIND 0: returns sum w * src[idx]; 1: dst[idx] += w * scale (LDS: LDS
 //   atomics in dst's type, else FP32 global atomics); 2: returns the count.
+//   LDS: src / dst is an LDS tile, whose idx has unit column stride.
 template <int KIND, int BX, int BY, bool LDS, typename D>
 __device__ __forceinline__ float slice_box(const TorParams &p, const Event &ev, const Track &t,
                                            const Cross &c, float amp, int s, float scale, int i_lo,
@@ -623,11 +624,20 @@ __device__ __forceinline__ float slice_box(const TorParams &p, const Event &ev, 
     if (KIND == 2)
         return (float)__builtin_popcount(in);
     float acc = 0.0f;
+    // LDS tiles have unit column stride: one address per box row, formed
+    // here (opaque, so not re-formed per voxel inside the branches below),
+    // and the columns as immediate offsets
+    int row[BY];
+#pragma unroll
+    for (int jj = 0; jj < BY; jj++)
+        row[jj] = LDS ? opaque(idx(ib, jb + jj)) : 0;
     if (KIND == 0) {
         float v[NC];
 #pragma unroll
-        for (int k = 0; k < NC; k++)
-            v[k] = ((in >> k) & 1u) ? src[idx(ib + k % BX, jb + k / BX)] : 0.0f;
+        for (int k = 0; k < NC; k++) {
+            const int a = LDS ? row[k / BX] + k % BX : idx(ib + k % BX, jb + k / BX);
+            v[k] = ((in >> k) & 1u) ? src[a] : 0.0f;
+        }
 #pragma unroll
         for (int k = 0; k < NC; k++)
             if ((in >> k) & 1u)
@@ -637,7 +647,7 @@ __device__ __forceinline__ float slice_box(const TorParams &p, const Event &ev, 
         for (int k = 0; k < NC; k++) {
             if ((in >> k) & 1u) {
                 float w = amp * __builtin_amdgcn_exp2f(d2[k] * ev.cw);
-                D *q = dst + idx(ib + k % BX, jb + k / BX);
+                D *q = dst + (LDS ? row[k / BX] + k % BX : idx(ib + k % BX, jb + k / BX));
                 if constexpr (LDS)
                     atomicAdd(q, (D)(w * scale));
                 else
