@@ -115,7 +115,7 @@ __global__ __launch_bounds__(256) void batch_project(const BatchJob *__restrict_
         const BatchJob &job = jobs[__builtin_amdgcn_readfirstlane(ch.job)];
         TorParams p = job.p;
         p.out = const_cast<float *>(reinterpret_cast<const float *>(job.rec)) + kWalkValue;
-        p.out_step = 4 * kWalkF4;
+        p.out_step = 4 * kWalkHead;
         const bool valid = (int)threadIdx.x < ch.count;
         const int64_t e = (int64_t)ch.start + threadIdx.x;
         switch (p.box) {

@@ -411,7 +411,7 @@ extern "C" int srfx_tor_plan_osem_subset(srfx_tor_plan plan, const float *image,
         TorParams q = p;
         q.img = image;
         q.out = reinterpret_cast<float *>(plan->rec) + kWalkValue;
-        q.out_step = 4 * kWalkF4;
+        q.out_step = 4 * kWalkHead;
         rc = launch<kProj>(q, flags, st);
         if (rc || proj_only)
             return rc;

@@ -101,9 +101,9 @@ struct PlanArgs {
 };
 
 __device__ __forceinline__ bool plan_init(const TorParams &p, int cols7, const RawEvent &r,
-                                          Event &ev)
+                                          const ExactGeom &eg, Event &ev)
 {
-    return cols7 ? init_event<kMap>(p, r, ev) : init_event<kProj>(p, r, ev);
+    return cols7 ? init_event<kMap>(p, r, eg, ev) : init_event<kProj>(p, r, eg, ev);
 }
 
 // Count pass (FILL false): pack the event's record, count its tiles per
@@ -134,7 +134,7 @@ __global__ __launch_bounds__(256) void plan_pass(TorParams p, TileGeom g, PlanAr
     if (valid) {
         if (FILL) {
             // the walk record the count pass wrote
-            unpack_walk(p, a.rec + kWalkF4 * e, ev, t);
+            unpack_walk(p, a.rec, e, ev, t);
             ok = ev.s_lo <= ev.s_hi;
         } else {
             RawEvent r;
@@ -142,10 +142,11 @@ __global__ __launch_bounds__(256) void plan_pass(TorParams p, TileGeom g, PlanAr
                 fetch_event<kMap>(p, e, r);
             else
                 fetch_event<kProj>(p, e, r);
-            ok = plan_init(p, a.cols7, r, ev);
+            const ExactGeom eg = exact_geom(p, r);
+            ok = plan_init(p, a.cols7, r, eg, ev);
             if (ok)
-                make_track(p, ev, t);
-            pack_walk(r, ev, t, ok, a.rec + kWalkF4 * e);
+                make_track(p, ev, eg, r, t);
+            pack_walk(p, r, ev, t, ok, a.rec, e);
         }
     }
     if (!FILL) {
@@ -380,7 +381,7 @@ __device__ __forceinline__ void scatter_chunk(const TorParams &p, const TileGeom
         const int64_t e = list[ch.start + j];
         Event ev;
         Track t;
-        const float rv = unpack_walk(p, rec + kWalkF4 * e, ev, t);
+        const float rv = unpack_walk(p, rec, e, ev, t);
         const float val = VREC ? rv : p.val[e];
         float scale;
         bool go = true;
@@ -449,7 +450,7 @@ __global__ __launch_bounds__(256) void tile_gather(TorParams p, TileGeom g,
         const int64_t e = list[ch.start + j];
         Event ev;
         Track t;
-        unpack_walk(p, rec + kWalkF4 * e, ev, t);
+        unpack_walk(p, rec, e, ev, t);
         const float acc = tile_event<0, NB>(p, g, b, ev, t, tof, 0.0f, lds);
         partial[ch.start + j] = acc;
     }
@@ -483,7 +484,7 @@ struct srfx_tor_plan_s {
     int nchunks;
     int block;
     int *list;             // event of each slot, tiles in order
-    float4 *rec;           // walk record of each event (kWalkF4 float4)
+    float4 *rec;           // walk records (kWalkF4 float4 per event: heads, then tails)
     srfx::TileChunk *chunks;
     float *proj;           // projection buffer of the OSEM step
     int *ev_off, *ev_slot; // tiled projection only: each event's slots

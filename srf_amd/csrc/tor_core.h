@@ -65,17 +65,28 @@ struct TorParams {
 };
 
 
+// An event as the walks carry it.  The FP64 geometry that only the rare
+// exact decisions need is not kept in registers: e1 / e2 locate the event's
+// FP32 end points (x1, y1, z1 at e1[0], e1[ld], e1[2 ld]; x2, y2, z2 and the
+// sigma2 factor at e2[0], e2[ld], e2[2 ld], e2[fo ld]) and exact_geom forms
+// it again from them.
 struct Event {
-    double x1, y1, z1, d0, d1, d2;
-    double rd2;       // 1 / d2
-    double dis2;      // |d|^2
+    const float *e1, *e2;
+    int64_t ld;
+    int fo;
     float ux, uy;     // FP32 direction cosines (the exact ones are recomputed where needed)
-    float xc, yc, zc;
-    double sc_d;      // sigma_c^2 = tor_sigma2 (* sigma2 factor)
     float thr;        // FP32 image of 9 sigma_c^2
     float cw;         // per-voxel exponent factor (log2 domain)
     float ctof;       // per-slice TOF exponent factor (log2 domain)
     int s_lo, s_hi;   // strictly crossed slices
+};
+
+// The reference's FP64 event geometry (tor.cu.cc:22-48): end point 1, the
+// differences d = p2 - p1 of the FP32 end points, |d|^2 and sigma_c^2.
+struct ExactGeom {
+    double x1, y1, z1, d0, d1, d2;
+    double dis2; // |d|^2
+    double sc_d; // sigma_c^2 = tor_sigma2 (* sigma2 factor)
 };
 
 namespace {
@@ -85,9 +96,34 @@ __device__ __forceinline__ double slice_z(const TorParams &p, int s)
     return __dadd_rn(p.zbase, __dmul_rn((double)s, p.inter_z));
 }
 
+__device__ __forceinline__ ExactGeom exact_geom(const TorParams &p, float x1, float y1, float z1,
+                                                float x2, float y2, float z2, float f)
+{
+    ExactGeom g;
+    g.x1 = x1;
+    g.y1 = y1;
+    g.z1 = z1;
+    g.d0 = (double)x2 - (double)x1;
+    g.d1 = (double)y2 - (double)y1;
+    g.d2 = (double)z2 - (double)z1;
+    g.dis2 = __dadd_rn(__dadd_rn(__dmul_rn(g.d0, g.d0), __dmul_rn(g.d1, g.d1)),
+                       __dmul_rn(g.d2, g.d2));
+    g.sc_d = p.geo ? p.tor_sigma2 * (double)f : p.tor_sigma2;
+    return g;
+}
+
+// The geometry again from the event's end points (the rare exact paths).
+__device__ __forceinline__ ExactGeom exact_geom(const TorParams &p, const Event &ev)
+{
+    const int64_t ld = ev.ld;
+    return exact_geom(p, ev.e1[0], ev.e1[ld], ev.e1[2 * ld], ev.e2[0], ev.e2[ld], ev.e2[2 * ld],
+                      ev.e2[ev.fo * ld]);
+}
+
 // Slices s with 0 < (z_s - z1)/d2 < 1 (tor.cu.cc:29-33), i.e. z_s strictly
 // between z1 and z2.  Returns false when there are none.
-__device__ __forceinline__ bool slice_range(const TorParams &p, Event &ev)
+__device__ __forceinline__ bool slice_range(const TorParams &p, const ExactGeom &ev, int &s_lo,
+                                            int &s_hi)
 {
     double za, zb; // open interval (za, zb)
     if (ev.d2 > 0.0) {
@@ -131,9 +167,9 @@ __device__ __forceinline__ bool slice_range(const TorParams &p, Event &ev)
         while (b >= a && !(((slice_z(p, a) - ev.z1) / ev.d2) < 1.0))
             a++;
     }
-    ev.s_lo = a < 0 ? 0 : a;
-    ev.s_hi = b > p.gz - 1 ? p.gz - 1 : b;
-    return ev.s_lo <= ev.s_hi;
+    s_lo = a < 0 ? 0 : a;
+    s_hi = b > p.gz - 1 ? p.gz - 1 : b;
+    return s_lo <= s_hi;
 }
 
 // Raw event record: end points, TOF centre (zero for Maplors rows) and the
@@ -168,45 +204,21 @@ __device__ __forceinline__ void fetch_event(const TorParams &p, int64_t e, RawEv
     }
 }
 
+// Per-event set-up (tor.cu.cc:18-40 hoisted out of the slice loop) from the
+// event's FP32 row r and its FP64 geometry g; the caller sets ev.e1 / e2.
 template <int MODE>
-__device__ __forceinline__ bool init_event(const TorParams &p, const RawEvent &r, Event &ev);
-
-template <int MODE>
-__device__ __forceinline__ bool load_event(const TorParams &p, int64_t e, Event &ev)
+__device__ __forceinline__ bool init_event(const TorParams &p, const RawEvent &r,
+                                           const ExactGeom &g, Event &ev)
 {
-    RawEvent r;
-    fetch_event<MODE>(p, e, r);
-    return init_event<MODE>(p, r, ev);
-}
-
-// Per-event set-up (tor.cu.cc:18-40 hoisted out of the slice loop).
-template <int MODE>
-__device__ __forceinline__ bool init_event(const TorParams &p, const RawEvent &r, Event &ev)
-{
-    const float x1 = r.x1, y1 = r.y1, z1 = r.z1, x2 = r.x2, y2 = r.y2, z2 = r.z2;
-    const float f = r.f;
-    ev.xc = r.xc;
-    ev.yc = r.yc;
-    ev.zc = r.zc;
-    ev.x1 = x1;
-    ev.y1 = y1;
-    ev.z1 = z1;
-    ev.d0 = (double)x2 - (double)x1;
-    ev.d1 = (double)y2 - (double)y1;
-    ev.d2 = (double)z2 - (double)z1;
-    ev.dis2 = __dadd_rn(__dadd_rn(__dmul_rn(ev.d0, ev.d0), __dmul_rn(ev.d1, ev.d1)),
-                        __dmul_rn(ev.d2, ev.d2));
-    const double dis = __dsqrt_rn(ev.dis2);
+    const double dis = __dsqrt_rn(g.dis2);
     // FP32 direction cosines for the weights and the margin-guarded cut-off
     // test; the exact d0 / dis of the reference is recomputed by the FP64
     // decision (inside_exact).
     const double rdis = 1.0 / dis;
-    ev.ux = (float)(ev.d0 * rdis);
-    ev.uy = (float)(ev.d1 * rdis);
-    ev.rd2 = 1.0 / ev.d2;
-    ev.sc_d = p.geo ? p.tor_sigma2 * (double)f : p.tor_sigma2;
-    ev.thr = (float)(9.0 * ev.sc_d);
-    float inv_sc = (float)(1.0 / ev.sc_d);
+    ev.ux = (float)(g.d0 * rdis);
+    ev.uy = (float)(g.d1 * rdis);
+    ev.thr = (float)(9.0 * g.sc_d);
+    float inv_sc = (float)(1.0 / g.sc_d);
     if (MODE != kMap && MODE != kCount && p.use_tof) {
         ev.cw = kLog2e * 0.5f * (p.tse_inv - inv_sc);
         ev.ctof = -0.5f * kLog2e * p.tse_inv;
@@ -214,12 +226,19 @@ __device__ __forceinline__ bool init_event(const TorParams &p, const RawEvent &r
         ev.cw = -0.5f * kLog2e * inv_sc;
         ev.ctof = 0.0f;
     }
-    if (!(ev.sc_d > 0.0) || !(dis > 0.0) || !(dis < 1.0e30))
+    ev.s_lo = 1;
+    ev.s_hi = 0;
+    if (!(g.sc_d > 0.0) || !(dis > 0.0) || !(dis < 1.0e30))
         return false;
     // (d1, d2) = 0 means no crossed slice (d2 = 0); keeps rden finite
-    if (!(ev.d1 * ev.d1 + ev.d2 * ev.d2 > 0.0))
+    if (!(g.d1 * g.d1 + g.d2 * g.d2 > 0.0))
         return false;
-    return slice_range(p, ev);
+    return slice_range(p, g, ev.s_lo, ev.s_hi);
+}
+
+__device__ __forceinline__ ExactGeom exact_geom(const TorParams &p, const RawEvent &r)
+{
+    return exact_geom(p, r.x1, r.y1, r.z1, r.x2, r.y2, r.z2, r.f);
 }
 
 // Per-event geometry carried across slices.  The crossing point of slice s,
@@ -243,28 +262,32 @@ struct Track {
 
 namespace {
 
-__device__ __forceinline__ void make_track(const TorParams &p, const Event &ev, Track &t)
+// The track of an event with crossed slices, from its FP64 geometry g and
+// its row r (TOF centre).
+__device__ __forceinline__ void make_track(const TorParams &p, const Event &ev,
+                                           const ExactGeom &g, const RawEvent &r, Track &t)
 {
-    double r0 = (p.zbase - ev.z1) * ev.rd2;        // ratio at slice 0
-    double rs = p.inter_z * ev.rd2;                // ratio step per slice
-    t.ax = (ev.x1 + ev.d0 * r0 - p.l_bound) * p.rinter_x;
-    t.bx = ev.d0 * rs * p.rinter_x;
-    t.ay = (ev.y1 + ev.d1 * r0 - p.b_bound) * p.rinter_y;
-    t.by = ev.d1 * rs * p.rinter_y;
+    const double rd2 = 1.0 / g.d2;
+    double r0 = (p.zbase - g.z1) * rd2;        // ratio at slice 0
+    double rs = p.inter_z * rd2;               // ratio step per slice
+    t.ax = (g.x1 + g.d0 * r0 - p.l_bound) * p.rinter_x;
+    t.bx = g.d0 * rs * p.rinter_x;
+    t.ay = (g.y1 + g.d1 * r0 - p.b_bound) * p.rinter_y;
+    t.by = g.d1 * rs * p.rinter_y;
     double rl = r0 + (double)ev.s_lo * rs;
-    t.v0x = (float)((double)ev.xc - (ev.x1 + ev.d0 * rl));
-    t.v0y = (float)((double)ev.yc - (ev.y1 + ev.d1 * rl));
-    t.v0z = (float)((double)ev.zc - slice_z(p, ev.s_lo));
-    t.dvx = (float)(ev.d0 * rs);
-    t.dvy = (float)(ev.d1 * rs);
+    t.v0x = (float)((double)r.xc - (g.x1 + g.d0 * rl));
+    t.v0y = (float)((double)r.yc - (g.y1 + g.d1 * rl));
+    t.v0z = (float)((double)r.zc - slice_z(p, ev.s_lo));
+    t.dvx = (float)(g.d0 * rs);
+    t.dvy = (float)(g.d1 * rs);
     t.dvz = (float)p.inter_z;
     // a = 1 - ux^2 = (d1^2 + d2^2) / |d|^2, formed without cancellation
     // (d2 != 0 for an event with crossed slices)
-    const double den = ev.d1 * ev.d1 + ev.d2 * ev.d2;
+    const double den = g.d1 * g.d1 + g.d2 * g.d2;
     const double rden = 1.0 / den;
-    t.A = (float)(ev.dis2 * rden);
-    t.cxy = (float)(ev.d0 * ev.d1 * rden);
-    t.q = (float)(ev.d2 * ev.d2 * rden);
+    t.A = (float)(g.dis2 * rden);
+    t.cxy = (float)(g.d0 * g.d1 * rden);
+    t.q = (float)(g.d2 * g.d2 * rden);
     t.tm = ev.thr * (1.0f + 1.0e-5f) + 2.0f * kThrMargin;
     t.dymax = sqrtf(t.tm / t.q);
     t.hx = sqrtf(t.tm * (t.A + t.cxy * t.cxy / t.q));
@@ -272,6 +295,24 @@ __device__ __forceinline__ void make_track(const TorParams &p, const Event &ev, 
     // rounding allowance of the box start (1e-4) on both ends
     t.nx = (int)fminf(floorf(2.0f * t.hx * p.rfinter_x + 2.0e-4f), 1.0e6f) + 1;
     t.ny = (int)fminf(floorf(2.0f * t.dymax * p.rfinter_y + 2.0e-4f), 1.0e6f) + 1;
+}
+
+// Event and track of row e of the raw LOR rows (MODE kMap: 7 columns, else
+// 10).  Returns false for an event without crossed slices (t unset).
+template <int MODE>
+__device__ __forceinline__ bool load_event(const TorParams &p, int64_t e, Event &ev, Track &t)
+{
+    RawEvent r;
+    fetch_event<MODE>(p, e, r);
+    const ExactGeom g = exact_geom(p, r);
+    ev.e1 = p.lors + e;
+    ev.e2 = p.lors + 3 * p.ld + e;
+    ev.ld = p.ld;
+    ev.fo = MODE == kMap ? 3 : 6;
+    if (!init_event<MODE>(p, r, g, ev))
+        return false;
+    make_track(p, ev, g, r, t);
+    return true;
 }
 
 } // namespace
@@ -284,14 +325,17 @@ struct Cross {
 namespace {
 // Walk record of one event: everything a walk needs, produced once by
 // init_event and make_track (a plan's count pass) and unpacked by each of the
-// event's tile listings instead of recomputing it: 9 float4 (144 bytes).
-//   [ax bx] [ay by] (FP64 pairs) [v0x v0y v0z dvx] [dvy dvz A cxy]
-//   [q tm dymax hx] [ux uy thr cw] [ctof s_lo s_hi nx|ny<<16]
-//   [x1 y1 z1 x2] [y2 z2 f value]
+// event's tile listings instead of recomputing it.  A record array of n
+// events holds n heads of 8 float4 (128 bytes, one cache line: all that the
+// common path of a walk reads) and then n tails of 1 float4:
+//   head [ax bx] [ay by] (FP64 pairs) [v0x v0y v0z dvx] [dvy dvz dymax hx]
+//        [ux uy thr cw] [ctof s_lo s_hi nx|ny<<16] [A cxy q tm] [x1 y1 z1 value]
+//   tail [x2 y2 z2 f]
 // value is the event's projection in a plan's OSEM step.  An event without a
 // crossed slice (or otherwise invalid) has s_lo = 1, s_hi = 0.
-constexpr int kWalkF4 = 9;
-constexpr int kWalkValue = 35; // float index of the value in a record
+constexpr int kWalkF4 = 9;    // float4 per event
+constexpr int kWalkHead = 8;  // float4 per head
+constexpr int kWalkValue = 31; // float index of the value in a head
 
 __device__ __forceinline__ float4 dpair(double a, double b)
 {
@@ -305,30 +349,31 @@ __device__ __forceinline__ void undpair(const float4 &f, double &a, double &b)
     b = __hiloint2double(__float_as_int(f.w), __float_as_int(f.z));
 }
 
-__device__ __forceinline__ void pack_walk(const RawEvent &r, const Event &ev, const Track &t,
-                                          bool valid, float4 *w)
+__device__ __forceinline__ void pack_walk(const TorParams &p, const RawEvent &r, const Event &ev,
+                                          const Track &t, bool valid, float4 *rec, int64_t e)
 {
+    float4 *w = rec + kWalkHead * e;
     w[0] = dpair(t.ax, t.bx);
     w[1] = dpair(t.ay, t.by);
     w[2] = make_float4(t.v0x, t.v0y, t.v0z, t.dvx);
-    w[3] = make_float4(t.dvy, t.dvz, t.A, t.cxy);
-    w[4] = make_float4(t.q, t.tm, t.dymax, t.hx);
-    w[5] = make_float4(ev.ux, ev.uy, ev.thr, ev.cw);
-    w[6] = make_float4(ev.ctof, __int_as_float(valid ? ev.s_lo : 1),
+    w[3] = make_float4(t.dvy, t.dvz, t.dymax, t.hx);
+    w[4] = make_float4(ev.ux, ev.uy, ev.thr, ev.cw);
+    w[5] = make_float4(ev.ctof, __int_as_float(valid ? ev.s_lo : 1),
                        __int_as_float(valid ? ev.s_hi : 0),
                        __int_as_float((t.nx & 0xffff) | (t.ny << 16)));
-    w[7] = make_float4(r.x1, r.y1, r.z1, r.x2);
-    w[8] = make_float4(r.y2, r.z2, r.f, 0.0f);
+    w[6] = make_float4(t.A, t.cxy, t.q, t.tm);
+    w[7] = make_float4(r.x1, r.y1, r.z1, 0.0f);
+    rec[kWalkHead * p.n + e] = make_float4(r.x2, r.y2, r.z2, r.f);
 }
 
-// Inverse of pack_walk: the Event and Track fields the walks read, the FP64
-// differences recomputed exactly from the end points as init_event forms
-// them.  Returns the record's value.
-__device__ __forceinline__ float unpack_walk(const TorParams &p, const float4 *__restrict__ w,
-                                             Event &ev, Track &t)
+// Inverse of pack_walk for event e of a record array of p.n events: the Event
+// and Track fields the walks read; the end points stay in the record for the
+// rare exact decisions.  Returns the record's value.
+__device__ __forceinline__ float unpack_walk(const TorParams &p, const float4 *__restrict__ rec,
+                                             int64_t e, Event &ev, Track &t)
 {
-    const float4 a = w[0], b = w[1], c = w[2], d = w[3], e = w[4], f = w[5], g = w[6],
-                 h = w[7], k = w[8];
+    const float4 *w = rec + kWalkHead * e;
+    const float4 a = w[0], b = w[1], c = w[2], d = w[3], f = w[4], g = w[5], h = w[6], k = w[7];
     undpair(a, t.ax, t.bx);
     undpair(b, t.ay, t.by);
     t.v0x = c.x;
@@ -337,12 +382,8 @@ __device__ __forceinline__ float unpack_walk(const TorParams &p, const float4 *_
     t.dvx = c.w;
     t.dvy = d.x;
     t.dvz = d.y;
-    t.A = d.z;
-    t.cxy = d.w;
-    t.q = e.x;
-    t.tm = e.y;
-    t.dymax = e.z;
-    t.hx = e.w;
+    t.dymax = d.z;
+    t.hx = d.w;
     ev.ux = f.x;
     ev.uy = f.y;
     ev.thr = f.z;
@@ -353,17 +394,14 @@ __device__ __forceinline__ float unpack_walk(const TorParams &p, const float4 *_
     const int nxy = __float_as_int(g.w);
     t.nx = nxy & 0xffff;
     t.ny = nxy >> 16;
-    ev.x1 = h.x;
-    ev.y1 = h.y;
-    ev.z1 = h.z;
-    ev.d0 = (double)h.w - (double)h.x;
-    ev.d1 = (double)k.x - (double)h.y;
-    ev.d2 = (double)k.y - (double)h.z;
-    ev.dis2 = __dadd_rn(__dadd_rn(__dmul_rn(ev.d0, ev.d0), __dmul_rn(ev.d1, ev.d1)),
-                        __dmul_rn(ev.d2, ev.d2));
-    ev.rd2 = 0.0; // only make_track reads it
-    ev.sc_d = p.geo ? p.tor_sigma2 * (double)k.z : p.tor_sigma2;
-    ev.xc = ev.yc = ev.zc = 0.0f; // folded into the track
+    t.A = h.x;
+    t.cxy = h.y;
+    t.q = h.z;
+    t.tm = h.w;
+    ev.e1 = reinterpret_cast<const float *>(w + 7);
+    ev.e2 = reinterpret_cast<const float *>(rec + kWalkHead * p.n + e);
+    ev.ld = 1;
+    ev.fo = 3;
     return k.w;
 }
 
@@ -373,7 +411,7 @@ namespace {
 
 // The reference's crossing point of slice s (tor.cu.cc:30-36) in FP64 with
 // its operation order.
-__device__ __forceinline__ void cross_point_exact(const TorParams &p, const Event &ev, int s,
+__device__ __forceinline__ void cross_point_exact(const TorParams &p, const ExactGeom &ev, int s,
                                                   double &cx, double &cy)
 {
     double ratio = __ddiv_rn(__dsub_rn(slice_z(p, s), ev.z1), ev.d2);
@@ -405,7 +443,7 @@ __device__ __forceinline__ void cross_at(const TorParams &p, const Event &ev, co
     const double au = fabs(fu), aw = fabs(fw);
     if (au < 1.0e-9 || au > 1.0 - 1.0e-9 || aw < 1.0e-9 || aw > 1.0 - 1.0e-9) {
         double cx, cy;
-        cross_point_exact(p, ev, s, cx, cy);
+        cross_point_exact(p, exact_geom(p, ev), s, cx, cy);
         c.ix0 = (int)__ddiv_rn(__dsub_rn(cx, p.l_bound), p.inter_x) - p.half;
         c.iy0 = (int)__ddiv_rn(__dsub_rn(cy, p.b_bound), p.inter_y) - p.half;
         c.dx0 = (float)(cx - mesh_x(p, c.ix0));
@@ -433,14 +471,15 @@ __device__ __forceinline__ bool inside_exact(const TorParams &p, const Event &ev
                                              int iy)
 {
     s = opaque(s);
+    const ExactGeom g = exact_geom(p, ev);
     double cx, cy;
-    cross_point_exact(p, ev, s, cx, cy);
-    double dis = __dsqrt_rn(ev.dis2);
-    double ux = __ddiv_rn(ev.d0, dis), uy = __ddiv_rn(ev.d1, dis);
+    cross_point_exact(p, g, s, cx, cy);
+    double dis = __dsqrt_rn(g.dis2);
+    double ux = __ddiv_rn(g.d0, dis), uy = __ddiv_rn(g.d1, dis);
     double dx = __dsub_rn(cx, mesh_x(p, ix)), dy = __dsub_rn(cy, mesh_y(p, iy));
     double rc = __dadd_rn(__dmul_rn(dx, ux), __dmul_rn(dy, uy));
     double d2 = __dsub_rn(__dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy)), __dmul_rn(rc, rc));
-    return d2 < 9.0 * ev.sc_d;
+    return d2 < 9.0 * g.sc_d;
 }
 
 // Per-slice TOF factor tof_bin/sqrt(2 pi s2) * exp(-D/(2 tse)), D the squared
@@ -704,11 +743,10 @@ __device__ __forceinline__ float slice_box_image(const TorParams &p, const Event
 // (the caller clips to [s_lo, s_hi]).  NB > 0: slices go through the NB x NB
 // candidate box when the event's box fits, else through the chord walk.
 template <int KIND, int NB>
-__device__ __forceinline__ float walk_event(const TorParams &p, const Event &ev, bool tof,
-                                            float scale, int s_first, int s_last, int rstep)
+__device__ __forceinline__ float walk_event(const TorParams &p, const Event &ev, const Track &t,
+                                            bool tof, float scale, int s_first, int s_last,
+                                            int rstep)
 {
-    Track t;
-    make_track(p, ev, t);
     constexpr int NBB = NB > 1 ? NB : 2;
     const int shape = p.full_patch ? 0 : box_shape<NB>(t);
     float acc = 0.0f;
@@ -747,31 +785,32 @@ template <int MODE, int NB, int LPE>
 __device__ __forceinline__ int run_event(const TorParams &p, int64_t e, int sub, bool valid)
 {
     Event ev;
-    bool any = valid ? load_event<MODE>(p, e, ev) : false;
+    Track t;
+    bool any = valid ? load_event<MODE>(p, e, ev, t) : false;
     const bool tof = (MODE != kMap && MODE != kCount) && p.use_tof;
     if (MODE == kProj) {
-        float part = any ? walk_event<0, NB>(p, ev, tof, 0.0f, ev.s_lo + sub, ev.s_hi, LPE) : 0.0f;
+        float part = any ? walk_event<0, NB>(p, ev, t, tof, 0.0f, ev.s_lo + sub, ev.s_hi, LPE) : 0.0f;
         float pv = group_sum<LPE>(part);
         if (valid && sub == 0)
             p.out[e * p.out_step] = pv;
     } else if (MODE == kBack) {
         float v = valid ? p.val[e] : 0.0f;
         if (any && v > 1e-7)
-            walk_event<1, NB>(p, ev, tof, 1.0f / v, ev.s_lo + sub, ev.s_hi, LPE);
+            walk_event<1, NB>(p, ev, t, tof, 1.0f / v, ev.s_lo + sub, ev.s_hi, LPE);
     } else if (MODE == kMap) {
         if (any)
-            walk_event<1, NB>(p, ev, false, 1.0f / p.val[e], ev.s_lo + sub, ev.s_hi, LPE);
+            walk_event<1, NB>(p, ev, t, false, 1.0f / p.val[e], ev.s_lo + sub, ev.s_hi, LPE);
     } else if (MODE == kAdj) {
         float g = valid ? p.val[e] : 0.0f;
         if (any && g != 0.0f)
-            walk_event<1, NB>(p, ev, tof, g, ev.s_lo + sub, ev.s_hi, LPE);
+            walk_event<1, NB>(p, ev, t, tof, g, ev.s_lo + sub, ev.s_hi, LPE);
     } else if (MODE == kOsem) {
-        float part = any ? walk_event<0, NB>(p, ev, tof, 0.0f, ev.s_lo + sub, ev.s_hi, LPE) : 0.0f;
+        float part = any ? walk_event<0, NB>(p, ev, t, tof, 0.0f, ev.s_lo + sub, ev.s_hi, LPE) : 0.0f;
         float pv = group_sum<LPE>(part);
         if (any && pv > 1e-7)
-            walk_event<1, NB>(p, ev, tof, 1.0f / pv, ev.s_lo + sub, ev.s_hi, LPE);
+            walk_event<1, NB>(p, ev, t, tof, 1.0f / pv, ev.s_lo + sub, ev.s_hi, LPE);
     } else { // kCount
-        float part = any ? walk_event<2, NB>(p, ev, false, 0.0f, ev.s_lo + sub, ev.s_hi, LPE) : 0.0f;
+        float part = any ? walk_event<2, NB>(p, ev, t, false, 0.0f, ev.s_lo + sub, ev.s_hi, LPE) : 0.0f;
         int cnt = (int)group_sum<LPE>(part);
         if (valid && sub == 0) {
             if (p.nnz)
@@ -836,7 +875,8 @@ __global__ __launch_bounds__(256) void tor_slab_kernel(TorParams p, float *part,
     const int sa = (int)((int64_t)slab * p.gz / nslab);
     const int sb = (int)((int64_t)(slab + 1) * p.gz / nslab) - 1;
     Event ev;
-    bool any = valid ? load_event<MODE>(p, e, ev) : false;
+    Track t;
+    bool any = valid ? load_event<MODE>(p, e, ev, t) : false;
     int s0 = 0, s1 = -1;
     if (any) {
         s0 = max(ev.s_lo, sa);
@@ -845,7 +885,7 @@ __global__ __launch_bounds__(256) void tor_slab_kernel(TorParams p, float *part,
     }
     const bool tof = MODE != kMap && p.use_tof;
     if (MODE == kProj) {
-        float v = any ? walk_event<0, NB>(p, ev, tof, 0.0f, s0, s1, 1) : 0.0f;
+        float v = any ? walk_event<0, NB>(p, ev, t, tof, 0.0f, s0, s1, 1) : 0.0f;
         if (valid)
             part[(int64_t)slab * p.n + e] = v;
     } else {
@@ -870,7 +910,7 @@ __global__ __launch_bounds__(256) void tor_slab_kernel(TorParams p, float *part,
             scale = g;
         }
         if (go)
-            walk_event<1, NB>(p, ev, tof, scale, s0, s1, 1);
+            walk_event<1, NB>(p, ev, t, tof, scale, s0, s1, 1);
     }
 }
 
