@@ -291,6 +291,16 @@ int srfx_tor_batch_destroy(srfx_tor_batch *plan);
  * n voxels, evaluated as (x * emap) * bp). */
 int srfx_mlem_update(float *x, const float *emap, const float *bp, int64_t n, void *stream);
 
+/* PSF blur around the projector (PSFReconStep, model/recon_step.py:56-115).
+ * The image, nx x ny x nz contiguous with z fastest, is a matrix X of nrows =
+ * nx * ny rows by nz columns; out = S X M, S a sparse nrows x nrows matrix in
+ * CSR (row_ptr [nrows + 1], col / val [row_ptr[nrows]], 0 <= col < nrows),
+ * M dense nz x nz row-major.  The reference's forward blur (recon_step.py:
+ * 79-90) is S = psf_xy, M = psf_z^T; its backward blur (103-110) is
+ * S = psf_xy^T, M = psf_z.  img and out must not overlap; nz <= 2048. */
+int srfx_psf_apply(const int32_t *row_ptr, const int32_t *col, const float *val, int32_t nrows,
+                   const float *m, int32_t nz, const float *img, float *out, void *stream);
+
 /* Status text for a return code. */
 const char *srfx_strerror(int status);
 

@@ -45,3 +45,31 @@ def initial_image(inner_radius, axial_length, grid, center, size, crop_ratio=0.9
     """MasterLoader.create_image, loader.py:28-37."""
     image = np.ones(grid, dtype=np.float32)
     return crop_image(inner_radius, axial_length, image, grid, center, size, crop_ratio)
+
+
+def merge_effmap(maps, inner_radius, axial_length, grid, center, size, crop_ratio=0.95):
+    """merge_map.py:8-44 without the files: maps[ir] is the (nx, ny, nz) map
+    the reference saves as effmap_{ir}.npy and reads back transposed."""
+    temp = np.asarray(maps[0]).T
+    nb_image_layers = int(temp.shape[0])
+    final_map = np.zeros(temp.shape)
+    nb_rings = len(maps)
+    for ir in range(nb_rings):
+        temp = np.asarray(maps[ir]).T
+        for jr in range(nb_rings - ir):
+            final_map[jr:nb_image_layers, :, :] += temp[0:nb_image_layers - jr, :, :]
+    final_map = crop_image(inner_radius, axial_length, final_map.T, grid, center, size,
+                           crop_ratio)
+    final_map = final_map / np.max(final_map)
+    final_map[final_map <= 1e-6] = 0
+    final_map[final_map > 1e-6] = 1 / final_map[final_map > 1e-6]
+    return final_map
+
+
+def merge_effmap_full(summed, inner_radius, axial_length, grid, center, size, crop_ratio=0.95):
+    """merge_map.py:84-88 on the summed ordered-ring-pair maps."""
+    final_map = crop_image(inner_radius, axial_length, np.asarray(summed, dtype=np.float64),
+                           grid, center, size, crop_ratio)
+    final_map = final_map / np.max(final_map)
+    final_map[final_map > 1e-7] = 1 / final_map[final_map > 1e-7]
+    return final_map

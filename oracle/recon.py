@@ -8,7 +8,8 @@ group_frame / back_to_world   split_lors.py:54-70, 79-85, 112: the image is
 project_split / backproject_split  the SplitLoRsModel projection /
                               backprojection (sum of the groups).
 osem                          ReconStep + mlem_update (recon_step.py:33-49)
-                              over contiguous subsets (worker.py:79-85).
+                              over contiguous subsets (worker.py:79-85), or
+                              PSFReconStep (recon_step.py:56-115) with psf.
 """
 import numpy as np
 
@@ -58,14 +59,32 @@ def backproject_split(groups, values, x_shape, center, size, kw, geo, tof_bin, t
     return acc
 
 
+def map_split(groups7, x_shape, center, size, kw, geo, prec="f64"):
+    """Sum over groups of Maplors (values 1) of map rows [7, N_a] per group,
+    in the world (nx, ny, nz) frame."""
+    acc = np.zeros(x_shape, np.float64)
+    dummy = np.zeros(x_shape, np.float32)
+    for a in ("x", "y", "z"):
+        lors = groups7[a]
+        if lors.shape[1] == 0:
+            continue
+        _, g, c, s = group_frame(dummy, center, size, a)
+        m = O.tor_map(lors, np.ones(lors.shape[1], np.float32), g, c, s, kw, geo, prec=prec)
+        acc += back_to_world(m, a)
+    return acc
+
+
 def subset_bounds(n, k, K):
     return (k * n) // K, ((k + 1) * n) // K
 
 
 def osem(groups, x0, emap, center, size, kw, geo, tof_bin, tof_sigma2, nb_iterations,
-         nb_subsets, prec="f64"):
+         nb_subsets, prec="f64", psf=None):
     """x <- x * E * sum_groups A^T(1 / A x) per subset (the list-mode ratio of
-    the reference, with its 1e-7 threshold inside the backprojection)."""
+    the reference, with its 1e-7 threshold inside the backprojection).  With
+    psf = (psf_xy, psf_z): PSFReconStep (recon_step.py:64-115), the projector
+    sees PSF(x) and the backprojection is blurred by PSF^T before the update."""
+    from . import psf as OPSF
     x = np.array(x0, np.float64)
     for _ in range(nb_iterations):
         for k in range(nb_subsets):
@@ -73,10 +92,13 @@ def osem(groups, x0, emap, center, size, kw, geo, tof_bin, tof_sigma2, nb_iterat
             for a in ("x", "y", "z"):
                 lo, hi = subset_bounds(groups[a].shape[1], k, nb_subsets)
                 sub[a] = np.ascontiguousarray(groups[a][:, lo:hi])
-            p = project_split(sub, x.astype(np.float32), center, size, kw, geo, tof_bin,
+            seen = x if psf is None else OPSF.psf_forward(x, *psf)
+            p = project_split(sub, seen.astype(np.float32), center, size, kw, geo, tof_bin,
                               tof_sigma2, prec)
             v = {a: p[a].astype(np.float32) for a in p}
             bp = backproject_split(sub, v, x.shape, center, size, kw, geo, tof_bin, tof_sigma2,
                                    prec)
+            if psf is not None:
+                bp = OPSF.psf_backward(bp, *psf)
             x = x * emap * bp
     return x

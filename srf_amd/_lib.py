@@ -131,10 +131,11 @@ def load():
 
 # bound in srf_amd/batch.py (they take the batch structs)
 _BATCH = ["srfx_tor_batch_create", "srfx_tor_batch_subset", "srfx_tor_batch_destroy"]
+_PSF = ["srfx_psf_apply"]  # bound in psf.py
 
 
 def exported_symbols():
-    return list(_PROTOS) + ["srfx_strerror"] + _BATCH
+    return list(_PROTOS) + ["srfx_strerror"] + _BATCH + _PSF
 
 
 def call(name, *args):

@@ -8,9 +8,16 @@ finalize             merge_map.py:37-41: crop, divide by the max, invert where
                      > 1e-6 (the stored map is the inverse, used as x*E*bp).
 initial_image        MasterLoader.create_image, data/loader.py:28-37.
 
-The reference sums per-ring maps shifted along z (merge_map.py:18-25) and its
-map task walks ordered ring pairs (srfapp_new.py:153-164), so cross-ring pairs
-count twice; here each unordered pair is backprojected once.
+The reference's map task, reproduced as well:
+ring_maps            SRFApp._make_map_single_ring (srfapp_new.py:186-201): the
+                     map of ring 0 with each ring.
+merge_effmap         merge_map.py:8-44: those maps summed with the axial
+                     translation of every ring pair, then finalize.
+ordered_pair_map     the map task without translation (srfapp_new.py:153-164):
+                     every ordered ring pair, so cross-ring pairs count twice.
+merge_effmap_full    merge_map.py:68-88: crop, divide by the max, invert where
+                     > 1e-7 (values at or below stay as they are).
+compute_tor_effmap backprojects each unordered pair once instead.
 """
 import torch
 
@@ -83,3 +90,63 @@ def compute_tor_effmap(scanner, model, grid, center, size, device, chunk=8_000_0
         else:
             model.maplors(data, img, out=out)
     return out
+
+
+def _map_lors(model, lors, img, out):
+    groups = map_process(lors)
+    model.maplors(split_from_groups(groups, without_tof=True), img, out=out)
+
+
+def ring_maps(scanner, model, grid, center, size, device, ring1=0):
+    """[map of the LORs of ring ``ring1`` with ring ir, for ir in range(nb_rings)]
+    (srfapp_new.py:186-201, each saved there as effmap_{ir}.npy), (nx, ny, nz)."""
+    maps = []
+    for ir in range(scanner.nb_rings):
+        out = torch.zeros(tuple(grid), dtype=torch.float32, device=device)
+        lors = scanner.ring_pair_lors(ring1, ir, device)
+        if len(lors):
+            _map_lors(model, lors, Image(out, center, size), out)
+        maps.append(out)
+    return maps
+
+
+def merge_effmap(maps, grid, center, size, inner_radius, axial_length, crop_ratio=0.95):
+    """merge_map.py:8-44 on the device: map ir (ring 0 with ring ir) is added
+    at every axial shift jr = 0 .. nb_rings - ir - 1 (image layer jr onwards
+    takes layers 0 .. L - jr - 1), then cropped, normalised and inverted
+    (finalize).  Float64 sums, float32 result."""
+    nz = maps[0].shape[2]
+    nb_rings = len(maps)
+    if nb_rings - 1 > nz:
+        # the reference's slices final[jr:L] += temp[0:L - jr] no longer match
+        raise ValueError(f"{nb_rings} rings need at least {nb_rings - 1} image layers, got {nz}")
+    acc = torch.zeros(maps[0].shape, dtype=torch.float64, device=maps[0].device)
+    for ir, m in enumerate(maps):
+        m = m.to(torch.float64)
+        for jr in range(min(nb_rings - ir, nz)):
+            acc[:, :, jr:] += m[:, :, :nz - jr]
+    return finalize(acc, grid, center, size, inner_radius, axial_length, crop_ratio)
+
+
+def ordered_pair_map(scanner, model, grid, center, size, device):
+    """Sum of the maps of every ordered ring pair (srfapp_new.py:153-164;
+    merge_map.py:78-82 adds the saved maps), (nx, ny, nz) float32."""
+    out = torch.zeros(tuple(grid), dtype=torch.float32, device=device)
+    img = Image(out, center, size)
+    for ir1 in range(scanner.nb_rings):
+        for ir2 in range(scanner.nb_rings):
+            lors = scanner.ring_pair_lors(ir1, ir2, device)
+            if len(lors):
+                _map_lors(model, lors, img, out)
+    return out
+
+
+def merge_effmap_full(summed, grid, center, size, inner_radius, axial_length, crop_ratio=0.95):
+    """merge_map.py:84-88: crop, divide by the max, then 1 / value where
+    value > 1e-7; values at or below 1e-7 are kept (not zeroed), as there."""
+    m = crop_image(summed.to(torch.float64), grid, center, size, inner_radius, axial_length,
+                   crop_ratio)
+    m = m / torch.max(m)
+    big = m > 1e-7
+    out = torch.where(big, 1.0 / torch.where(big, m, torch.ones_like(m)), m)
+    return out.to(torch.float32)

@@ -11,6 +11,11 @@ Follows the reference's semantics:
                                     (local_reconstruction.py:44-51)
 E is the stored inverse efficiency map and is applied unscaled to every
 subset, as the reference does.
+
+PSFReconStep (recon_step.py:56-115): with a psf.PSF, the projector sees the
+blurred image PSF(x) and the backprojection is blurred back with PSF^T before
+the update, x <- x * E * PSF^T(sum_groups A^T(1 / A PSF(x))).  ReconStep and
+OSEM take it as ``psf=``.
 """
 import torch
 
@@ -47,27 +52,39 @@ class ReconStep:
     """One reconstruction step with the reference's structure
     (recon_step.py:17-45): projection, ratio backprojection, update."""
 
-    def __init__(self, model, fused=True):
+    def __init__(self, model, fused=True, psf=None):
         self.model = model
         self.fused = fused
+        self.psf = psf
         self._bp = None
+        self._blur = None
 
     def __call__(self, image, data, efficiency_map, plans=None):
         if self._bp is None or self._bp.shape != image.data.shape:
             self._bp = torch.empty_like(image.data)
+            self._blur = torch.empty_like(image.data) if self.psf is not None else None
         self._bp.zero_()
+        seen = image if self.psf is None else self.psf.forward(image, out=self._blur)
         if self.fused:
             if plans is None:
-                self.model.osem_subset(image, data, out=self._bp)
+                self.model.osem_subset(seen, data, out=self._bp)
             else:
-                self.model.osem_subset(image, data, out=self._bp, plans=plans)
+                self.model.osem_subset(seen, data, out=self._bp, plans=plans)
         else:
-            proj = self.model.projection(image, data)
+            proj = self.model.projection(seen, data)
             if plans is None:
-                self.model.backprojection(proj, image, out=self._bp)
+                self.model.backprojection(proj, seen, out=self._bp)
             else:
-                self.model.backprojection(proj, image, out=self._bp, plans=plans)
-        return mlem_update(image, Image(self._bp, image.center, image.size), efficiency_map)
+                self.model.backprojection(proj, seen, out=self._bp, plans=plans)
+        bp = Image(self._bp, image.center, image.size)
+        if self.psf is not None:
+            bp = self.psf.backward(bp, out=self._blur)  # the blurred image is no longer needed
+        return mlem_update(image, bp, efficiency_map)
+
+
+def PSFReconStep(model, psf, fused=True):
+    """recon_step.py:56-115: ReconStep with the PSF blur around the projector."""
+    return ReconStep(model, fused=fused, psf=psf)
 
 
 class OSEM:
@@ -77,8 +94,9 @@ class OSEM:
     subset are binned into image tiles once (``plan``, on the first subset
     run or explicitly) and every later step of that subset reuses the plan."""
 
-    def __init__(self, model, data, efficiency_map, nb_subsets=1, fused=True, tiled=False):
-        self.step = ReconStep(model, fused=fused)
+    def __init__(self, model, data, efficiency_map, nb_subsets=1, fused=True, tiled=False,
+                 psf=None):
+        self.step = ReconStep(model, fused=fused, psf=psf)
         self.model = model
         self.data = data
         self.emap = efficiency_map

@@ -102,6 +102,20 @@ class CylindricalScanner:
         table = torch.from_numpy(self.crystal_table()).to(device)
         return torch.cat([table[a], table[b]], dim=1)
 
+    def ring_pair_lors(self, ring1, ring2, device="cpu"):
+        """PETScanner.make_ring_pairs_lors (pet.py:151-177) for single-crystal
+        blocks: the crystals i1 of ring1 and i2 of ring2 with i1 < i2 when the
+        rings are the same, i1 != i2 otherwise, i1 outer and i2 inner, as
+        (N, 6) float32 end points [ring1 crystal, ring2 crystal] (torch, on
+        ``device``)."""
+        import torch
+        b = self.nb_crystals
+        i1 = torch.arange(b, device=device).repeat_interleave(b)
+        i2 = torch.arange(b, device=device).repeat(b)
+        keep = i1 < i2 if ring1 == ring2 else i1 != i2
+        table = torch.from_numpy(self.crystal_table()).to(device)
+        return torch.cat([table[ring1 * b + i1[keep]], table[ring2 * b + i2[keep]]], dim=1)
+
     def nb_pairs(self):
         nc = self.nb_rings * self.nb_crystals
         return nc * (nc - 1) // 2

@@ -74,6 +74,14 @@ def test_invalid_arguments_are_rejected():
                                     None) == _lib.SRFX_EINVAL
     # update: negative length
     assert lib.srfx_mlem_update(fake, fake, fake, -3, None) == _lib.SRFX_EINVAL
+    # PSF blur: negative rows, blur in place, nz beyond the LDS rows
+    from srf_amd import psf
+    psf._bind()
+    other = ctypes.c_void_p(0x2000)
+    assert lib.srfx_psf_apply(fake, fake, fake, -1, fake, 4, fake, other, None) == _lib.SRFX_EINVAL
+    assert lib.srfx_psf_apply(fake, fake, fake, 8, fake, 4, fake, fake, None) == _lib.SRFX_EINVAL
+    assert lib.srfx_psf_apply(fake, fake, fake, 8, fake, 2049, fake, other,
+                              None) == _lib.SRFX_EUNSUPPORTED
     assert lib.srfx_strerror(_lib.SRFX_EINVAL) == b"invalid argument"
 
 

@@ -96,3 +96,80 @@ def test_all_pairs_device_matches_host():
     for chunk in ((0, 2000), (m // 2, m // 2 + 3000), (m - 5000, m)):
         np.testing.assert_array_equal(big.all_pairs_device(chunk, torch.device("cpu")).numpy(),
                                       big.all_pairs(chunk))
+
+
+def _delta_maps(grid, nb_rings, seed):
+    """Ring maps that are single voxels: map ir holds value v_ir at (ix, iy, z0)."""
+    rng = np.random.default_rng(seed)
+    vals = rng.uniform(0.5, 2.0, nb_rings)
+    maps = []
+    for v in vals:
+        m = np.zeros(grid)
+        m[grid[0] // 2, grid[1] // 2, 1] = v
+        maps.append(m)
+    return maps, vals
+
+
+def test_merge_effmap_translation_closed_form():
+    """merge_map.py:18-25: layer z = 1 + jr collects v_ir for every ring ir with
+    jr < nb_rings - ir, so the summed map there is sum_{ir < nb_rings - jr} v_ir;
+    the oracle's numpy restatement and the device code agree with it."""
+    grid, center, size = (8, 8, 12), (0.0, 0.0, 0.0), (40.0, 40.0, 36.0)
+    nb_rings = 6
+    maps, vals = _delta_maps(grid, nb_rings, 3)
+    expect = np.zeros(grid)
+    for jr in range(nb_rings):
+        expect[4, 4, 1 + jr] = vals[:nb_rings - jr].sum()
+    # the unnormalised sum, through the oracle with crop and inversion undone
+    ref = OE.merge_effmap(maps, 390.0, 1000.0, grid, center, size)
+    got = effmap.merge_effmap([torch.from_numpy(m) for m in maps], grid, center, size, 390.0,
+                              1000.0).numpy()
+    fin = OE.finalize_effmap(expect, 390.0, 1000.0, grid, center, size)
+    np.testing.assert_allclose(ref, fin, rtol=1e-12)
+    np.testing.assert_allclose(got, fin.astype(np.float32), rtol=1e-7)
+    assert np.count_nonzero(got) == nb_rings
+
+
+def test_merge_effmap_matches_oracle_random():
+    grid, center, size = (10, 12, 9), (1.0, -2.0, 0.5), (780.0, 800.0, 30.6)
+    rng = np.random.default_rng(5)
+    maps = [rng.uniform(0, 3, grid) for _ in range(7)]
+    ref = OE.merge_effmap(maps, 390.0, 30.6, grid, center, size)
+    got = effmap.merge_effmap([torch.from_numpy(m) for m in maps], grid, center, size, 390.0,
+                              30.6)
+    np.testing.assert_allclose(got.numpy(), ref.astype(np.float32), rtol=1e-6)
+    with pytest.raises(ValueError):
+        effmap.merge_effmap([torch.from_numpy(m[:, :, :3]) for m in maps], grid, center, size,
+                            390.0, 30.6)
+
+
+def test_merge_effmap_full_keeps_small_values():
+    """merge_map.py:86-87: only values > 1e-7 of the max are inverted; the rest
+    stay as normalised (no zeroing, unlike merge_effmap)."""
+    grid, center, size = (4, 4, 2), (0.0, 0.0, 0.0), (40.0, 40.0, 4.0)
+    s = np.zeros(grid)
+    s[1, 1, 0] = 8.0
+    s[2, 2, 1] = 2.0
+    s[1, 2, 1] = 4e-7  # 5e-8 of the max: kept as 5e-8
+    got = effmap.merge_effmap_full(torch.from_numpy(s), grid, center, size, 390.0, 108.8)
+    ref = OE.merge_effmap_full(s, 390.0, 108.8, grid, center, size)
+    assert got[1, 1, 0] == 1.0 and got[2, 2, 1] == 4.0
+    assert got[1, 2, 1].item() == pytest.approx(5e-8, rel=1e-6)
+    np.testing.assert_allclose(got.numpy(), ref.astype(np.float32), rtol=1e-7)
+
+
+def test_ring_pair_lors_rule():
+    """pet.py:160-167 and block.py:140-152 on a small scanner: same ring
+    i1 < i2, different rings i1 != i2, i1 outer; B(B-1)/2 and B(B-1) LORs."""
+    sc = CylindricalScanner(nb_rings=3, nb_crystals=7)
+    tab = sc.crystal_table()
+    b = sc.nb_crystals
+    for r1 in range(3):
+        for r2 in range(3):
+            got = sc.ring_pair_lors(r1, r2).numpy()
+            pairs = [(i1, i2) for i1 in range(b) for i2 in range(b)
+                     if (i1 < i2 if r1 == r2 else i1 != i2)]
+            assert len(got) == (b * (b - 1) // 2 if r1 == r2 else b * (b - 1))
+            want = np.array([np.concatenate([tab[r1 * b + i1], tab[r2 * b + i2]])
+                             for i1, i2 in pairs], dtype=np.float32)
+            np.testing.assert_array_equal(got, want)
