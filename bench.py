@@ -43,7 +43,7 @@ def parse():
                     help="time separate projection + backprojection kernels")
     ap.add_argument("--workload", default="tof_tor_c2",
                     choices=["tof_tor_c2", "siddon_c1", "autograd_c3", "batched_c4",
-                             "large_c5", "hbm_copy"])
+                             "large_c5", "hbm_copy", "psf_c2", "io_c2"])
     ap.add_argument("--autotune", action="store_true",
                     help="tune the projection schedule and the tile shape on a sample of the "
                          "events before timing (default: the fixed defaults, tile 8x16x32, "

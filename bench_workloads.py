@@ -16,6 +16,12 @@ batched_c4   64 independent TOF OSEM reconstructions (image sizes mixed from
              reconstructions run one after another.
 large_c5     256x256x160 (1.6x1.6x1.7 mm) torso phantom, 300M events at
              400 ps, attenuated efficiency map, 2 iterations x 20 subsets.
+psf_c2       the PSF blur pair of one PSF-modelled OSEM step (forward blur
+             of the image, backward blur of the backprojection) on the
+             200x200x100 image, Gaussian kernels (4 mm transaxial, 4.4 mm
+             axial FWHM).
+io_c2        20M list-mode rows (float32 N x 7 .npy, 560 MB) streamed from a
+             file to device memory through the pinned double buffer.
 """
 import json
 import os
@@ -47,7 +53,8 @@ def _timer():
 
 def run(args):
     {"siddon_c1": siddon_c1, "autograd_c3": autograd_c3, "batched_c4": batched_c4,
-     "large_c5": large_c5, "hbm_copy": hbm_copy}[args.workload](args)
+     "large_c5": large_c5, "hbm_copy": hbm_copy, "psf_c2": psf_c2,
+     "io_c2": io_c2}[args.workload](args)
 
 
 # ------------------------------------------------------------ calibration ---
@@ -429,3 +436,78 @@ def large_c5(args):
          "plan_build_s": round(plan_s, 3), "listings_per_event": round(listings / nev, 2),
          "peak_gib": round(torch.cuda.max_memory_allocated() / 2**30, 1),
          "effmap": eff_stats})), flush=True)
+
+
+# -------------------------------------------------------------------- PSF ---
+
+def psf_c2(args):
+    """Forward + backward PSF blur at 200x200x100 (srfx_psf_apply)."""
+    import torch
+    from srf_amd.data import Image
+    from srf_amd.psf import gaussian_psf
+    dev = torch.device("cuda:0")
+    grid, size = (200, 200, 100), (400.0, 400.0, 220.0)
+    t0 = time.time()
+    psf = gaussian_psf(grid, size, 4.0, 4.4, dev)
+    build_s = time.time() - t0
+    x = Image(torch.rand(grid, device=dev) + 0.5, (0, 0, 0), size)
+    f = torch.empty_like(x.data)
+    b = torch.empty_like(x.data)
+    for _ in range(max(args.warmup, 1)):
+        psf.forward(x, out=f)
+        psf.backward(x, out=b)
+    torch.cuda.synchronize()
+    a, e = _timer()
+    a.record()
+    for _ in range(args.steps):
+        psf.forward(x, out=f)
+        psf.backward(Image(f, x.center, size), out=b)
+    e.record()
+    torch.cuda.synchronize()
+    ms = a.elapsed_time(e) / args.steps
+    nvox = 200 * 200 * 100
+    nnz = int(psf._fwd[2].numel())
+    # per blur: image read + written, CSR (8 bytes per entry + row pointers), dense z matrix
+    per_blur = 2 * 4 * nvox + 8 * nnz + 4 * (200 * 200 + 1) + 4 * 100 * 100
+    gbs = 2 * per_blur / (ms / 1e3) / 1e9
+    log(f"psf pair {ms * 1e3:.1f} us, {nnz / (200 * 200):.1f} transaxial entries per row")
+    print(json.dumps(_base("PSF blur pair (forward + backward) per OSEM step", ms * 1e3, "us",
+                           args.steps, args.warmup, ms,
+                           {"workload": "psf_c2", "image": list(grid), "voxel_mm": [2.0, 2.0, 2.2],
+                            "fwhm_mm": [4.0, 4.4], "xy_entries_per_row": nnz / (200 * 200),
+                            "build_s": round(build_s, 2)},
+                           higher_is_better=False,
+                           roofline={"bound": "hbm", "achieved": round(gbs, 1), "peak": 8000.0,
+                                     "unit": "GB/s", "frac": round(gbs / 8000.0, 4),
+                                     "traffic": None})), flush=True)
+
+
+# --------------------------------------------------------------------- IO ---
+
+def io_c2(args):
+    """load_lors of a 20M-row float32 .npy file into device memory."""
+    import tempfile
+    import torch
+    from srf_amd import io
+    n = 20_000_000
+    rows = np.random.default_rng(0).normal(0, 200, (n, 7)).astype(np.float32)
+    with tempfile.TemporaryDirectory(dir="/tmp") as d:
+        path = os.path.join(d, "lors.npy")
+        np.save(path, rows)
+        del rows
+        io.load_lors(path, "cuda:0", dtype=torch.float32)  # warm the page cache
+        torch.cuda.synchronize()
+        ts = []
+        for _ in range(max(args.steps, 1)):
+            t0 = time.perf_counter()
+            out = io.load_lors(path, "cuda:0", dtype=torch.float32)
+            torch.cuda.synchronize()
+            ts.append(time.perf_counter() - t0)
+            del out
+    s = float(np.median(ts))
+    gbs = 28 * n / s / 1e9
+    log(f"load_lors: {s * 1e3:.1f} ms for {n} rows, {gbs:.2f} GB/s")
+    print(json.dumps(_base("list-mode .npy rows to device GB/s", gbs, "GB/s", len(ts), 1,
+                           s * 1e3, {"workload": "io_c2", "rows": n, "bytes": 28 * n,
+                                     "source": "page-cached file, pinned double buffer"})),
+          flush=True)

@@ -295,11 +295,15 @@ int srfx_mlem_update(float *x, const float *emap, const float *bp, int64_t n, vo
  * The image, nx x ny x nz contiguous with z fastest, is a matrix X of nrows =
  * nx * ny rows by nz columns; out = S X M, S a sparse nrows x nrows matrix in
  * CSR (row_ptr [nrows + 1], col / val [row_ptr[nrows]], 0 <= col < nrows),
- * M dense nz x nz row-major.  The reference's forward blur (recon_step.py:
- * 79-90) is S = psf_xy, M = psf_z^T; its backward blur (103-110) is
- * S = psf_xy^T, M = psf_z.  img and out must not overlap; nz <= 2048. */
+ * M dense nz x nz row-major, zero outside the band z - below <= z' <= z +
+ * above of each column z (below = above = nz - 1 for any M; the entries
+ * outside the band are not read).  The reference's forward blur
+ * (recon_step.py:79-90) is S = psf_xy, M = psf_z^T; its backward blur
+ * (103-110) is S = psf_xy^T, M = psf_z.  img and out must not overlap;
+ * nz <= 2048. */
 int srfx_psf_apply(const int32_t *row_ptr, const int32_t *col, const float *val, int32_t nrows,
-                   const float *m, int32_t nz, const float *img, float *out, void *stream);
+                   const float *m, int32_t nz, int32_t below, int32_t above, const float *img,
+                   float *out, void *stream);
 
 /* Status text for a return code. */
 const char *srfx_strerror(int status);

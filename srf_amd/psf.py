@@ -31,10 +31,19 @@ def _bind():
     if not getattr(lib, "_psf_bound", False):
         lib.srfx_psf_apply.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                        ctypes.c_int32, ctypes.c_void_p, ctypes.c_int32,
-                                       ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+                                       ctypes.c_int32, ctypes.c_int32, ctypes.c_void_p,
+                                       ctypes.c_void_p, ctypes.c_void_p]
         lib.srfx_psf_apply.restype = ctypes.c_int
         lib._psf_bound = True
     return lib
+
+
+def _band(m):
+    """(below, above): m[z', z] == 0 unless z - below <= z' <= z + above."""
+    zp, z = np.nonzero(m)
+    if zp.size == 0:
+        return 0, 0
+    return int(max(0, np.max(z - zp))), int(max(0, np.max(zp - z)))
 
 
 def _csr(mat, n):
@@ -64,10 +73,12 @@ class PSF:
         xy = sparse.csr_matrix(psf_xy, dtype=np.float64)
         self._fwd = [torch.from_numpy(a).to(self.device) for a in _csr(xy, nxy)]
         self._bwd = [torch.from_numpy(a).to(self.device) for a in _csr(xy.T, nxy)]
-        self._mf = torch.from_numpy(np.ascontiguousarray(z.T, dtype=np.float32)).to(self.device)
-        self._mb = torch.from_numpy(np.ascontiguousarray(z, dtype=np.float32)).to(self.device)
+        mf, mb = np.ascontiguousarray(z.T, dtype=np.float32), np.ascontiguousarray(z, np.float32)
+        self._band_f, self._band_b = _band(mf), _band(mb)
+        self._mf = torch.from_numpy(mf).to(self.device)
+        self._mb = torch.from_numpy(mb).to(self.device)
 
-    def _apply(self, csr, m, x, out):
+    def _apply(self, csr, m, band, x, out):
         _check_dev(x, "image")
         if tuple(x.shape) != self.grid or x.dtype != torch.float32 or not x.is_contiguous():
             raise ValueError(f"image must be a contiguous float32 {self.grid} tensor")
@@ -76,18 +87,21 @@ class PSF:
         lib = _bind()
         row_ptr, col, val = csr
         st = lib.srfx_psf_apply(_ptr(row_ptr), _ptr(col), _ptr(val), self.grid[0] * self.grid[1],
-                                _ptr(m), self.grid[2], _ptr(x), _ptr(out), _stream())
+                                _ptr(m), self.grid[2], band[0], band[1], _ptr(x), _ptr(out),
+                                _stream())
         if st != _lib.SRFX_OK:
             raise _lib.SrfxError("srfx_psf_apply", st, lib.srfx_strerror(st).decode())
         return out
 
     def forward(self, image, out=None):
         """psf_xy (x psf_z^T), as an Image (recon_step.py:79-91)."""
-        return Image(self._apply(self._fwd, self._mf, image.data, out), image.center, image.size)
+        return Image(self._apply(self._fwd, self._mf, self._band_f, image.data, out),
+                     image.center, image.size)
 
     def backward(self, image, out=None):
         """psf_xy^T (b psf_z), as an Image (recon_step.py:103-113)."""
-        return Image(self._apply(self._bwd, self._mb, image.data, out), image.center, image.size)
+        return Image(self._apply(self._bwd, self._mb, self._band_b, image.data, out),
+                     image.center, image.size)
 
 
 def load_psf(xy_path, z_path, grid, device):

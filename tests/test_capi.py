@@ -78,9 +78,13 @@ def test_invalid_arguments_are_rejected():
     from srf_amd import psf
     psf._bind()
     other = ctypes.c_void_p(0x2000)
-    assert lib.srfx_psf_apply(fake, fake, fake, -1, fake, 4, fake, other, None) == _lib.SRFX_EINVAL
-    assert lib.srfx_psf_apply(fake, fake, fake, 8, fake, 4, fake, fake, None) == _lib.SRFX_EINVAL
-    assert lib.srfx_psf_apply(fake, fake, fake, 8, fake, 2049, fake, other,
+    assert lib.srfx_psf_apply(fake, fake, fake, -1, fake, 4, 3, 3, fake, other,
+                              None) == _lib.SRFX_EINVAL
+    assert lib.srfx_psf_apply(fake, fake, fake, 8, fake, 4, 3, 3, fake, fake,
+                              None) == _lib.SRFX_EINVAL
+    assert lib.srfx_psf_apply(fake, fake, fake, 8, fake, 4, -1, 3, fake, other,
+                              None) == _lib.SRFX_EINVAL
+    assert lib.srfx_psf_apply(fake, fake, fake, 8, fake, 2049, 0, 0, fake, other,
                               None) == _lib.SRFX_EUNSUPPORTED
     assert lib.srfx_strerror(_lib.SRFX_EINVAL) == b"invalid argument"
 

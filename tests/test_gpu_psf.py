@@ -10,7 +10,7 @@ from tests import helpers as H
 pytestmark = pytest.mark.gpu
 
 
-def _mats(grid, seed, nnz_row=9, empty_rows=0):
+def _mats(grid, seed, nnz_row=9, empty_rows=0, band=None):
     rng = np.random.default_rng(seed)
     nxy, nz = grid[0] * grid[1], grid[2]
     rows = np.repeat(np.arange(nxy), nnz_row)
@@ -19,16 +19,20 @@ def _mats(grid, seed, nnz_row=9, empty_rows=0):
     keep = rows >= empty_rows  # the first rows stay empty
     xy = sparse.csr_matrix((vals[keep], (rows[keep], cols[keep])), shape=(nxy, nxy))
     z = rng.uniform(0.0, 1.0, (nz, nz))
+    if band is not None:  # zero outside z' - z in [-band[0], band[1]]
+        d = np.arange(nz)[:, None] - np.arange(nz)[None, :]
+        z[(d < -band[0]) | (d > band[1])] = 0.0
     return xy, z
 
 
-@pytest.mark.parametrize("grid,empty", [((20, 24, 13), 0), ((7, 5, 1), 3), ((6, 9, 130), 5),
-                                        ((3, 3, 64), 0)])
-def test_psf_vs_oracle(cuda, grid, empty):
+@pytest.mark.parametrize("grid,empty,band", [((20, 24, 13), 0, None), ((7, 5, 1), 3, None),
+                                             ((6, 9, 130), 5, None), ((3, 3, 64), 0, None),
+                                             ((8, 6, 70), 0, (2, 5)), ((5, 5, 9), 2, (0, 3))])
+def test_psf_vs_oracle(cuda, grid, empty, band):
     from oracle import psf as OPSF
     from srf_amd.data import Image
     from srf_amd.psf import PSF
-    xy, z = _mats(grid, 1, empty_rows=empty)
+    xy, z = _mats(grid, 1, empty_rows=empty, band=band)
     x = np.random.default_rng(2).uniform(0.5, 2.0, grid).astype(np.float32)
     p = PSF(xy, z, grid, cuda)
     img = Image(torch.from_numpy(x).to(cuda), (0, 0, 0), (1, 1, 1))

@@ -71,3 +71,15 @@ def test_load_psf_files(tmp_path):
     np.testing.assert_allclose(p._mf.numpy(), z.T.astype(np.float32))
     with pytest.raises(ValueError):
         load_psf(tmp_path / "psf_xy.mat", tmp_path / "psf_z.npy", (3, 4, 6), "cpu")
+
+
+def test_band_of_z_matrix():
+    from srf_amd.psf import _band
+    m = np.zeros((6, 6))
+    m[2, 4] = 1.0  # z' = 2 feeds z = 4: below 2
+    m[5, 3] = 1.0  # z' = 5 feeds z = 3: above 2
+    m[1, 1] = 1.0
+    assert _band(m) == (2, 2)
+    assert _band(np.eye(5)) == (0, 0)
+    assert _band(np.ones((4, 4))) == (3, 3)
+    assert _band(np.triu(np.ones((4, 4)))) == (3, 0)
