@@ -277,7 +277,7 @@ def main():
         byts = float(np.mean([n * (LOR_BYTES + 4) + 4 * z + 4 * nvox for n, z in launches]))
         kernels = {}
         for name, sel, kname in (("projection", lambda t: t[0].elapsed_time(t[1]),
-                                  "tor_kernel<kProj> (candidate-box walk, values into the event records)"),
+                                  "record_proj_kernel (candidate-box walk from the walk records, values into their value slots)"),
                                  ("backprojection", lambda t: t[1].elapsed_time(t[2]),
                                   "tile_scatter<kBack> (FP64 LDS tile accumulation, coalesced flush)")):
             ms = float(np.mean([sel(t) for t in timings[a]]))

@@ -7,8 +7,8 @@
 //                  two work counters;
 //   batch_project  persistent projection: each block takes the next chunk of
 //                  <= 256 events (of any reconstruction and group) from a
-//                  counter and writes their projections into the value slot
-//                  of the plan's walk records;
+//                  counter, walks them from their plan's walk records and
+//                  writes their projections into the records' value slots;
 //   batch_scatter  persistent tiled backprojection of 1/p: each block takes
 //                  the next tile chunk (of any plan), accumulates it in an
 //                  FP64 LDS tile and flushes it into that reconstruction's
@@ -113,20 +113,18 @@ __global__ __launch_bounds__(256) void batch_project(const BatchJob *__restrict_
             break;
         const EvChunk ch = chunks[c];
         const BatchJob &job = jobs[__builtin_amdgcn_readfirstlane(ch.job)];
-        TorParams p = job.p;
-        p.out = const_cast<float *>(reinterpret_cast<const float *>(job.rec)) + kWalkValue;
-        p.out_step = 4 * kWalkHead;
+        float4 *rec = const_cast<float4 *>(job.rec);
         const bool valid = (int)threadIdx.x < ch.count;
         const int64_t e = (int64_t)ch.start + threadIdx.x;
-        switch (p.box) {
+        switch (job.p.box) {
         case 3:
-            run_event<kProj, 3, 1>(p, e, 0, valid);
+            record_project<3, 1>(job.p, rec, e, 0, valid);
             break;
         case 4:
-            run_event<kProj, 4, 1>(p, e, 0, valid);
+            record_project<4, 1>(job.p, rec, e, 0, valid);
             break;
         default:
-            run_event<kProj, 0, 1>(p, e, 0, valid);
+            record_project<0, 1>(job.p, rec, e, 0, valid);
             break;
         }
     }

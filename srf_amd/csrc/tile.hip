@@ -65,6 +65,45 @@ void launch_pass(const TorParams &p, const TileGeom &g, const PlanArgs &a, hipSt
 }
 
 // Count, scan (host: per tile; device: per event when slots are kept), fill.
+template <int NB, int LPE>
+int launch_record_pg(const TorParams &p, float4 *rec, int block, hipStream_t st)
+{
+    hipLaunchKernelGGL((record_proj_kernel<NB, LPE>), dim3(blocks_for(p.n * LPE, block)),
+                       dim3(block), 0, st, p, rec);
+    return launch_status();
+}
+
+template <int LPE>
+int launch_record_lpe(const TorParams &p, float4 *rec, int block, hipStream_t st)
+{
+    switch (p.full_patch ? 0 : p.box) {
+    case 3:
+        return launch_record_pg<3, LPE>(p, rec, block, st);
+    case 4:
+        return launch_record_pg<4, LPE>(p, rec, block, st);
+    default:
+        return launch_record_pg<0, LPE>(p, rec, block, st);
+    }
+}
+
+// The record projection with the schedule of the flags (lanes per event,
+// block size), as launch<kProj> takes it.
+int launch_record_project(const TorParams &p, float4 *rec, uint32_t flags, hipStream_t st)
+{
+    if (p.n == 0)
+        return SRFX_OK;
+    static const int kBlock[4] = {256, 64, 128, 256};
+    const int block = kBlock[(flags >> SRFX_BLOCK_SHIFT) & 3u];
+    switch ((flags >> SRFX_LPE_SHIFT) & 3u) {
+    case 2:
+        return launch_record_lpe<4>(p, rec, block, st);
+    case 3:
+        return launch_record_lpe<16>(p, rec, block, st);
+    default:
+        return launch_record_lpe<1>(p, rec, block, st);
+    }
+}
+
 int plan_build(srfx_tor_plan_s *pl, hipStream_t st, int chunk, bool keep_slots)
 {
     TorParams &p = pl->p;
@@ -410,9 +449,7 @@ extern "C" int srfx_tor_plan_osem_subset(srfx_tor_plan plan, const float *image,
     if (!scat_only) {
         TorParams q = p;
         q.img = image;
-        q.out = reinterpret_cast<float *>(plan->rec) + kWalkValue;
-        q.out_step = 4 * kWalkHead;
-        rc = launch<kProj>(q, flags, st);
+        rc = launch_record_project(q, plan->rec, flags, st);
         if (rc || proj_only)
             return rc;
     }

@@ -353,6 +353,37 @@ __device__ __forceinline__ float tile_event(const TorParams &p, const TileGeom &
     return acc;
 }
 
+// The projection of a plan's events from their walk records (the first
+// kernel of a plan's OSEM step): LPE lanes per event as tor_kernel<kProj>
+// runs them (the same walk, so the same bits), with the event set-up
+// unpacked from the record head instead of formed again from the LOR row by
+// each of the event's lanes; the value goes into the head's value slot.
+template <int NB, int LPE>
+__device__ __forceinline__ void record_project(const TorParams &p, float4 *rec, int64_t e,
+                                               int sub, bool valid)
+{
+    float part = 0.0f;
+    if (valid) {
+        Event ev;
+        Track t;
+        unpack_walk(p, rec, e, ev, t);
+        if (ev.s_lo <= ev.s_hi)
+            part = walk_event<0, NB>(p, ev, t, p.use_tof, 0.0f, ev.s_lo + sub, ev.s_hi, LPE);
+    }
+    const float pv = group_sum<LPE>(part);
+    if (valid && sub == 0)
+        reinterpret_cast<float *>(rec)[4 * kWalkHead * e + kWalkValue] = pv;
+}
+
+template <int NB, int LPE>
+__global__ __launch_bounds__(256) void record_proj_kernel(TorParams p, float4 *rec)
+{
+    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t e = gid / LPE;
+    record_project<NB, LPE>(p, rec, e, (int)(gid % LPE), e < p.n);
+}
+
+
 // Scatter of one chunk (modes kBack, kMap, kAdj) into p.out, through the
 // FP64 LDS tile ldsd (S * Wy * Wx doubles).  Every thread of the workgroup
 // calls it; it synchronises the workgroup before and after the flush.
