@@ -440,8 +440,9 @@ __device__ __forceinline__ void cross_at(const TorParams &p, const Event &ev, co
     // an integer
     const int iu = (int)u, iw = (int)w;
     const double fu = u - (double)iu, fw = w - (double)iw;
-    const double au = fabs(fu), aw = fabs(fw);
-    if (au < 1.0e-9 || au > 1.0 - 1.0e-9 || aw < 1.0e-9 || aw > 1.0 - 1.0e-9) {
+    // |fu - rint(fu)| is the distance of u from the nearest integer (fu lies
+    // in (-1, 1)); a non-short-circuit or, so one branch
+    if ((int)(fabs(fu - rint(fu)) < 1.0e-9) | (int)(fabs(fw - rint(fw)) < 1.0e-9)) {
         double cx, cy;
         cross_point_exact(p, exact_geom(p, ev), s, cx, cy);
         c.ix0 = (int)__ddiv_rn(__dsub_rn(cx, p.l_bound), p.inter_x) - p.half;
@@ -607,7 +608,8 @@ namespace {
 // all squared distances and the FP32 include decisions; the FP64 decision for
 // the rare voxels within the margin of the cut-off; the weights of the kept
 // voxels in the reference's order (rows outer, columns inner), same
-// arithmetic as patch_walk.
+// arithmetic as patch_walk.  The gather keeps the decisions as per-voxel lane
+// masks, the scatter and the count as one packed bit mask per lane.
 //   KIND 0: returns sum w * src[idx]; 1: dst[idx] += w * scale (LDS: LDS
 //   atomics in dst's type, else FP32 global atomics); 2: returns the count.
 //   LDS: src / dst is an LDS tile, whose idx has unit column stride.
@@ -636,52 +638,91 @@ __device__ __forceinline__ float slice_box(const TorParams &p, const Event &ev, 
         ry2[k] = dy * dy;
         ruy[k] = ev.uy * dy;
     }
-    float d2[NC];
-    uint32_t in = 0, near = 0;
-#pragma unroll
-    for (int jj = 0; jj < BY; jj++) {
-#pragma unroll
-        for (int ii = 0; ii < BX; ii++) {
-            const int k = jj * BX + ii;
-            const bool ok = ib + ii <= ie && jb + jj <= je;
-            d2[k] = tube_d2(cx2[ii], cux[ii], ry2[jj], ruy[jj]);
-            const bool a = ok && d2[k] < ev.thr - kThrMargin;
-            const bool b = ok && !a && d2[k] < ev.thr + kThrMargin;
-            in |= (uint32_t)a << k;
-            near |= (uint32_t)b << k;
-        }
-    }
-    if (near) {
-        // rare: the reference's FP64 decision, one voxel at a time
-        for (uint32_t m = (uint32_t)opaque((int)near); m; m &= m - 1) {
-            const int k = __builtin_ctz(m);
-            const int ii = k % BX, jj = k / BX;
-            if (inside_exact(p, ev, s, c.ix0 + ib + ii, c.iy0 + jb + jj))
-                in |= 1u << k;
-        }
-    }
-    if (KIND == 2)
-        return (float)__builtin_popcount(in);
-    float acc = 0.0f;
     // LDS tiles have unit column stride: one address per box row, formed
-    // here (opaque, so not re-formed per voxel inside the branches below),
-    // and the columns as immediate offsets
+    // after the decisions (opaque, so not re-formed per voxel inside the
+    // branches), and the columns as immediate offsets
     int row[BY];
+    float d2[NC];
+    if constexpr (KIND == 0) {
+        // Gather: the FP32 decisions stay per-candidate lane masks; only the
+        // rare margin case packs them (measured faster than a packed mask
+        // for the gather, slower for the scatter).
+        bool in[NC];
+        bool anynear = false;
 #pragma unroll
-    for (int jj = 0; jj < BY; jj++)
-        row[jj] = LDS ? opaque(idx(ib, jb + jj)) : 0;
-    if (KIND == 0) {
+        for (int jj = 0; jj < BY; jj++) {
+#pragma unroll
+            for (int ii = 0; ii < BX; ii++) {
+                const int k = jj * BX + ii;
+                const bool ok = (ib + ii <= ie) & (jb + jj <= je);
+                d2[k] = tube_d2(cx2[ii], cux[ii], ry2[jj], ruy[jj]);
+                in[k] = ok & (d2[k] < ev.thr - kThrMargin);
+                anynear |= ok & !in[k] & (d2[k] < ev.thr + kThrMargin);
+            }
+        }
+        if (anynear) {
+            // rare: the reference's FP64 decision, one voxel at a time
+            uint32_t near = 0, add = 0;
+#pragma unroll
+            for (int k = 0; k < NC; k++) {
+                const bool ok = (ib + k % BX <= ie) & (jb + k / BX <= je);
+                near |= (uint32_t)(ok & !in[k] & (d2[k] < ev.thr + kThrMargin)) << k;
+            }
+            for (uint32_t m = (uint32_t)opaque((int)near); m; m &= m - 1) {
+                const int k = __builtin_ctz(m);
+                if (inside_exact(p, ev, s, c.ix0 + ib + k % BX, c.iy0 + jb + k / BX))
+                    add |= 1u << k;
+            }
+#pragma unroll
+            for (int k = 0; k < NC; k++)
+                in[k] |= ((add >> k) & 1u) != 0;
+        }
+#pragma unroll
+        for (int jj = 0; jj < BY; jj++)
+            row[jj] = LDS ? opaque(idx(ib, jb + jj)) : 0;
+        // the kept voxels in the reference's order (rows outer, columns
+        // inner), all loads issued first
         float v[NC];
 #pragma unroll
         for (int k = 0; k < NC; k++) {
             const int a = LDS ? row[k / BX] + k % BX : idx(ib + k % BX, jb + k / BX);
-            v[k] = ((in >> k) & 1u) ? src[a] : 0.0f;
+            v[k] = in[k] ? src[a] : 0.0f;
         }
+        float acc = 0.0f;
 #pragma unroll
         for (int k = 0; k < NC; k++)
-            if ((in >> k) & 1u)
+            if (in[k])
                 acc = fmaf(amp * __builtin_amdgcn_exp2f(d2[k] * ev.cw), v[k], acc);
+        return acc;
     } else {
+        // Scatter and count: the decisions packed in a bit mask
+        uint32_t in = 0, near = 0;
+#pragma unroll
+        for (int jj = 0; jj < BY; jj++) {
+#pragma unroll
+            for (int ii = 0; ii < BX; ii++) {
+                const int k = jj * BX + ii;
+                const bool ok = ib + ii <= ie && jb + jj <= je;
+                d2[k] = tube_d2(cx2[ii], cux[ii], ry2[jj], ruy[jj]);
+                const bool a = ok && d2[k] < ev.thr - kThrMargin;
+                const bool b = ok && !a && d2[k] < ev.thr + kThrMargin;
+                in |= (uint32_t)a << k;
+                near |= (uint32_t)b << k;
+            }
+        }
+        if (near) {
+            // rare: the reference's FP64 decision, one voxel at a time
+            for (uint32_t m = (uint32_t)opaque((int)near); m; m &= m - 1) {
+                const int k = __builtin_ctz(m);
+                if (inside_exact(p, ev, s, c.ix0 + ib + k % BX, c.iy0 + jb + k / BX))
+                    in |= 1u << k;
+            }
+        }
+        if (KIND == 2)
+            return (float)__builtin_popcount(in);
+#pragma unroll
+        for (int jj = 0; jj < BY; jj++)
+            row[jj] = LDS ? opaque(idx(ib, jb + jj)) : 0;
 #pragma unroll
         for (int k = 0; k < NC; k++) {
             if ((in >> k) & 1u) {
@@ -693,8 +734,8 @@ __device__ __forceinline__ float slice_box(const TorParams &p, const Event &ev, 
                     unsafeAtomicAdd(q, w * scale);
             }
         }
+        return 0.0f;
     }
-    return acc;
 }
 
 // Box shape of an event for a compiled box size NB: 1 = NB x (NB - 1)
