@@ -287,7 +287,10 @@ __device__ __forceinline__ float window_walk(const TorParams &p, const TileGeom 
 // window columns x0..x1 and rows y0..y1.  The patch origin is trunc(u) - half
 // (or the exact origin, one voxel away) with u(s) = ax + s bx, so the patch
 // meets the columns only if x0 - P + half - 2 < u(s) < x1 + half + 2; the
-// bounds on s get one more slice of slack on each side.
+// bounds on s get one more slice of slack on each side.  The quotients are
+// formed in FP32 (approximate reciprocal): their error, a few 1e-7 of the
+// slice index, is far inside that slack, and slices outside the bounds
+// only cost work (the walk clips every voxel to the window).
 __device__ __forceinline__ void axis_slices(double a, double b, double lo, double hi, int &s0,
                                             int &s1)
 {
@@ -298,11 +301,11 @@ __device__ __forceinline__ void axis_slices(double a, double b, double lo, doubl
             s1 = s0 - 1;
         return;
     }
-    const double r = 1.0 / b;
-    const double t1 = (lo - a) * r, t2 = (hi - a) * r;
-    const double tl = fmax(fmin(t1, t2), -1.0e8), th = fmin(fmax(t1, t2), 1.0e8);
-    s0 = max(s0, (int)floor(tl) - 1);
-    s1 = min(s1, (int)ceil(th) + 1);
+    const float r = __builtin_amdgcn_rcpf((float)b);
+    const float t1 = (float)(lo - a) * r, t2 = (float)(hi - a) * r;
+    const float tl = fmaxf(fminf(t1, t2), -1.0e8f), th = fminf(fmaxf(t1, t2), 1.0e8f);
+    s0 = max(s0, (int)floorf(tl) - 1);
+    s1 = min(s1, (int)ceilf(th) + 1);
 }
 
 __device__ __forceinline__ void window_slices(const Track &t, int half, int P, int x0, int x1,

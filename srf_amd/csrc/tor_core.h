@@ -647,8 +647,10 @@ __device__ __forceinline__ float slice_box(const TorParams &p, const Event &ev, 
         // Gather: the FP32 decisions stay per-candidate lane masks; only the
         // rare margin case packs them (measured faster than a packed mask
         // for the gather, slower for the scatter).
+        // Kept: d2 - thr < -margin; within |d2 - thr| <= margin the FP64
+        // decision (d2 - thr is exact there, so the two cases cover every d2)
         bool in[NC];
-        bool anynear = false;
+        float gap = INFINITY; // the smallest |d2 - thr| over the box
 #pragma unroll
         for (int jj = 0; jj < BY; jj++) {
 #pragma unroll
@@ -656,17 +658,18 @@ __device__ __forceinline__ float slice_box(const TorParams &p, const Event &ev, 
                 const int k = jj * BX + ii;
                 const bool ok = (ib + ii <= ie) & (jb + jj <= je);
                 d2[k] = tube_d2(cx2[ii], cux[ii], ry2[jj], ruy[jj]);
-                in[k] = ok & (d2[k] < ev.thr - kThrMargin);
-                anynear |= ok & !in[k] & (d2[k] < ev.thr + kThrMargin);
+                const float dt = d2[k] - ev.thr; // exact near the cut-off
+                in[k] = ok & (dt < -kThrMargin);
+                gap = fminf(gap, fabsf(dt));
             }
         }
-        if (anynear) {
+        if (gap <= kThrMargin) {
             // rare: the reference's FP64 decision, one voxel at a time
             uint32_t near = 0, add = 0;
 #pragma unroll
             for (int k = 0; k < NC; k++) {
                 const bool ok = (ib + k % BX <= ie) & (jb + k / BX <= je);
-                near |= (uint32_t)(ok & !in[k] & (d2[k] < ev.thr + kThrMargin)) << k;
+                near |= (uint32_t)(ok & !in[k] & (fabsf(d2[k] - ev.thr) <= kThrMargin)) << k;
             }
             for (uint32_t m = (uint32_t)opaque((int)near); m; m &= m - 1) {
                 const int k = __builtin_ctz(m);
@@ -696,22 +699,36 @@ __device__ __forceinline__ float slice_box(const TorParams &p, const Event &ev, 
         return acc;
     } else {
         // Scatter and count: the decisions packed in a bit mask
-        uint32_t in = 0, near = 0;
+        // candidates inside the clip: columns < ie - ib + 1, rows < je - jb + 1
+        const int nc = ie - ib + 1, nr = je - jb + 1;
+        const uint32_t cols = nc <= 0 ? 0u : (nc >= BX ? (1u << BX) - 1u : (1u << nc) - 1u);
+        uint32_t okm = 0;
+#pragma unroll
+        for (int jj = 0; jj < BY; jj++)
+            okm |= jj < nr ? cols << (jj * BX) : 0u;
+        // kept and margin cases as for the gather
+        uint32_t in = 0;
+        float gap = INFINITY; // the smallest |d2 - thr| over the box
 #pragma unroll
         for (int jj = 0; jj < BY; jj++) {
 #pragma unroll
             for (int ii = 0; ii < BX; ii++) {
                 const int k = jj * BX + ii;
-                const bool ok = ib + ii <= ie && jb + jj <= je;
                 d2[k] = tube_d2(cx2[ii], cux[ii], ry2[jj], ruy[jj]);
-                const bool a = ok && d2[k] < ev.thr - kThrMargin;
-                const bool b = ok && !a && d2[k] < ev.thr + kThrMargin;
-                in |= (uint32_t)a << k;
-                near |= (uint32_t)b << k;
+                const float dt = d2[k] - ev.thr; // exact near the cut-off
+                in |= (uint32_t)(dt < -kThrMargin) << k;
+                gap = fminf(gap, fabsf(dt));
             }
         }
-        if (near) {
-            // rare: the reference's FP64 decision, one voxel at a time
+        in &= okm;
+        if (gap <= kThrMargin) {
+            // rare: the reference's FP64 decision, one voxel at a time, for
+            // the candidates within the margin of the cut-off
+            uint32_t near = 0;
+#pragma unroll
+            for (int k = 0; k < NC; k++)
+                near |= (uint32_t)(!((in >> k) & 1u) & (fabsf(d2[k] - ev.thr) <= kThrMargin)) << k;
+            near &= okm;
             for (uint32_t m = (uint32_t)opaque((int)near); m; m &= m - 1) {
                 const int k = __builtin_ctz(m);
                 if (inside_exact(p, ev, s, c.ix0 + ib + k % BX, c.iy0 + jb + k / BX))
