@@ -215,13 +215,12 @@ def main():
             if len(sub[a]) == 0:
                 continue
             e0, e1, e2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
-            sch = model.schedule.get("projection")
             e0.record()
-            plans[a].osem_subset(img.data, out=out, accumulate=True, schedule=sch,
-                                 stage="project")
+            # the projection half includes the copy of the image into the
+            # group's op frame
+            model.plan_osem_group(plans[a], img.data, out, a, stage="project")
             e1.record()
-            plans[a].osem_subset(img.data, out=out, accumulate=True, schedule=sch,
-                                 stage="scatter")
+            model.plan_osem_group(plans[a], img.data, out, a, stage="scatter")
             e2.record()
             timings.setdefault(a, []).append((e0, e1, e2))
         return Image(out, img.center, img.size)

@@ -89,6 +89,12 @@ enum srfx_status {
  * them (zeroing bp unless SRFX_ACCUMULATE).  Not with SRFX_TILED_PROJECTION. */
 #define SRFX_OSEM_PROJECT_ONLY (1u << 11)
 #define SRFX_OSEM_SCATTER_ONLY (1u << 12)
+/* srfx_tor_plan_osem_subset: the image argument is the op-frame copy of the
+ * image, contiguous [gz][gy][gx] (what the reference's transpose into the
+ * group frame produces, split_lors.py:79), whatever strides the plan was
+ * built with.  The box walk then reads each voxel row from one or two cache
+ * lines instead of one line per voxel column. */
+#define SRFX_IMAGE_OP_FRAME (1u << 13)
 
 typedef struct srfx_grid {
     int32_t shape[3];  /* gx, gy, gz: voxels along the op's x, y, z */

@@ -25,6 +25,7 @@ SRFX_SLAB_SHIFT = 8
 SRFX_TILED_PROJECTION = 1 << 10
 SRFX_OSEM_PROJECT_ONLY = 1 << 11
 SRFX_OSEM_SCATTER_ONLY = 1 << 12
+SRFX_IMAGE_OP_FRAME = 1 << 13
 _SLAB_CODE = {None: 0, "xcd8": 1, "blocked8": 2, "xcd16": 3}
 
 

@@ -430,6 +430,8 @@ extern "C" int srfx_tor_plan_osem_subset(srfx_tor_plan plan, const float *image,
     const bool scat_only = flags & SRFX_OSEM_SCATTER_ONLY;
     if ((proj_only && scat_only) || ((proj_only || scat_only) && (flags & SRFX_TILED_PROJECTION)))
         return SRFX_EINVAL;
+    if ((flags & SRFX_IMAGE_OP_FRAME) && (flags & SRFX_TILED_PROJECTION))
+        return SRFX_EUNSUPPORTED; // the tiled gather stages tiles with the plan's strides
     hipStream_t st = (hipStream_t)stream;
     int rc = proj_only ? SRFX_OK : zero_grid(bp, bp_grid, flags, st);
     if (rc)
@@ -449,6 +451,11 @@ extern "C" int srfx_tor_plan_osem_subset(srfx_tor_plan plan, const float *image,
     if (!scat_only) {
         TorParams q = p;
         q.img = image;
+        if (flags & SRFX_IMAGE_OP_FRAME) {
+            q.sx = 1;
+            q.sy = p.gx;
+            q.sz = p.gx * p.gy;
+        }
         rc = launch_record_project(q, plan->rec, flags, st);
         if (rc || proj_only)
             return rc;

@@ -51,6 +51,8 @@ class SplitLoRsModel:
         self.geo_sigma2_flag = bool(geo_sigma2_flag)
         self.tof_sigma2 = 1.0e4 if tof_sigma2 is None else float(tof_sigma2)
         self.tof_bin = 1.0e4 if tof_bin is None else float(tof_bin)
+        # tile plans' OSEM step: project from op-frame copies of the image
+        self.stage_frames = True
         # launch schedule per op: (lanes_per_event, block) or None (default)
         self.schedule = {"projection": None, "backprojection": None, "osem_subset": None}
         # tile shape (slices, wx, wy) and chunk of the tiled plans (make_plans)
@@ -229,10 +231,7 @@ class SplitLoRsModel:
             if n == 0:
                 continue
             if plans is not None:
-                plan = self._planned(plans, a, n, out)
-                self._check_plan_layout(plan, x)
-                plan.osem_subset(x, out=out, accumulate=True,
-                                 schedule=self.schedule.get("projection"))
+                self.plan_osem_group(self._planned(plans, a, n, out), x, out, a)
                 continue
             g_in = self._grid(image, a)
             g_out = self._grid(Image(out, image.center, image.size), a)
@@ -240,6 +239,29 @@ class SplitLoRsModel:
                       self._model(), _ptr(out), _lib.SRFX_ACCUMULATE | self._flags("osem_subset"),
                       _stream())
         return Image(out, image.center, image.size)
+
+    @staticmethod
+    def frame_copy(x, a):
+        """The op frame of group ``a`` of the world image x as a contiguous
+        [gz][gy][gx] tensor: the reference's transpose (split_lors.py:79)."""
+        return x.permute(*PERM[a]).contiguous()
+
+    def plan_osem_group(self, plan, x, out, a, stage=None):
+        """The OSEM step of one group's plan into the world image ``out``
+        (accumulating).  With stage_frames the projection reads the
+        op-frame copy of x (a box row of voxels then shares a cache line,
+        where the world layout puts each voxel column of the box on its own
+        line: 0.57 against 0.79 ms per 1M events at 200x200x100), else x in
+        place."""
+        self._check_plan_layout(plan, x)
+        if stage == "scatter":
+            src, op = x, False  # the projection is already in the plan
+        elif self.stage_frames:
+            src, op = self.frame_copy(x, a), True
+        else:
+            src, op = x, False
+        plan.osem_subset(src, out=out, accumulate=True, schedule=self.schedule.get("projection"),
+                         stage=stage, op_frame=op)
 
     def count_nnz(self, image, data):
         """Total non-zero (event, voxel) pairs over the groups (int64 tensor)."""

@@ -101,11 +101,21 @@ class TorPlan:
         return self._scatter("srfx_tor_plan_maplors", lors_value, out, accumulate, None)
 
     def osem_subset(self, image, out=None, accumulate=False, tiled_projection=False,
-                    schedule=None, out_strides=None, stage=None):
+                    schedule=None, out_strides=None, stage=None, op_frame=False):
         """Backprojection of 1/p (p > 1e-7) with p the projection of image.
         stage "project" / "scatter" runs only that half (the projection is
-        kept in the plan's event records in between)."""
-        image = self._image(image)
+        kept in the plan's event records in between).  op_frame: image is the
+        contiguous op-frame copy [gz][gy][gx] (any plan strides); out keeps
+        the plan's strides (or out_strides)."""
+        if op_frame:
+            _check_dev(image, "image")
+            g = self.grid
+            if tuple(image.shape) != (g[2], g[1], g[0]) or not image.is_contiguous():
+                raise ValueError(f"op-frame image must be a contiguous {g[::-1]} tensor")
+            if out is None:
+                raise ValueError("op_frame needs an explicit output image")
+        else:
+            image = self._image(image)
         out = self._out(out, image)
         og = self._cgrid if out_strides is None else _lib.make_grid(
             self.grid, self.center, self.size, out_strides)
@@ -113,7 +123,8 @@ class TorPlan:
               "scatter": _lib.SRFX_OSEM_SCATTER_ONLY}[stage]
         flags = ((_lib.SRFX_ACCUMULATE if accumulate else 0)
                  | (_lib.SRFX_TILED_PROJECTION if tiled_projection else 0)
-                 | _lib.schedule_flags(schedule) | st)
+                 | _lib.schedule_flags(schedule) | st
+                 | (_lib.SRFX_IMAGE_OP_FRAME if op_frame else 0))
         _lib.call("srfx_tor_plan_osem_subset", self._h, _ptr(image), og, _ptr(out), flags,
                   _stream())
         return out

@@ -104,6 +104,17 @@ def test_osem_vs_oracle(cuda, events, fused, tiled):
         model.tile, model.chunk = (4, 8, 8), 300  # many tiles and chunks at this size
     osem.iterate(Image(x, CENTER, SIZE), 2)
     if tiled:
+        # the same with the projection reading the world image in place
+        # instead of op-frame copies
+        model.stage_frames = False
+        x2 = torch.from_numpy(x0).to(cuda)
+        osem2 = OSEM(model, split_from_groups(groups),
+                     Image(torch.from_numpy(E).to(cuda), CENTER, SIZE), nb_subsets=3,
+                     fused=fused, tiled=True)
+        osem2.iterate(Image(x2, CENTER, SIZE), 2)
+        osem2.close()
+        assert H.rel_l2(x2.cpu().numpy(), x.cpu().numpy()) <= 1e-6
+    if tiled:
         assert osem.plans is not None and all(len(ps) == 3 for ps in osem.plans)
         osem.close()
     ref_groups = {a: np.ascontiguousarray(g.T.astype(np.float32))

@@ -167,3 +167,36 @@ def test_plan_osem_stages(cuda, case):
     with pytest.raises(SrfxError):
         plan.osem_subset(X, stage="project", tiled_projection=True)
     plan.close()
+
+
+@pytest.mark.parametrize("axis", ["x", "y"])
+def test_plan_osem_op_frame_copy(cuda, case, axis):
+    """A plan built for a strided (world-layout) image: the OSEM step reading
+    the contiguous op-frame copy (SRFX_IMAGE_OP_FRAME) equals the step
+    reading the world image in place, into the same world-layout output."""
+    import torch
+    from srf_amd import physics
+    from srf_amd._lib import SrfxError
+    from srf_amd.plan import TorPlan
+    lors, img = case
+    gx, gy, gz = H.TOR_GRID
+    # a world image whose group-`axis` frame is the op frame of the case
+    inv = {a: np.argsort(physics.PERM[a]) for a in "xy"}
+    frame = torch.from_numpy(img).to(cuda)                 # [gz][gy][gx]
+    world = frame.permute(*inv[axis]).contiguous()         # (nx, ny, nz)
+    assert torch.equal(physics.SplitLoRsModel.frame_copy(world, axis), frame)
+    g, c, s, st = physics.group_frame(world.shape, (0, 0, 0), (1, 1, 1), world.stride(), axis)
+    assert g == [gx, gy, gz]
+    L = _t(lors, cuda)
+    plan = TorPlan(L, H.TOR_GRID, H.TOR_CENTER, H.TOR_SIZE, H.KW, True, H.TOF_BIN, H.TOF_SIGMA2,
+                   strides=st, tile=(8, 16, 16), chunk=512, axis=axis)
+    a = plan.osem_subset(world, out=torch.zeros_like(world))
+    b = plan.osem_subset(frame, out=torch.zeros_like(world), op_frame=True)
+    assert float(a.abs().sum()) > 0
+    assert H.rel_l2(b.cpu().numpy(), a.cpu().numpy()) <= 1e-6
+    with pytest.raises(ValueError):
+        plan.osem_subset(world, out=torch.zeros_like(world), op_frame=True)  # wrong shape
+    with pytest.raises(SrfxError):
+        plan.osem_subset(frame, out=torch.zeros_like(world), op_frame=True,
+                         tiled_projection=True)
+    plan.close()
