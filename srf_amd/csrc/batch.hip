@@ -130,6 +130,22 @@ __global__ __launch_bounds__(256) void batch_project(const BatchJob *__restrict_
     }
 }
 
+template <bool YF>
+__device__ __forceinline__ void scatter_job(const BatchJob &job, const TileChunk &ch, double *ldsd)
+{
+    switch (job.p.box) {
+    case 3:
+        scatter_chunk<kBack, 3, true, YF>(job.p, job.g, ch, job.list, job.rec, ldsd);
+        break;
+    case 4:
+        scatter_chunk<kBack, 4, true, YF>(job.p, job.g, ch, job.list, job.rec, ldsd);
+        break;
+    default:
+        scatter_chunk<kBack, 0, true, YF>(job.p, job.g, ch, job.list, job.rec, ldsd);
+        break;
+    }
+}
+
 __global__ __launch_bounds__(256) void batch_scatter(const BatchJob *__restrict__ jobs,
                                                      const TileWork *__restrict__ work,
                                                      int nwork, unsigned *counter)
@@ -142,17 +158,10 @@ __global__ __launch_bounds__(256) void batch_scatter(const BatchJob *__restrict_
         const TileWork w = work[c];
         const BatchJob &job = jobs[__builtin_amdgcn_readfirstlane(w.job)];
         const TileChunk ch{w.tile, w.start, w.count, 0};
-        switch (job.p.box) {
-        case 3:
-            scatter_chunk<kBack, 3, true>(job.p, job.g, ch, job.list, job.rec, ldsd);
-            break;
-        case 4:
-            scatter_chunk<kBack, 4, true>(job.p, job.g, ch, job.list, job.rec, ldsd);
-            break;
-        default:
-            scatter_chunk<kBack, 0, true>(job.p, job.g, ch, job.list, job.rec, ldsd);
-            break;
-        }
+        if (flush_y_fastest(job.p))
+            scatter_job<true>(job, ch, ldsd);
+        else
+            scatter_job<false>(job, ch, ldsd);
     }
 }
 

@@ -222,24 +222,37 @@ done:
     return rc;
 }
 
+template <int MODE, bool VREC, bool YF>
+void launch_scatter_l(const TorParams &p, const srfx_tor_plan_s *pl, dim3 grid, dim3 block,
+                      unsigned sh, hipStream_t st)
+{
+    switch (p.box) {
+    case 3:
+        hipLaunchKernelGGL((tile_scatter<MODE, 3, VREC, YF>), grid, block, sh, st, p, pl->g,
+                           pl->chunks, pl->list, pl->rec);
+        break;
+    case 4:
+        hipLaunchKernelGGL((tile_scatter<MODE, 4, VREC, YF>), grid, block, sh, st, p, pl->g,
+                           pl->chunks, pl->list, pl->rec);
+        break;
+    default:
+        hipLaunchKernelGGL((tile_scatter<MODE, 0, VREC, YF>), grid, block, sh, st, p, pl->g,
+                           pl->chunks, pl->list, pl->rec);
+        break;
+    }
+}
+
+// The LDS tile runs fastest along the output's smaller stride (op y for the
+// world layout of the axis groups, op x for a contiguous op frame), so the
+// flush's atomics are coalesced.
 template <int MODE, bool VREC>
 void launch_scatter(const TorParams &p, const srfx_tor_plan_s *pl, dim3 grid, dim3 block,
                     unsigned sh, hipStream_t st)
 {
-    switch (p.box) {
-    case 3:
-        hipLaunchKernelGGL((tile_scatter<MODE, 3, VREC>), grid, block, sh, st, p, pl->g,
-                           pl->chunks, pl->list, pl->rec);
-        break;
-    case 4:
-        hipLaunchKernelGGL((tile_scatter<MODE, 4, VREC>), grid, block, sh, st, p, pl->g,
-                           pl->chunks, pl->list, pl->rec);
-        break;
-    default:
-        hipLaunchKernelGGL((tile_scatter<MODE, 0, VREC>), grid, block, sh, st, p, pl->g,
-                           pl->chunks, pl->list, pl->rec);
-        break;
-    }
+    if (flush_y_fastest(p))
+        launch_scatter_l<MODE, VREC, true>(p, pl, grid, block, sh, st);
+    else
+        launch_scatter_l<MODE, VREC, false>(p, pl, grid, block, sh, st);
 }
 
 int plan_scatter(srfx_tor_plan_s *pl, int mode, const float *vals, float *image,

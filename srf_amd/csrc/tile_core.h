@@ -239,9 +239,25 @@ __device__ __forceinline__ TileBox tile_box(const TorParams &p, const TileGeom &
     return b;
 }
 
+// Whether a scatter into p.out should use the y-fastest LDS tile.
+__host__ __device__ inline bool flush_y_fastest(const TorParams &p)
+{
+    return (p.oy < 0 ? -p.oy : p.oy) < (p.ox < 0 ? -p.ox : p.ox);
+}
+
+// Index of voxel (ix, iy) of slice s in the LDS tile of box b: x fastest, or
+// (YF) y fastest, so that a flush along the tile's fastest axis writes
+// consecutive addresses of an output whose unit stride is along op y.
+template <bool YF>
+__device__ __forceinline__ int lds_index(const TileGeom &g, const TileBox &b, int s, int ix, int iy)
+{
+    return YF ? ((s - b.sa) * g.Wx + (ix - b.x0)) * g.Wy + (iy - b.y0)
+              : ((s - b.sa) * g.Wy + (iy - b.y0)) * g.Wx + (ix - b.x0);
+}
+
 // The chord walk of one slice (patch_walk) clipped to the tile's window.
 //   KIND 0: returns sum w * lds[voxel]; KIND 1: lds[voxel] += w * scale.
-template <int KIND, typename T>
+template <int KIND, bool YF, typename T>
 __device__ __forceinline__ float window_walk(const TorParams &p, const TileGeom &g,
                                              const TileBox &b, const Event &ev, const Track &t,
                                              const Cross &c, float amp, int s, float scale,
@@ -253,7 +269,8 @@ __device__ __forceinline__ float window_walk(const TorParams &p, const TileGeom 
     const float eps = 1.0e-4f;
     j_lo = max(j_lo, (int)ceilf((c.dy0 - t.dymax) * p.rfinter_y - eps));
     j_hi = min(j_hi, (int)floorf((c.dy0 + t.dymax) * p.rfinter_y + eps));
-    const int row0 = ((s - b.sa) * g.Wy + (c.iy0 - b.y0)) * g.Wx + (c.ix0 - b.x0);
+    const int org = lds_index<YF>(g, b, s, c.ix0, c.iy0);
+    const int si = YF ? g.Wy : 1, sj = YF ? 1 : g.Wx;
     float acc = 0.0f;
     for (int j = j_lo; j <= j_hi; j++) {
         float dy = voxel_offset(c.dy0, j, p.finter_y);
@@ -272,7 +289,7 @@ __device__ __forceinline__ float window_walk(const TorParams &p, const TileGeom 
                 in = inside_exact(p, ev, s, c.ix0 + i, c.iy0 + j);
             if (in) {
                 float w = amp * __builtin_amdgcn_exp2f(d2 * ev.cw);
-                int li = row0 + j * g.Wx + i;
+                int li = org + j * sj + i * si;
                 if constexpr (KIND == 0)
                     acc = fmaf(w, lds[li], acc);
                 else
@@ -318,7 +335,7 @@ __device__ __forceinline__ void window_slices(const Track &t, int half, int P, i
 // One listing: the event's slices inside the tile, each through the NB x NB
 // candidate box clipped to the window when the event's box fits, else the
 // clipped chord walk.
-template <int KIND, int NB, typename T>
+template <int KIND, int NB, bool YF, typename T>
 __device__ __forceinline__ float tile_event(const TorParams &p, const TileGeom &g,
                                             const TileBox &b, const Event &ev, const Track &t,
                                             bool tof, float scale, T *lds)
@@ -335,22 +352,23 @@ __device__ __forceinline__ float tile_event(const TorParams &p, const TileGeom &
         float amp = tof ? tof_factor(p, ev, t, s) : 1.0f;
         if (NB > 0 && shape) {
             PatchIdx idx;
-            idx.base = ((s - b.sa) * g.Wy + (c.iy0 - b.y0)) * g.Wx + (c.ix0 - b.x0);
-            idx.si = 1;
-            idx.sj = g.Wx;
+            idx.base = lds_index<YF>(g, b, s, c.ix0, c.iy0);
+            idx.si = YF ? g.Wy : 1;
+            idx.sj = YF ? 1 : g.Wx;
             const float *src = nullptr;
             if constexpr (KIND == 0)
                 src = lds;
             const int il = max(0, b.x0 - c.ix0), ih = min(P - 1, b.x1 - c.ix0);
             const int jl = max(0, b.y0 - c.iy0), jh = min(P - 1, b.y1 - c.iy0);
+            constexpr int L = YF ? 2 : 1;
             if (shape == 1)
-                acc += slice_box<KIND, NBB, NBB - 1, true>(p, ev, t, c, amp, s, scale, il, ih, jl,
-                                                           jh, idx, src, lds);
+                acc += slice_box<KIND, NBB, NBB - 1, L>(p, ev, t, c, amp, s, scale, il, ih, jl,
+                                                        jh, idx, src, lds);
             else
-                acc += slice_box<KIND, NBB, NBB, true>(p, ev, t, c, amp, s, scale, il, ih, jl, jh,
-                                                       idx, src, lds);
+                acc += slice_box<KIND, NBB, NBB, L>(p, ev, t, c, amp, s, scale, il, ih, jl, jh,
+                                                    idx, src, lds);
         } else {
-            acc += window_walk<KIND>(p, g, b, ev, t, c, amp, s, scale, lds);
+            acc += window_walk<KIND, YF>(p, g, b, ev, t, c, amp, s, scale, lds);
         }
     }
     return acc;
@@ -392,7 +410,7 @@ __global__ __launch_bounds__(256) void record_proj_kernel(TorParams p, float4 *r
 // calls it; it synchronises the workgroup before and after the flush.
 // VREC: the values are the projections stored in the event records (the OSEM
 // step of a plan), not p.val.
-template <int MODE, int NB, bool VREC = false>
+template <int MODE, int NB, bool VREC = false, bool YF = false>
 __device__ __forceinline__ void scatter_chunk(const TorParams &p, const TileGeom &g,
                                               const TileChunk &ch,
                                               const int *__restrict__ list,
@@ -429,14 +447,24 @@ __device__ __forceinline__ void scatter_chunk(const TorParams &p, const TileGeom
             go = scale != 0.0f;
         }
         if (go)
-            tile_event<1, NB>(p, g, b, ev, t, tof, scale, ldsd);
+            tile_event<1, NB, YF>(p, g, b, ev, t, tof, scale, ldsd);
     }
     __syncthreads();
     for (int i = threadIdx.x; i < nv; i += blockDim.x) {
         float v = (float)ldsd[i];
         if (v != 0.0f) {
-            int lx = i % g.Wx, r = i / g.Wx;
-            int ly = r % g.Wy, ls = r / g.Wy;
+            int lx, ly, ls;
+            if (YF) {
+                ly = i % g.Wy;
+                const int r = i / g.Wy;
+                lx = r % g.Wx;
+                ls = r / g.Wx;
+            } else {
+                lx = i % g.Wx;
+                const int r = i / g.Wx;
+                ly = r % g.Wy;
+                ls = r / g.Wy;
+            }
             int ix = b.x0 + lx, iy = b.y0 + ly, s = b.sa + ls;
             if (ix <= b.x1 && iy <= b.y1 && s <= b.sb)
                 unsafeAtomicAdd(p.out + (ix * p.ox + iy * p.oy + s * p.oz), v);
@@ -448,14 +476,14 @@ __device__ __forceinline__ void scatter_chunk(const TorParams &p, const TileGeom
 // One workgroup per chunk.  The LDS tile is FP64: LDS FP64 atomic adds run
 // at several lanes per cycle where FP32 ones are serialised lane by lane,
 // and the tile sums gain precision.
-template <int MODE, int NB, bool VREC>
+template <int MODE, int NB, bool VREC, bool YF>
 __global__ __launch_bounds__(256) void tile_scatter(TorParams p, TileGeom g,
                                                     const TileChunk *__restrict__ chunks,
                                                     const int *__restrict__ list,
                                                     const float4 *__restrict__ rec)
 {
     extern __shared__ double ldsd[];
-    scatter_chunk<MODE, NB, VREC>(p, g, chunks[blockIdx.x], list, rec, ldsd);
+    scatter_chunk<MODE, NB, VREC, YF>(p, g, chunks[blockIdx.x], list, rec, ldsd);
 }
 
 // Gather of one chunk: partial[slot] = the listing's projection over the tile.
@@ -485,7 +513,7 @@ __global__ __launch_bounds__(256) void tile_gather(TorParams p, TileGeom g,
         Event ev;
         Track t;
         unpack_walk(p, rec, e, ev, t);
-        const float acc = tile_event<0, NB>(p, g, b, ev, t, tof, 0.0f, lds);
+        const float acc = tile_event<0, NB, false>(p, g, b, ev, t, tof, 0.0f, lds);
         partial[ch.start + j] = acc;
     }
 }

@@ -612,8 +612,9 @@ namespace {
 // masks, the scatter and the count as one packed bit mask per lane.
 //   KIND 0: returns sum w * src[idx]; 1: dst[idx] += w * scale (LDS: LDS
 //   atomics in dst's type, else FP32 global atomics); 2: returns the count.
-//   LDS: src / dst is an LDS tile, whose idx has unit column stride.
-template <int KIND, int BX, int BY, bool LDS, typename D>
+//   LDS: 0 strided image; 1 LDS tile with unit column stride (idx.si = 1);
+//   2 LDS tile with unit row stride (idx.sj = 1).
+template <int KIND, int BX, int BY, int LDS, typename D>
 __device__ __forceinline__ float slice_box(const TorParams &p, const Event &ev, const Track &t,
                                            const Cross &c, float amp, int s, float scale, int i_lo,
                                            int i_hi, int j_lo, int j_hi, const PatchIdx &idx,
@@ -638,10 +639,11 @@ __device__ __forceinline__ float slice_box(const TorParams &p, const Event &ev, 
         ry2[k] = dy * dy;
         ruy[k] = ev.uy * dy;
     }
-    // LDS tiles have unit column stride: one address per box row, formed
-    // after the decisions (opaque, so not re-formed per voxel inside the
-    // branches), and the columns as immediate offsets
-    int row[BY];
+    // LDS tiles have unit column (LDS 1) or row (LDS 2) stride: one address
+    // per box row (column), formed after the decisions (opaque, so not
+    // re-formed per voxel inside the branches), and the other index as an
+    // immediate offset
+    int row[BY], col[BX];
     float d2[NC];
     if constexpr (KIND == 0) {
         // Gather: the FP32 decisions stay per-candidate lane masks; only the
@@ -680,6 +682,7 @@ __device__ __forceinline__ float slice_box(const TorParams &p, const Event &ev, 
             for (int k = 0; k < NC; k++)
                 in[k] |= ((add >> k) & 1u) != 0;
         }
+        static_assert(LDS != 2, "gathers read x-fastest tiles");
 #pragma unroll
         for (int jj = 0; jj < BY; jj++)
             row[jj] = LDS ? opaque(idx(ib, jb + jj)) : 0;
@@ -737,15 +740,23 @@ __device__ __forceinline__ float slice_box(const TorParams &p, const Event &ev, 
         }
         if (KIND == 2)
             return (float)__builtin_popcount(in);
+        if constexpr (LDS == 2) {
 #pragma unroll
-        for (int jj = 0; jj < BY; jj++)
-            row[jj] = LDS ? opaque(idx(ib, jb + jj)) : 0;
+            for (int ii = 0; ii < BX; ii++)
+                col[ii] = opaque(idx(ib + ii, jb));
+        } else {
+#pragma unroll
+            for (int jj = 0; jj < BY; jj++)
+                row[jj] = LDS ? opaque(idx(ib, jb + jj)) : 0;
+        }
 #pragma unroll
         for (int k = 0; k < NC; k++) {
             if ((in >> k) & 1u) {
                 float w = amp * __builtin_amdgcn_exp2f(d2[k] * ev.cw);
-                D *q = dst + (LDS ? row[k / BX] + k % BX : idx(ib + k % BX, jb + k / BX));
-                if constexpr (LDS)
+                D *q = dst + (LDS == 2   ? col[k % BX] + k / BX
+                              : LDS == 1 ? row[k / BX] + k % BX
+                                         : idx(ib + k % BX, jb + k / BX));
+                if constexpr (LDS != 0)
                     atomicAdd(q, (D)(w * scale));
                 else
                     unsafeAtomicAdd(q, w * scale);
@@ -791,9 +802,9 @@ __device__ __forceinline__ float slice_box_image(const TorParams &p, const Event
         idx.sj = p.oy;
     }
     if (shape == 1)
-        return slice_box<KIND, NB, NB - 1, false>(p, ev, t, c, amp, s, scale, i_lo, i_hi, j_lo,
+        return slice_box<KIND, NB, NB - 1, 0>(p, ev, t, c, amp, s, scale, i_lo, i_hi, j_lo,
                                                   j_hi, idx, p.img, p.out);
-    return slice_box<KIND, NB, NB, false>(p, ev, t, c, amp, s, scale, i_lo, i_hi, j_lo, j_hi,
+    return slice_box<KIND, NB, NB, 0>(p, ev, t, c, amp, s, scale, i_lo, i_hi, j_lo, j_hi,
                                           idx, p.img, p.out);
 }
 
