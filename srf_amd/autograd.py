@@ -37,8 +37,9 @@ class _TorProject(torch.autograd.Function):
             lors, n, ld = _soa(soa, 10)
             p = torch.empty(n, dtype=torch.float32, device=x.device)
             if n:
-                _lib.call("srfx_tor_project", _ptr(lors), n, ld, _ptr(img.data),
-                          _grid(model, img, a), model._model(), _ptr(p), 0, _stream())
+                src, g = model._source(img, a)  # op-frame copy when the model stages
+                _lib.call("srfx_tor_project", _ptr(lors), n, ld, _ptr(src), g, model._model(),
+                          _ptr(p), 0, _stream())
             outs.append(p)
         ctx.model = model
         ctx.plans = plans

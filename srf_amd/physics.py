@@ -51,7 +51,8 @@ class SplitLoRsModel:
         self.geo_sigma2_flag = bool(geo_sigma2_flag)
         self.tof_sigma2 = 1.0e4 if tof_sigma2 is None else float(tof_sigma2)
         self.tof_bin = 1.0e4 if tof_bin is None else float(tof_bin)
-        # tile plans' OSEM step: project from op-frame copies of the image
+        # projections read op-frame copies of the image (the reference's
+        # transposes) instead of the world image in place
         self.stage_frames = True
         # launch schedule per op: (lanes_per_event, block) or None (default)
         self.schedule = {"projection": None, "backprojection": None, "osem_subset": None}
@@ -151,14 +152,24 @@ class SplitLoRsModel:
                 t.shape, plan.center, plan.size, t.stride(), plan.axis)[3]]:
             raise ValueError("image layout differs from the one the plan was built for")
 
+    def _source(self, image, a):
+        """The image a projection of group ``a`` reads and its grid: the
+        op-frame copy (contiguous, stage_frames) or the world image in place."""
+        x = _world(image)
+        if self.stage_frames:
+            g, c, s, _ = group_frame(x.shape, image.center, image.size, x.stride(), a)
+            return self.frame_copy(x, a), _lib.make_grid(g, c, s)
+        return x, self._grid(image, a)
+
     def project_group(self, image, lm, a):
         """Projection of the events ``lm`` (ListModeData) of group ``a``."""
         x = _world(image)
         lors, n, ld = _soa(lm.soa, 10)
         p = torch.empty(n, dtype=torch.float32, device=x.device)
         if n:
-            _lib.call("srfx_tor_project", _ptr(lors), n, ld, _ptr(x), self._grid(image, a),
-                      self._model(), _ptr(p), self._flags("projection"), _stream())
+            src, g = self._source(image, a)
+            _lib.call("srfx_tor_project", _ptr(lors), n, ld, _ptr(src), g, self._model(),
+                      _ptr(p), self._flags("projection"), _stream())
         return p
 
     def projection(self, image, data):
@@ -233,9 +244,9 @@ class SplitLoRsModel:
             if plans is not None:
                 self.plan_osem_group(self._planned(plans, a, n, out), x, out, a)
                 continue
-            g_in = self._grid(image, a)
+            src, g_in = self._source(image, a)
             g_out = self._grid(Image(out, image.center, image.size), a)
-            _lib.call("srfx_tor_osem_subset", _ptr(lors), n, ld, _ptr(x), g_in, g_out,
+            _lib.call("srfx_tor_osem_subset", _ptr(lors), n, ld, _ptr(src), g_in, g_out,
                       self._model(), _ptr(out), _lib.SRFX_ACCUMULATE | self._flags("osem_subset"),
                       _stream())
         return Image(out, image.center, image.size)
