@@ -297,6 +297,14 @@ int srfx_tor_batch_destroy(srfx_tor_batch *plan);
  * n voxels, evaluated as (x * emap) * bp). */
 int srfx_mlem_update(float *x, const float *emap, const float *bp, int64_t n, void *stream);
 
+/* Copy of an image into an op frame, the reference's transpose of the image
+ * into an axis group's frame (split_lors.py:79): out[k][j][i] = img[i * sx +
+ * j * sy + k * sz] with (sx, sy, sz) the element strides of grid (all zero:
+ * already contiguous), out contiguous [gz][gy][gx].  img and out must not
+ * overlap.  The projections then read voxel rows from shared cache lines
+ * (SRFX_IMAGE_OP_FRAME, or a contiguous grid). */
+int srfx_frame_copy(const float *img, srfx_grid grid, float *out, void *stream);
+
 /* PSF blur around the projector (PSFReconStep, model/recon_step.py:56-115).
  * The image, nx x ny x nz contiguous with z fastest, is a matrix X of nrows =
  * nx * ny rows by nz columns; out = S X M, S a sparse nrows x nrows matrix in

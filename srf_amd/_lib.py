@@ -96,6 +96,7 @@ _PROTOS = {
     "srfx_siddon_osem_subset": [_P, _I64, _I64, _P, SrfxGrid, _F, _F, _P, _U32, _P],
     "srfx_siddon_count_nnz": [_P, _I64, _I64, SrfxGrid, _F, _F, _P, _P, _P],
     "srfx_mlem_update": [_P, _P, _P, _I64, _P],
+    "srfx_frame_copy": [_P, SrfxGrid, _P, _P],
     "srfx_tor_plan_create": [_P, _I64, _I64, _I32, SrfxGrid, SrfxTorModel, SrfxTile,
                              ctypes.POINTER(ctypes.c_void_p), _P],
     "srfx_tor_plan_info": [_P, ctypes.POINTER(_I64), ctypes.POINTER(_I32)],

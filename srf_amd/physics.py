@@ -252,10 +252,18 @@ class SplitLoRsModel:
         return Image(out, image.center, image.size)
 
     @staticmethod
-    def frame_copy(x, a):
+    def frame_copy(x, a, out=None):
         """The op frame of group ``a`` of the world image x as a contiguous
-        [gz][gy][gx] tensor: the reference's transpose (split_lors.py:79)."""
-        return x.permute(*PERM[a]).contiguous()
+        [gz][gy][gx] tensor: the reference's transpose (split_lors.py:79),
+        one LDS-tiled pass (srfx_frame_copy)."""
+        _check_dev(x, "image")
+        g, c, s, st = group_frame(x.shape, (0, 0, 0), (1, 1, 1), x.stride(), a)
+        if out is None:
+            out = torch.empty((g[2], g[1], g[0]), dtype=torch.float32, device=x.device)
+        elif tuple(out.shape) != (g[2], g[1], g[0]) or not out.is_contiguous():
+            raise ValueError(f"out must be a contiguous {tuple(g[::-1])} tensor")
+        _lib.call("srfx_frame_copy", _ptr(x), _lib.make_grid(g, c, s, st), _ptr(out), _stream())
+        return out
 
     def plan_osem_group(self, plan, x, out, a, stage=None):
         """The OSEM step of one group's plan into the world image ``out``

@@ -86,6 +86,9 @@ def test_invalid_arguments_are_rejected():
                               None) == _lib.SRFX_EINVAL
     assert lib.srfx_psf_apply(fake, fake, fake, 8, fake, 2049, 0, 0, fake, other,
                               None) == _lib.SRFX_EUNSUPPORTED
+    # frame copy: in place, null output
+    assert lib.srfx_frame_copy(fake, _grid(), fake, None) == _lib.SRFX_EINVAL
+    assert lib.srfx_frame_copy(fake, _grid(), None, None) == _lib.SRFX_EINVAL
     assert lib.srfx_strerror(_lib.SRFX_EINVAL) == b"invalid argument"
 
 

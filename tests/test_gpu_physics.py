@@ -189,3 +189,19 @@ def test_siddon_model_vs_oracle(cuda):
     xt = np.ascontiguousarray(np.transpose(x, [2, 1, 0]))
     ref = O.siddon_project(lors, xt, grid, center, size, 6.0, 400.0, prec="f64")
     assert H.rel_l2(p.values.cpu().numpy(), ref) <= 1e-5
+
+
+@pytest.mark.parametrize("shape", [(40, 36, 24), (33, 65, 7), (1, 5, 64), (200, 200, 100)])
+def test_frame_copy_equals_transpose(cuda, shape):
+    """srfx_frame_copy equals the reference's transpose into each group frame
+    (split_lors.py:79, torch permute + contiguous), also from a strided view."""
+    from srf_amd import physics
+    x = torch.rand(shape, generator=torch.Generator().manual_seed(1)).to(cuda)
+    for a in "xyz":
+        want = x.permute(*physics.PERM[a]).contiguous()
+        assert torch.equal(physics.SplitLoRsModel.frame_copy(x, a), want), a
+    big = torch.rand((shape[0] + 3, shape[1] + 2, shape[2] + 5), device=cuda)
+    view = big[1:1 + shape[0], 2:2 + shape[1], 0:shape[2]]
+    for a in "xy":
+        assert torch.equal(physics.SplitLoRsModel.frame_copy(view, a),
+                           view.permute(*physics.PERM[a]).contiguous()), a
