@@ -256,7 +256,8 @@ __device__ __forceinline__ int lds_index(const TileGeom &g, const TileBox &b, in
 }
 
 // The chord walk of one slice (patch_walk) clipped to the tile's window.
-//   KIND 0: returns sum w * lds[voxel]; KIND 1: lds[voxel] += w * scale.
+//   KIND 0: returns sum w * lds[voxel]; KIND 1: lds[voxel] += w (amp carries
+//   the scale).
 template <int KIND, bool YF, typename T>
 __device__ __forceinline__ float window_walk(const TorParams &p, const TileGeom &g,
                                              const TileBox &b, const Event &ev, const Track &t,
@@ -293,7 +294,7 @@ __device__ __forceinline__ float window_walk(const TorParams &p, const TileGeom 
                 if constexpr (KIND == 0)
                     acc = fmaf(w, lds[li], acc);
                 else
-                    atomicAdd(&lds[li], (T)(w * scale));
+                    atomicAdd(&lds[li], (T)w);
             }
         }
     }
@@ -345,11 +346,15 @@ __device__ __forceinline__ float tile_event(const TorParams &p, const TileGeom &
     int s0 = max(ev.s_lo, b.sa), s1 = min(ev.s_hi, b.sb);
     const int P = p.patch;
     window_slices(t, p.half, P, b.x0, b.x1, b.y0, b.y1, s0, s1);
+    const float ts = p.tofk * scale;
     float acc = 0.0f;
     for (int s = s0; s <= s1; s++) {
         Cross c;
         cross_at(p, ev, t, s, c);
-        float amp = tof ? tof_factor(p, ev, t, s) : 1.0f;
+        // scatters: the scale rides in amp (one product per slice, not per
+        // voxel), with the TOF constant folded into it once
+        float amp = KIND == 1 ? (tof ? tof_exp(p, ev, t, s) * ts : scale)
+                              : (tof ? tof_factor(p, ev, t, s) : 1.0f);
         if (NB > 0 && shape) {
             PatchIdx idx;
             idx.base = lds_index<YF>(g, b, s, c.ix0, c.iy0);

@@ -487,13 +487,19 @@ __device__ __forceinline__ bool inside_exact(const TorParams &p, const Event &ev
 // distance from the TOF centre to the crossing point; the reference's
 // d2_tof = D - d2 (tor.cu.cc:87) has its -d2 part folded into the per-voxel
 // exponent.
-__device__ __forceinline__ float tof_factor(const TorParams &p, const Event &ev, const Track &t,
-                                            int s)
+__device__ __forceinline__ float tof_exp(const TorParams &p, const Event &ev, const Track &t,
+                                         int s)
 {
     float k = (float)(s - ev.s_lo);
     float vx = t.v0x - k * t.dvx, vy = t.v0y - k * t.dvy, vz = t.v0z - k * t.dvz;
     float D = vx * vx + vy * vy + vz * vz;
-    return p.tofk * __builtin_amdgcn_exp2f(D * ev.ctof);
+    return __builtin_amdgcn_exp2f(D * ev.ctof);
+}
+
+__device__ __forceinline__ float tof_factor(const TorParams &p, const Event &ev, const Track &t,
+                                            int s)
+{
+    return p.tofk * tof_exp(p, ev, t, s);
 }
 
 // Offset of voxel column i (row j) from the crossing point, and the squared
@@ -537,7 +543,7 @@ __device__ __forceinline__ void visit(const TorParams &p, const Event &ev, int s
             if (KIND == 0)
                 acc = fmaf(w, p.img[ix * p.sx + iy * p.sy + s * p.sz], acc);
             else
-                unsafeAtomicAdd(p.out + (ix * p.ox + iy * p.oy + s * p.oz), w * scale);
+                unsafeAtomicAdd(p.out + (ix * p.ox + iy * p.oy + s * p.oz), w);
         }
     }
 }
@@ -610,7 +616,8 @@ namespace {
 // voxels in the reference's order (rows outer, columns inner), same
 // arithmetic as patch_walk.  The gather keeps the decisions as per-voxel lane
 // masks, the scatter and the count as one packed bit mask per lane.
-//   KIND 0: returns sum w * src[idx]; 1: dst[idx] += w * scale (LDS: LDS
+//   KIND 0: returns sum w * src[idx]; 1: dst[idx] += w, amp carrying the
+//   scale (LDS: LDS
 //   atomics in dst's type, else FP32 global atomics); 2: returns the count.
 //   LDS: 0 strided image; 1 LDS tile with unit column stride (idx.si = 1);
 //   2 LDS tile with unit row stride (idx.sj = 1).
@@ -757,9 +764,9 @@ __device__ __forceinline__ float slice_box(const TorParams &p, const Event &ev, 
                               : LDS == 1 ? row[k / BX] + k % BX
                                          : idx(ib + k % BX, jb + k / BX));
                 if constexpr (LDS != 0)
-                    atomicAdd(q, (D)(w * scale));
+                    atomicAdd(q, (D)w);
                 else
-                    unsafeAtomicAdd(q, w * scale);
+                    unsafeAtomicAdd(q, w);
             }
         }
         return 0.0f;
@@ -818,11 +825,15 @@ __device__ __forceinline__ float walk_event(const TorParams &p, const Event &ev,
 {
     constexpr int NBB = NB > 1 ? NB : 2;
     const int shape = p.full_patch ? 0 : box_shape<NB>(t);
+    const float ts = p.tofk * scale;
     float acc = 0.0f;
     for (int s = s_first; s <= s_last; s += rstep) {
         Cross c;
         cross_at(p, ev, t, s, c);
-        float amp = tof ? tof_factor(p, ev, t, s) : 1.0f;
+        // scatters: the scale rides in amp (one product per slice, not per
+        // voxel), with the TOF constant folded into it once
+        float amp = KIND == 1 ? (tof ? tof_exp(p, ev, t, s) * ts : scale)
+                              : (tof ? tof_factor(p, ev, t, s) : 1.0f);
         if (NB > 0 && shape)
             acc += slice_box_image<KIND, NBB>(p, ev, t, c, amp, s, scale, shape);
         else if (p.full_patch)
