@@ -455,25 +455,40 @@ __device__ __forceinline__ void scatter_chunk(const TorParams &p, const TileGeom
             tile_event<1, NB, YF>(p, g, b, ev, t, tof, scale, ldsd);
     }
     __syncthreads();
+    // Flush.  Tile element i = (c * nb + m) * na + f (f along the fastest
+    // axis: y for YF, else x; m the middle one; c the slice) and its output
+    // offset advance by the block size each step, with carries instead of
+    // divisions by the tile extents.
+    const int na = YF ? g.Wy : g.Wx, nb = YF ? g.Wx : g.Wy;
+    const int oa = YF ? p.oy : p.ox, ob = YF ? p.ox : p.oy;
+    const int la = YF ? b.y1 - b.y0 : b.x1 - b.x0, lb = YF ? b.x1 - b.x0 : b.y1 - b.y0;
+    const int lc = b.sb - b.sa;
+    int f = threadIdx.x % na, m, c;
+    {
+        const int r = threadIdx.x / na;
+        m = r % nb;
+        c = r / nb;
+    }
+    const int df = blockDim.x % na, dr = blockDim.x / na;
+    const int dm = dr % nb, dc = dr / nb;
+    float *const base = p.out + ((int64_t)b.x0 * p.ox + (int64_t)b.y0 * p.oy + (int64_t)b.sa * p.oz);
+    int off = f * oa + m * ob + c * p.oz;
+    const int step = df * oa + dm * ob + dc * p.oz;
+    const int wrap_f = ob - na * oa, wrap_m = p.oz - nb * ob;
     for (int i = threadIdx.x; i < nv; i += blockDim.x) {
-        float v = (float)ldsd[i];
-        if (v != 0.0f) {
-            int lx, ly, ls;
-            if (YF) {
-                ly = i % g.Wy;
-                const int r = i / g.Wy;
-                lx = r % g.Wx;
-                ls = r / g.Wx;
-            } else {
-                lx = i % g.Wx;
-                const int r = i / g.Wx;
-                ly = r % g.Wy;
-                ls = r / g.Wy;
-            }
-            int ix = b.x0 + lx, iy = b.y0 + ly, s = b.sa + ls;
-            if (ix <= b.x1 && iy <= b.y1 && s <= b.sb)
-                unsafeAtomicAdd(p.out + (ix * p.ox + iy * p.oy + s * p.oz), v);
-        }
+        const float v = (float)ldsd[i];
+        if (v != 0.0f && f <= la && m <= lb && c <= lc)
+            unsafeAtomicAdd(base + off, v);
+        f += df;
+        off += step;
+        const bool cf = f >= na;
+        f -= cf ? na : 0;
+        off += cf ? wrap_f : 0;
+        m += dm + (cf ? 1 : 0);
+        const bool cm = m >= nb;
+        m -= cm ? nb : 0;
+        off += cm ? wrap_m : 0;
+        c += dc + (cm ? 1 : 0);
     }
     __syncthreads();
 }
