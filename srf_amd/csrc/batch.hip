@@ -5,6 +5,9 @@
 // its events (tile_core.h).  A subset step is four kernels on one stream:
 //   batch_zero     zero every reconstruction's backprojection buffer and the
 //                  two work counters;
+//   batch_frames   copy every reconstruction's image into the op frame of
+//                  each of its axis groups (frame_core.h), which the
+//                  projection reads (a box row of voxels shares a line);
 //   batch_project  persistent projection: each block takes the next chunk of
 //                  <= 256 events (of any reconstruction and group) from a
 //                  counter, walks them from their plan's walk records and
@@ -16,11 +19,12 @@
 //   batch_update   x <- x * E * bp for every reconstruction.
 // Every block of a persistent kernel exits once its counter passes the end of
 // the table.  The step allocates nothing and issues no memset, so a caller can
-// capture it in a HIP graph.
+// capture it in a HIP graph.  Five kernels in all.
 #include <algorithm>
 #include <new>
 #include <vector>
 
+#include "frame_core.h"
 #include "tile_core.h"
 
 namespace srfx {
@@ -165,6 +169,38 @@ __global__ __launch_bounds__(256) void batch_scatter(const BatchJob *__restrict_
     }
 }
 
+// One op-frame copy: the world image read with the group's op-frame strides
+// into a contiguous [gz][gy][gx] buffer, tiles [first, first + tx * ty * gz)
+// of the batch_frames grid.
+struct FrameJob {
+    const float *src;
+    float *dst;
+    int gx, gy, gz, tx, ty;
+    int64_t sx, sy, sz;
+    long long first;
+};
+
+__global__ __launch_bounds__(256) void batch_frames(const FrameJob *__restrict__ jobs, int njobs)
+{
+    __shared__ float tile[kFrameTile][kFrameTile + 1];
+    const long long blk = blockIdx.x;
+    int lo = 0, hi = njobs - 1;
+    while (lo < hi) { // last job whose first tile is <= blk
+        const int mid = (lo + hi + 1) >> 1;
+        if (jobs[mid].first <= blk)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    const FrameJob &f = jobs[lo];
+    const long long t = blk - f.first;
+    const int bx = (int)(t % f.tx);
+    const long long r = t / f.tx;
+    const int by = (int)(r % f.ty);
+    frame_tile(f.src, f.gx, f.gy, f.sx, f.sy, f.sz, f.dst, bx * kFrameTile, by * kFrameTile,
+               r / f.ty, tile);
+}
+
 // Op frame of axis group a for a C-order (nx, ny, nz) world image, as
 // physics/split_lors.py:54-85 derives it by transposing: op (x, y, z) =
 // world (y, z, x) for the x group, (x, z, y) for y, (x, y, z) for z.
@@ -201,6 +237,10 @@ struct srfx_tor_batch {
     srfx::VoxRange *d_vox = nullptr;
     long long total_vox = 0;
     unsigned *d_counters = nullptr;
+    std::vector<float *> frames; // op-frame copies, one per (reconstruction, non-empty group)
+    srfx::FrameJob *d_frames = nullptr;
+    int nframes = 0;
+    long long frame_blocks = 0;
     unsigned scatter_lds = 0;
     int proj_blocks = 0, scatter_blocks = 0, elem_blocks = 0;
 };
@@ -221,6 +261,9 @@ static void free_plan(srfx_tor_batch *b)
         (void)hipFree(w);
     (void)hipFree(b->d_vox);
     (void)hipFree(b->d_counters);
+    for (float *f : b->frames)
+        (void)hipFree(f);
+    (void)hipFree(b->d_frames);
     delete b;
 }
 
@@ -261,6 +304,7 @@ extern "C" int srfx_tor_batch_create(const srfx_recon *recons, int32_t nrecon,
     std::vector<std::vector<EvChunk>> ev(nb_subsets);
     std::vector<std::vector<TileWork>> work(nb_subsets);
     std::vector<VoxRange> vox;
+    std::vector<FrameJob> frames;
     long long off = 0;
     int rc_status = SRFX_OK;
     // Per subset: contiguous rows [k n / K, (k+1) n / K) of every group
@@ -269,6 +313,31 @@ extern "C" int srfx_tor_batch_create(const srfx_recon *recons, int32_t nrecon,
         const srfx_recon &rc = recons[r];
         for (int a = 0; a < 3 && !rc_status; a++) {
             const int64_t n = rc.n[a];
+            // the group's op-frame copy of the image, read by its projections
+            float *frame = nullptr;
+            const srfx_grid gg = group_grid(rc, a);
+            if (n > 0) {
+                const size_t nv = (size_t)rc.shape[0] * rc.shape[1] * rc.shape[2];
+                if (hipMalloc((void **)&frame, sizeof(float) * nv) != hipSuccess) {
+                    rc_status = SRFX_ENOMEM;
+                    break;
+                }
+                b->frames.push_back(frame);
+                FrameJob f;
+                f.src = rc.image;
+                f.dst = frame;
+                f.gx = gg.shape[0];
+                f.gy = gg.shape[1];
+                f.gz = gg.shape[2];
+                f.tx = (f.gx + kFrameTile - 1) / kFrameTile;
+                f.ty = (f.gy + kFrameTile - 1) / kFrameTile;
+                f.sx = gg.stride[0];
+                f.sy = gg.stride[1];
+                f.sz = gg.stride[2];
+                f.first = b->frame_blocks;
+                b->frame_blocks += (long long)f.tx * f.ty * f.gz;
+                frames.push_back(f);
+            }
             for (int k = 0; k < nb_subsets; k++) {
                 const int64_t lo = k * n / nb_subsets, hi = (k + 1) * n / nb_subsets;
                 if (hi <= lo)
@@ -282,7 +351,10 @@ extern "C" int srfx_tor_batch_create(const srfx_recon *recons, int32_t nrecon,
                 b->plans.push_back(plan);
                 BatchJob job;
                 job.p = plan->p;
-                job.p.img = rc.image;
+                job.p.img = frame; // contiguous op frame
+                job.p.sx = 1;
+                job.p.sy = gg.shape[0];
+                job.p.sz = gg.shape[0] * gg.shape[1];
                 job.p.out = rc.bp;
                 job.p.val = plan->proj;
                 job.g = plan->g;
@@ -315,7 +387,8 @@ extern "C" int srfx_tor_batch_create(const srfx_recon *recons, int32_t nrecon,
     }
     b->njobs = (int)jobs.size();
     b->total_vox = off;
-    bool ok = upload(&b->d_jobs, jobs) && upload(&b->d_vox, vox) &&
+    b->nframes = (int)frames.size();
+    bool ok = upload(&b->d_jobs, jobs) && upload(&b->d_vox, vox) && upload(&b->d_frames, frames) &&
               hipMalloc(&b->d_counters, 2 * sizeof(unsigned)) == hipSuccess;
     for (int k = 0; ok && k < nb_subsets; k++) {
         EvChunk *e = nullptr;
@@ -352,6 +425,9 @@ extern "C" int srfx_tor_batch_subset(srfx_tor_batch *b, int32_t subset, void *st
     hipStream_t st = (hipStream_t)stream;
     hipLaunchKernelGGL(batch_zero, dim3(b->elem_blocks), dim3(256), 0, st, b->d_vox, b->nrec,
                        b->total_vox, b->d_counters);
+    if (b->nframes > 0)
+        hipLaunchKernelGGL(batch_frames, dim3((unsigned)b->frame_blocks), dim3(256), 0, st,
+                           b->d_frames, b->nframes);
     if (b->nev[subset] > 0)
         hipLaunchKernelGGL(batch_project, dim3(b->proj_blocks), dim3(256), 0, st, b->d_jobs,
                            b->d_ev[subset], b->nev[subset], b->d_counters);

@@ -5,34 +5,18 @@
 // axis is world z (unit stride) and op x has a large stride, so each
 // workgroup moves a 32 x 32 (op x, op y) tile of one slice through LDS,
 // reading along op y and writing along op x, both coalesced.
-#include "common.h"
+#include "frame_core.h"
 
 namespace srfx {
 namespace {
-
-constexpr int kFrameTile = 32;
 
 __global__ __launch_bounds__(256) void frame_copy_kernel(const float *__restrict__ img, int gx,
                                                          int gy, int64_t sx, int64_t sy,
                                                          int64_t sz, float *__restrict__ out)
 {
     __shared__ float tile[kFrameTile][kFrameTile + 1];
-    const int tx = threadIdx.x % kFrameTile, ty = threadIdx.x / kFrameTile; // 32 x 8
-    const int i0 = blockIdx.x * kFrameTile, j0 = blockIdx.y * kFrameTile;
-    const int64_t k = blockIdx.z;
-    // read: lanes along op y (j), rows along op x (i)
-    for (int r = ty; r < kFrameTile; r += 8) {
-        const int i = i0 + r, j = j0 + tx;
-        if (i < gx && j < gy)
-            tile[r][tx] = img[i * sx + j * sy + k * sz];
-    }
-    __syncthreads();
-    // write: lanes along op x (i), rows along op y (j)
-    for (int r = ty; r < kFrameTile; r += 8) {
-        const int i = i0 + tx, j = j0 + r;
-        if (i < gx && j < gy)
-            out[(k * gy + j) * gx + i] = tile[tx][r];
-    }
+    frame_tile(img, gx, gy, sx, sy, sz, out, blockIdx.x * kFrameTile, blockIdx.y * kFrameTile,
+               blockIdx.z, tile);
 }
 
 } // namespace
