@@ -210,8 +210,20 @@ int plan_build(srfx_tor_plan_s *pl, hipStream_t st, int chunk, bool keep_slots)
         rc = SRFX_ELAUNCH;
         goto done;
     }
-    if (mismatch)
+    if (mismatch) {
         rc = SRFX_ELAUNCH; // count and fill passes disagreed: no valid plan
+        goto done;
+    }
+    // without the tiled projection's slot tables the listings of a chunk may
+    // be reordered: longest walks first
+    if (!keep_slots && pl->nchunks > 0 && chunk <= 16 * 256) {
+        hipLaunchKernelGGL(sort_chunks, dim3(pl->nchunks), dim3(256),
+                           sizeof(int) * (33 + chunk), st, pl->p, pl->g, pl->chunks, pl->list,
+                           pl->rec);
+        if ((rc = launch_status()))
+            goto done;
+        pl->g.sorted = 1;
+    }
 done:
     // host buffers above are read by the copies: finish them before returning
     (void)hipStreamSynchronize(st);

@@ -33,6 +33,7 @@ namespace srfx {
 struct TileGeom {
     int S, Wx, Wy;       // slices per slab, window width along op x and y (voxels)
     int nwx, nwy, nslab; // windows per axis and number of slabs
+    int sorted;          // chunks' listings ordered by slice count (sort_chunks)
 };
 
 
@@ -432,7 +433,8 @@ __device__ __forceinline__ void scatter_chunk(const TorParams &p, const TileGeom
     // whose LDS atomics would hit the same voxels in the same instruction.
     const int stride = (ch.count + 63) >> 6;
     for (int q = threadIdx.x; q < 64 * stride; q += blockDim.x) {
-        const int j = (q & 63) * stride + (q >> 6);
+        // sorted plans: consecutive listings (similar slice counts) per wave
+        const int j = g.sorted ? q : (q & 63) * stride + (q >> 6);
         if (j >= ch.count)
             continue;
         const int64_t e = list[ch.start + j];
@@ -491,6 +493,63 @@ __device__ __forceinline__ void scatter_chunk(const TorParams &p, const TileGeom
         c += dc + (cm ? 1 : 0);
     }
     __syncthreads();
+}
+
+// Plan build, after the fill pass: order each chunk's listings by the number
+// of slices they walk in the tile, longest first (a counting sort in LDS,
+// stable), so that the consecutive listings a wave of the scatter takes walk
+// about as many slices each (the order inside a bucket is the atomics').  One
+// workgroup per chunk; LDS: 33 counters and `chunk` ints.  Needs ch.count <=
+// 16 * blockDim.x.
+__global__ __launch_bounds__(256) void sort_chunks(TorParams p, TileGeom g,
+                                                   const TileChunk *__restrict__ chunks,
+                                                   int *__restrict__ list,
+                                                   const float4 *__restrict__ rec)
+{
+    extern __shared__ int sh[];
+    constexpr int kBuckets = 32, kPer = 16;
+    int *cnt = sh, *ord = sh + kBuckets + 1;
+    const TileChunk ch = chunks[blockIdx.x];
+    const TileBox b = tile_box(p, g, ch.tile);
+    for (int i = threadIdx.x; i <= kBuckets; i += blockDim.x)
+        cnt[i] = 0;
+    __syncthreads();
+    int key[kPer], ev[kPer];
+#pragma unroll
+    for (int r = 0; r < kPer; r++) {
+        const int j = (int)threadIdx.x * kPer + r;
+        key[r] = -1;
+        if (j < ch.count) {
+            ev[r] = list[ch.start + j];
+            Event e;
+            Track t;
+            unpack_walk(p, rec, ev[r], e, t);
+            int s0, s1;
+            s0 = max(e.s_lo, b.sa);
+            s1 = min(e.s_hi, b.sb);
+            window_slices(t, p.half, p.patch, b.x0, b.x1, b.y0, b.y1, s0, s1);
+            // longest first: bucket 0 holds the longest walks
+            key[r] = kBuckets - 1 - min(max(s1 - s0 + 1, 0), kBuckets - 1);
+            atomicAdd(&cnt[key[r]], 1);
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int acc = 0;
+        for (int k = 0; k < kBuckets; k++) {
+            const int c = cnt[k];
+            cnt[k] = acc;
+            acc += c;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < kPer; r++)
+        if (key[r] >= 0)
+            ord[atomicAdd(&cnt[key[r]], 1)] = ev[r];
+    __syncthreads();
+    for (int j = threadIdx.x; j < ch.count; j += blockDim.x)
+        list[ch.start + j] = ord[j];
 }
 
 // One workgroup per chunk.  The LDS tile is FP64: LDS FP64 atomic adds run
