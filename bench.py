@@ -46,7 +46,7 @@ def parse():
                              "large_c5", "hbm_copy", "psf_c2", "io_c2"])
     ap.add_argument("--autotune", action="store_true",
                     help="tune the projection schedule and the tile shape on a sample of the "
-                         "events before timing (default: the fixed defaults, tile 8x16x32, "
+                         "events before timing (default: the fixed defaults, tile 8x32x20, "
                          "chunk 2048; the tuner's candidates lie within a few percent of "
                          "each other, inside its timing noise)")
     ap.add_argument("--no-sort", dest="sort", action="store_false",

@@ -219,9 +219,9 @@ srfx_grid group_grid(const srfx_recon &r, int a)
     return g;
 }
 
-// Tile of the batch's plans: 8 slices x 16 x 32 voxels, <= 2048 events per
-// workgroup (the FP64 LDS tile is 32 KB).
-constexpr srfx_tile kBatchTile = {8, 16, 32, 2048, 256, 0};
+// Tile of the batch's plans: 8 slices x 32 x 20 voxels, <= 2048 events per
+// workgroup (the FP64 LDS tile is 40 KB: four workgroups per CU).
+constexpr srfx_tile kBatchTile = {8, 32, 20, 2048, 256, 0};
 
 } // namespace
 } // namespace srfx
