@@ -613,12 +613,12 @@ namespace {
 // is in the box when t.nx, t.ny <= NB (the caller checks).  Three passes:
 // all squared distances and the FP32 include decisions; the FP64 decision for
 // the rare voxels within the margin of the cut-off; the weights of the kept
-// voxels in the reference's order (rows outer, columns inner), same
-// arithmetic as patch_walk.  The gather keeps the decisions as per-voxel lane
-// masks, the scatter and the count as one packed bit mask per lane.
+// voxels.  The gather keeps the decisions as per-voxel lane masks and sums in
+// the reference's order (rows outer, columns inner) with the arithmetic of
+// patch_walk; the scatter and the count decide from d2 - thr directly.
 //   KIND 0: returns sum w * src[idx]; 1: dst[idx] += w, amp carrying the
-//   scale (LDS: LDS
-//   atomics in dst's type, else FP32 global atomics); 2: returns the count.
+//   scale (LDS: LDS atomics in dst's type, else FP32 global atomics);
+//   2: returns the count.
 //   LDS: 0 strided image; 1 LDS tile with unit column stride (idx.si = 1);
 //   2 LDS tile with unit row stride (idx.sj = 1).
 template <int KIND, int BX, int BY, int LDS, typename D>
@@ -633,26 +633,26 @@ __device__ __forceinline__ float slice_box(const TorParams &p, const Event &ev, 
     const int jb0 = (int)ceilf((c.dy0 - t.dymax) * p.rfinter_y - eps);
     const int ib = max(i_lo, ib0), ie = min(i_hi, ib0 + t.nx - 1);
     const int jb = max(j_lo, jb0), je = min(j_hi, jb0 + t.ny - 1);
-    float cx2[BX], cux[BX], ry2[BY], ruy[BY];
-#pragma unroll
-    for (int k = 0; k < BX; k++) {
-        float dx = voxel_offset(c.dx0, ib + k, p.finter_x);
-        cx2[k] = dx * dx;
-        cux[k] = ev.ux * dx;
-    }
-#pragma unroll
-    for (int k = 0; k < BY; k++) {
-        float dy = voxel_offset(c.dy0, jb + k, p.finter_y);
-        ry2[k] = dy * dy;
-        ruy[k] = ev.uy * dy;
-    }
     // LDS tiles have unit column (LDS 1) or row (LDS 2) stride: one address
     // per box row (column), formed after the decisions (opaque, so not
     // re-formed per voxel inside the branches), and the other index as an
     // immediate offset
     int row[BY], col[BX];
-    float d2[NC];
     if constexpr (KIND == 0) {
+        float cx2[BX], cux[BX], ry2[BY], ruy[BY];
+#pragma unroll
+        for (int k = 0; k < BX; k++) {
+            float dx = voxel_offset(c.dx0, ib + k, p.finter_x);
+            cx2[k] = dx * dx;
+            cux[k] = ev.ux * dx;
+        }
+#pragma unroll
+        for (int k = 0; k < BY; k++) {
+            float dy = voxel_offset(c.dy0, jb + k, p.finter_y);
+            ry2[k] = dy * dy;
+            ruy[k] = ev.uy * dy;
+        }
+        float d2[NC];
         // Gather: the FP32 decisions stay per-candidate lane masks; only the
         // rare margin case packs them (measured faster than a packed mask
         // for the gather, slower for the scatter).
@@ -708,45 +708,78 @@ __device__ __forceinline__ float slice_box(const TorParams &p, const Event &ev, 
                 acc = fmaf(amp * __builtin_amdgcn_exp2f(d2[k] * ev.cw), v[k], acc);
         return acc;
     } else {
-        // Scatter and count: the decisions packed in a bit mask
-        // candidates inside the clip: columns < ie - ib + 1, rows < je - jb + 1
+        // Scatter and count.  dt = d2 - thr straight from the row and column
+        // terms, the cut-off folded into the row terms; columns and rows
+        // outside the clip get an infinite term, so that their candidates
+        // are neither kept nor near.  The FP32 error of dt is far below the
+        // margin (|D|^2 <= 3 d2 for an event along its dominant axis), so
+        // dt < -margin keeps a voxel, |dt| <= margin leaves it to the FP64
+        // decision and anything else drops it.
         const int nc = ie - ib + 1, nr = je - jb + 1;
-        const uint32_t cols = nc <= 0 ? 0u : (nc >= BX ? (1u << BX) - 1u : (1u << nc) - 1u);
-        uint32_t okm = 0;
+        float cx2[BX], cux[BX], rt2[BY], ruy[BY];
+        {
+            float dx = voxel_offset(c.dx0, ib, p.finter_x);
 #pragma unroll
-        for (int jj = 0; jj < BY; jj++)
-            okm |= jj < nr ? cols << (jj * BX) : 0u;
-        // kept and margin cases as for the gather
-        uint32_t in = 0;
-        float gap = INFINITY; // the smallest |d2 - thr| over the box
+            for (int k = 0; k < BX; k++) {
+                cx2[k] = k < nc ? dx * dx : INFINITY;
+                cux[k] = ev.ux * dx;
+                dx -= p.finter_x;
+            }
+            float dy = voxel_offset(c.dy0, jb, p.finter_y);
+#pragma unroll
+            for (int k = 0; k < BY; k++) {
+                rt2[k] = k < nr ? fmaf(dy, dy, -ev.thr) : INFINITY;
+                ruy[k] = ev.uy * dy;
+                dy -= p.finter_y;
+            }
+        }
+        float dt[NC];
+        float gap = INFINITY; // the smallest |dt| over the box
 #pragma unroll
         for (int jj = 0; jj < BY; jj++) {
 #pragma unroll
             for (int ii = 0; ii < BX; ii++) {
                 const int k = jj * BX + ii;
-                d2[k] = tube_d2(cx2[ii], cux[ii], ry2[jj], ruy[jj]);
-                const float dt = d2[k] - ev.thr; // exact near the cut-off
-                in |= (uint32_t)(dt < -kThrMargin) << k;
-                gap = fminf(gap, fabsf(dt));
+                const float rc = cux[ii] + ruy[jj];
+                dt[k] = fmaf(-rc, rc, cx2[ii] + rt2[jj]);
+                gap = fminf(gap, fabsf(dt[k]));
             }
         }
-        in &= okm;
+        float cnt = 0.0f;
         if (gap <= kThrMargin) {
             // rare: the reference's FP64 decision, one voxel at a time, for
-            // the candidates within the margin of the cut-off
+            // the candidates within the margin of the cut-off; a kept one is
+            // added here (the common pass below only adds dt < -margin)
             uint32_t near = 0;
 #pragma unroll
             for (int k = 0; k < NC; k++)
-                near |= (uint32_t)(!((in >> k) & 1u) & (fabsf(d2[k] - ev.thr) <= kThrMargin)) << k;
-            near &= okm;
+                near |= (uint32_t)(fabsf(dt[k]) <= kThrMargin) << k;
             for (uint32_t m = (uint32_t)opaque((int)near); m; m &= m - 1) {
                 const int k = __builtin_ctz(m);
-                if (inside_exact(p, ev, s, c.ix0 + ib + k % BX, c.iy0 + jb + k / BX))
-                    in |= 1u << k;
+                const int i = ib + k % BX, j = jb + k / BX;
+                if (inside_exact(p, ev, s, c.ix0 + i, c.iy0 + j)) {
+                    if (KIND == 2) {
+                        cnt += 1.0f;
+                    } else {
+                        const float d2 = tube_dist2(voxel_offset(c.dx0, i, p.finter_x),
+                                                    voxel_offset(c.dy0, j, p.finter_y), ev.ux,
+                                                    ev.uy);
+                        const float w = amp * __builtin_amdgcn_exp2f(d2 * ev.cw);
+                        D *q = dst + idx(i, j);
+                        if constexpr (LDS != 0)
+                            atomicAdd(q, (D)w);
+                        else
+                            unsafeAtomicAdd(q, w);
+                    }
+                }
             }
         }
-        if (KIND == 2)
-            return (float)__builtin_popcount(in);
+        if (KIND == 2) {
+#pragma unroll
+            for (int k = 0; k < NC; k++)
+                cnt += dt[k] < -kThrMargin ? 1.0f : 0.0f;
+            return cnt;
+        }
         if constexpr (LDS == 2) {
 #pragma unroll
             for (int ii = 0; ii < BX; ii++)
@@ -756,10 +789,12 @@ __device__ __forceinline__ float slice_box(const TorParams &p, const Event &ev, 
             for (int jj = 0; jj < BY; jj++)
                 row[jj] = LDS ? opaque(idx(ib, jb + jj)) : 0;
         }
+        // d2 cw = dt cw + thr cw
+        const float tcw = ev.thr * ev.cw;
 #pragma unroll
         for (int k = 0; k < NC; k++) {
-            if ((in >> k) & 1u) {
-                float w = amp * __builtin_amdgcn_exp2f(d2[k] * ev.cw);
+            if (dt[k] < -kThrMargin) {
+                float w = amp * __builtin_amdgcn_exp2f(fmaf(dt[k], ev.cw, tcw));
                 D *q = dst + (LDS == 2   ? col[k % BX] + k / BX
                               : LDS == 1 ? row[k / BX] + k % BX
                                          : idx(ib + k % BX, jb + k / BX));
