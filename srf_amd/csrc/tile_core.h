@@ -285,12 +285,12 @@ __device__ __forceinline__ float window_walk(const TorParams &p, const TileGeom 
         int ih = min(i_hi, (int)floorf((ctr + hw) * p.rfinter_x + eps));
         for (int i = il; i <= ih; i++) {
             float dx = voxel_offset(c.dx0, i, p.finter_x);
-            float d2 = tube_dist2(dx, dy, ev.ux, ev.uy);
-            bool in = d2 < ev.thr - kThrMargin;
-            if (!in && d2 < ev.thr + kThrMargin)
+            const float dt = voxel_dt(ev, dx, dy);
+            bool in = dt < -kThrMargin;
+            if (!in && dt <= kThrMargin)
                 in = inside_exact(p, ev, s, c.ix0 + i, c.iy0 + j);
             if (in) {
-                float w = amp * __builtin_amdgcn_exp2f(d2 * ev.cw);
+                const float w = voxel_weight(ev, amp, dt);
                 int li = org + j * sj + i * si;
                 if constexpr (KIND == 0)
                     acc = fmaf(w, lds[li], acc);

@@ -502,28 +502,54 @@ __device__ __forceinline__ float tof_factor(const TorParams &p, const Event &ev,
     return p.tofk * tof_exp(p, ev, t, s);
 }
 
-// Offset of voxel column i (row j) from the crossing point, and the squared
-// distance of a voxel centre from the LOR axis, d2 = |D|^2 - (D.u)^2.  Every
-// walk evaluates them through these helpers with contraction off and the fused
-// steps written out, so the chord walk and the full patch produce the same
-// bits for the same voxel.
+// Offset of voxel column i (row j) from the crossing point, and dt = d2 - thr,
+// d2 = |D|^2 - (D.u)^2 the squared distance of a voxel centre from the LOR
+// axis, with the cut-off folded into the row term.  Every walk evaluates them
+// through these helpers with contraction off and the fused steps written out,
+// so that the box walk, the chord walk and the full patch produce the same
+// bits for the same voxel.  For a voxel near the cut-off the FP32 error of dt
+// is far below kThrMargin: |D|^2 <= 3 d2 for an event along its dominant
+// axis, so every term is at most a few times thr.
 __device__ __forceinline__ float voxel_offset(float o0, int i, float step)
 {
     return fmaf(-(float)i, step, o0);
 }
 
-// From the per-column terms (dx^2, ux dx) and per-row terms (dy^2, uy dy).
-__device__ __forceinline__ float tube_d2(float dx2, float uxdx, float dy2, float uydy)
+// Column terms (dx^2, ux dx) and row terms (dy^2 - thr, uy dy).
+__device__ __forceinline__ void col_terms(float dx, float ux, float &dx2, float &uxdx)
+{
+#pragma clang fp contract(off)
+    dx2 = dx * dx;
+    uxdx = ux * dx;
+}
+
+__device__ __forceinline__ void row_terms(float dy, float uy, float thr, float &dyt, float &uydy)
+{
+#pragma clang fp contract(off)
+    dyt = fmaf(dy, dy, -thr);
+    uydy = uy * dy;
+}
+
+__device__ __forceinline__ float tube_dt(float dx2, float uxdx, float dyt, float uydy)
 {
 #pragma clang fp contract(off)
     float rc = uxdx + uydy;
-    return fmaf(-rc, rc, dx2 + dy2);
+    return fmaf(-rc, rc, dx2 + dyt);
 }
 
-__device__ __forceinline__ float tube_dist2(float dx, float dy, float ux, float uy)
+__device__ __forceinline__ float voxel_dt(const Event &ev, float dx, float dy)
 {
-#pragma clang fp contract(off)
-    return tube_d2(dx * dx, ux * dx, dy * dy, uy * dy);
+    float dx2, uxdx, dyt, uydy;
+    col_terms(dx, ev.ux, dx2, uxdx);
+    row_terms(dy, ev.uy, ev.thr, dyt, uydy);
+    return tube_dt(dx2, uxdx, dyt, uydy);
+}
+
+// Weight of a kept voxel: amp exp(-d2 / (2 sigma^2)), the exponent as
+// dt cw + thr cw.
+__device__ __forceinline__ float voxel_weight(const Event &ev, float amp, float dt)
+{
+    return amp * __builtin_amdgcn_exp2f(fmaf(dt, ev.cw, ev.thr * ev.cw));
 }
 
 // One voxel: exact include test, then gather / scatter / count.
@@ -531,15 +557,15 @@ template <int KIND>
 __device__ __forceinline__ void visit(const TorParams &p, const Event &ev, int s, int ix, int iy,
                                       float dx, float dy, float amp, float scale, float &acc)
 {
-    float d2 = tube_dist2(dx, dy, ev.ux, ev.uy);
-    bool in = d2 < ev.thr - kThrMargin;
-    if (!in && d2 < ev.thr + kThrMargin)
+    const float dt = voxel_dt(ev, dx, dy);
+    bool in = dt < -kThrMargin;
+    if (!in && dt <= kThrMargin)
         in = inside_exact(p, ev, s, ix, iy);
     if (in) {
         if (KIND == 2) {
             acc += 1.0f;
         } else {
-            float w = amp * __builtin_amdgcn_exp2f(d2 * ev.cw);
+            float w = voxel_weight(ev, amp, dt);
             if (KIND == 0)
                 acc = fmaf(w, p.img[ix * p.sx + iy * p.sy + s * p.sz], acc);
             else
@@ -638,47 +664,44 @@ __device__ __forceinline__ float slice_box(const TorParams &p, const Event &ev, 
     // re-formed per voxel inside the branches), and the other index as an
     // immediate offset
     int row[BY], col[BX];
+    // dt = d2 - thr of every candidate from its column and row terms; the
+    // smallest |dt| decides whether the FP64 decision is needed at all
+    float cx2[BX], cux[BX], rt2[BY], ruy[BY];
     if constexpr (KIND == 0) {
-        float cx2[BX], cux[BX], ry2[BY], ruy[BY];
+        // offsets as the patch walks form them (same bits)
 #pragma unroll
-        for (int k = 0; k < BX; k++) {
-            float dx = voxel_offset(c.dx0, ib + k, p.finter_x);
-            cx2[k] = dx * dx;
-            cux[k] = ev.ux * dx;
-        }
+        for (int k = 0; k < BX; k++)
+            col_terms(voxel_offset(c.dx0, ib + k, p.finter_x), ev.ux, cx2[k], cux[k]);
 #pragma unroll
-        for (int k = 0; k < BY; k++) {
-            float dy = voxel_offset(c.dy0, jb + k, p.finter_y);
-            ry2[k] = dy * dy;
-            ruy[k] = ev.uy * dy;
-        }
-        float d2[NC];
-        // Gather: the FP32 decisions stay per-candidate lane masks; only the
-        // rare margin case packs them (measured faster than a packed mask
-        // for the gather, slower for the scatter).
-        // Kept: d2 - thr < -margin; within |d2 - thr| <= margin the FP64
-        // decision (d2 - thr is exact there, so the two cases cover every d2)
-        bool in[NC];
-        float gap = INFINITY; // the smallest |d2 - thr| over the box
+        for (int k = 0; k < BY; k++)
+            row_terms(voxel_offset(c.dy0, jb + k, p.finter_y), ev.uy, ev.thr, rt2[k], ruy[k]);
+        // Gather.  Kept: dt < -margin; within |dt| <= margin the FP64
+        // decision.  The decisions stay per-candidate lane masks (measured
+        // faster than a packed mask for the gather); the kept voxels are
+        // summed in the reference's order (rows outer, columns inner), all
+        // loads first.
+        float dt[NC];
+        float gap = INFINITY; // the smallest |dt| over the box
 #pragma unroll
         for (int jj = 0; jj < BY; jj++) {
 #pragma unroll
             for (int ii = 0; ii < BX; ii++) {
                 const int k = jj * BX + ii;
-                const bool ok = (ib + ii <= ie) & (jb + jj <= je);
-                d2[k] = tube_d2(cx2[ii], cux[ii], ry2[jj], ruy[jj]);
-                const float dt = d2[k] - ev.thr; // exact near the cut-off
-                in[k] = ok & (dt < -kThrMargin);
-                gap = fminf(gap, fabsf(dt));
+                dt[k] = tube_dt(cx2[ii], cux[ii], rt2[jj], ruy[jj]);
+                gap = fminf(gap, fabsf(dt[k]));
             }
         }
+        bool in[NC];
+#pragma unroll
+        for (int k = 0; k < NC; k++)
+            in[k] = (ib + k % BX <= ie) & (jb + k / BX <= je) & (dt[k] < -kThrMargin);
         if (gap <= kThrMargin) {
             // rare: the reference's FP64 decision, one voxel at a time
             uint32_t near = 0, add = 0;
 #pragma unroll
             for (int k = 0; k < NC; k++) {
                 const bool ok = (ib + k % BX <= ie) & (jb + k / BX <= je);
-                near |= (uint32_t)(ok & !in[k] & (fabsf(d2[k] - ev.thr) <= kThrMargin)) << k;
+                near |= (uint32_t)(ok & (fabsf(dt[k]) <= kThrMargin)) << k;
             }
             for (uint32_t m = (uint32_t)opaque((int)near); m; m &= m - 1) {
                 const int k = __builtin_ctz(m);
@@ -693,8 +716,6 @@ __device__ __forceinline__ float slice_box(const TorParams &p, const Event &ev, 
 #pragma unroll
         for (int jj = 0; jj < BY; jj++)
             row[jj] = LDS ? opaque(idx(ib, jb + jj)) : 0;
-        // the kept voxels in the reference's order (rows outer, columns
-        // inner), all loads issued first
         float v[NC];
 #pragma unroll
         for (int k = 0; k < NC; k++) {
@@ -705,31 +726,29 @@ __device__ __forceinline__ float slice_box(const TorParams &p, const Event &ev, 
 #pragma unroll
         for (int k = 0; k < NC; k++)
             if (in[k])
-                acc = fmaf(amp * __builtin_amdgcn_exp2f(d2[k] * ev.cw), v[k], acc);
+                acc = fmaf(voxel_weight(ev, amp, dt[k]), v[k], acc);
         return acc;
     } else {
-        // Scatter and count.  dt = d2 - thr straight from the row and column
-        // terms, the cut-off folded into the row terms; columns and rows
-        // outside the clip get an infinite term, so that their candidates
-        // are neither kept nor near.  The FP32 error of dt is far below the
-        // margin (|D|^2 <= 3 d2 for an event along its dominant axis), so
-        // dt < -margin keeps a voxel, |dt| <= margin leaves it to the FP64
-        // decision and anything else drops it.
+        // Scatter and count: no packed decisions.  Columns and rows outside
+        // the clip get an infinite term, so that their candidates are
+        // neither kept nor near; dt < -margin keeps a voxel, |dt| <= margin
+        // leaves it to the FP64 decision (which adds a kept one itself) and
+        // anything else drops it.
         const int nc = ie - ib + 1, nr = je - jb + 1;
-        float cx2[BX], cux[BX], rt2[BY], ruy[BY];
         {
+            // offsets by repeated steps from the first column / row
             float dx = voxel_offset(c.dx0, ib, p.finter_x);
 #pragma unroll
             for (int k = 0; k < BX; k++) {
-                cx2[k] = k < nc ? dx * dx : INFINITY;
-                cux[k] = ev.ux * dx;
+                col_terms(dx, ev.ux, cx2[k], cux[k]);
+                cx2[k] = k < nc ? cx2[k] : INFINITY;
                 dx -= p.finter_x;
             }
             float dy = voxel_offset(c.dy0, jb, p.finter_y);
 #pragma unroll
             for (int k = 0; k < BY; k++) {
-                rt2[k] = k < nr ? fmaf(dy, dy, -ev.thr) : INFINITY;
-                ruy[k] = ev.uy * dy;
+                row_terms(dy, ev.uy, ev.thr, rt2[k], ruy[k]);
+                rt2[k] = k < nr ? rt2[k] : INFINITY;
                 dy -= p.finter_y;
             }
         }
@@ -740,8 +759,7 @@ __device__ __forceinline__ float slice_box(const TorParams &p, const Event &ev, 
 #pragma unroll
             for (int ii = 0; ii < BX; ii++) {
                 const int k = jj * BX + ii;
-                const float rc = cux[ii] + ruy[jj];
-                dt[k] = fmaf(-rc, rc, cx2[ii] + rt2[jj]);
+                dt[k] = tube_dt(cx2[ii], cux[ii], rt2[jj], ruy[jj]);
                 gap = fminf(gap, fabsf(dt[k]));
             }
         }
@@ -761,10 +779,9 @@ __device__ __forceinline__ float slice_box(const TorParams &p, const Event &ev, 
                     if (KIND == 2) {
                         cnt += 1.0f;
                     } else {
-                        const float d2 = tube_dist2(voxel_offset(c.dx0, i, p.finter_x),
-                                                    voxel_offset(c.dy0, j, p.finter_y), ev.ux,
-                                                    ev.uy);
-                        const float w = amp * __builtin_amdgcn_exp2f(d2 * ev.cw);
+                        const float w = voxel_weight(
+                            ev, amp, voxel_dt(ev, voxel_offset(c.dx0, i, p.finter_x),
+                                              voxel_offset(c.dy0, j, p.finter_y)));
                         D *q = dst + idx(i, j);
                         if constexpr (LDS != 0)
                             atomicAdd(q, (D)w);
@@ -789,12 +806,10 @@ __device__ __forceinline__ float slice_box(const TorParams &p, const Event &ev, 
             for (int jj = 0; jj < BY; jj++)
                 row[jj] = LDS ? opaque(idx(ib, jb + jj)) : 0;
         }
-        // d2 cw = dt cw + thr cw
-        const float tcw = ev.thr * ev.cw;
 #pragma unroll
         for (int k = 0; k < NC; k++) {
             if (dt[k] < -kThrMargin) {
-                float w = amp * __builtin_amdgcn_exp2f(fmaf(dt[k], ev.cw, tcw));
+                const float w = voxel_weight(ev, amp, dt[k]);
                 D *q = dst + (LDS == 2   ? col[k % BX] + k / BX
                               : LDS == 1 ? row[k / BX] + k % BX
                                          : idx(ib + k % BX, jb + k / BX));
