@@ -51,6 +51,7 @@ struct TorParams {
     double rinter_x, rinter_y, rinter_z;
     double l_bound, b_bound;
     double dx_half, dy_half; // (half - 0.5) * inter_x, (half - 0.5) * inter_y
+    float fdx_half, fdy_half; // the same in FP32
     double zbase; // center_z - (lz - inter_z)/2: slice s is at zbase + s*inter_z
     double tor_sigma2;
     float finter_x, finter_y;
@@ -436,13 +437,15 @@ __device__ __forceinline__ void cross_at(const TorParams &p, const Event &ev, co
     double u = t.ax + (double)s * t.bx;
     double w = t.ay + (double)s * t.by;
     // trunc as the reference's (int) cast; the fraction u - trunc(u) lies in
-    // (-1, 1) and is within 1e-9 of 0 or +-1 exactly when u is within 1e-9 of
-    // an integer
+    // (-1, 1), here rounded to FP32 (relative error 2^-24).  u near an
+    // integer means a fraction near 0 or +-1: every |fraction| below 1e-9
+    // or above 1 - 1e-9 is caught as |fu| < 1e-9 or |fu| >= 1 - 2^-24 (a
+    // fraction above 1 - 1e-9 rounds to 1), far wider than the FP64 error of
+    // u (~1e-13 voxel).  A non-short-circuit or, so one branch.
     const int iu = (int)u, iw = (int)w;
-    const double fu = u - (double)iu, fw = w - (double)iw;
-    // |fu - rint(fu)| is the distance of u from the nearest integer (fu lies
-    // in (-1, 1)); a non-short-circuit or, so one branch
-    if ((int)(fabs(fu - rint(fu)) < 1.0e-9) | (int)(fabs(fw - rint(fw)) < 1.0e-9)) {
+    const float fu = (float)(u - (double)iu), fw = (float)(w - (double)iw);
+    const float alo = fminf(fabsf(fu), fabsf(fw)), ahi = fmaxf(fabsf(fu), fabsf(fw));
+    if ((int)(alo < 1.0e-9f) | (int)(ahi >= 1.0f - 0x1p-24f)) {
         double cx, cy;
         cross_point_exact(p, exact_geom(p, ev), s, cx, cy);
         c.ix0 = (int)__ddiv_rn(__dsub_rn(cx, p.l_bound), p.inter_x) - p.half;
@@ -452,9 +455,11 @@ __device__ __forceinline__ void cross_at(const TorParams &p, const Event &ev, co
     } else {
         c.ix0 = iu - p.half;
         c.iy0 = iw - p.half;
-        // (u - (ix0 + 0.5)) * inter = fu * inter + (half - 0.5) * inter
-        c.dx0 = (float)fma(fu, p.inter_x, p.dx_half);
-        c.dy0 = (float)fma(fw, p.inter_y, p.dy_half);
+        // (u - (ix0 + 0.5)) * inter = fu * inter + (half - 0.5) * inter, in
+        // FP32 from the rounded fraction: error ~3e-7 mm, whose effect on
+        // d2 stays far inside kThrMargin
+        c.dx0 = fmaf(fu, p.finter_x, p.fdx_half);
+        c.dy0 = fmaf(fw, p.finter_y, p.fdy_half);
     }
 }
 
@@ -1096,6 +1101,8 @@ int setup(TorParams &p, const float *lors, int64_t n, int64_t ld, const srfx_gri
     p.half = (int)((unsigned)p.patch / 2);
     p.dx_half = ((double)p.half - 0.5) * p.inter_x;
     p.dy_half = ((double)p.half - 0.5) * p.inter_y;
+    p.fdx_half = (float)p.dx_half;
+    p.fdy_half = (float)p.dy_half;
     p.geo = geo ? 1 : 0;
     p.use_tof = tof_sigma2 < kTofThreshold ? 1 : 0;
     if (p.use_tof) {
