@@ -250,7 +250,10 @@ __device__ __forceinline__ ExactGeom exact_geom(const TorParams &p, const RawEve
 
 struct Track {
     double ax, bx, ay, by;
-    float v0x, v0y, v0z, dvx, dvy, dvz;
+    // TOF exponent of slice s (log2 domain): tc0 + s (tc1 + s tc2), the
+    // squared distance from the TOF centre to the crossing point times ctof
+    // expanded in s (FP64, so the expansion loses nothing)
+    double tc0, tc1, tc2;
     // in-slice ellipse d2 < tm: d2 = a (dx - cxy dy)^2 + q dy^2
     float A;     // 1 / (1 - ux^2)
     float cxy;   // ux uy / (1 - ux^2)
@@ -275,13 +278,14 @@ __device__ __forceinline__ void make_track(const TorParams &p, const Event &ev,
     t.bx = g.d0 * rs * p.rinter_x;
     t.ay = (g.y1 + g.d1 * r0 - p.b_bound) * p.rinter_y;
     t.by = g.d1 * rs * p.rinter_y;
-    double rl = r0 + (double)ev.s_lo * rs;
-    t.v0x = (float)((double)r.xc - (g.x1 + g.d0 * rl));
-    t.v0y = (float)((double)r.yc - (g.y1 + g.d1 * rl));
-    t.v0z = (float)((double)r.zc - slice_z(p, ev.s_lo));
-    t.dvx = (float)(g.d0 * rs);
-    t.dvy = (float)(g.d1 * rs);
-    t.dvz = (float)p.inter_z;
+    // TOF centre minus the crossing point of slice 0, and its step per slice
+    const double vx = (double)r.xc - (g.x1 + g.d0 * r0), vy = (double)r.yc - (g.y1 + g.d1 * r0);
+    const double vz = (double)r.zc - slice_z(p, 0);
+    const double dx = g.d0 * rs, dy = g.d1 * rs, dz = p.inter_z;
+    const double ct = (double)ev.ctof;
+    t.tc0 = ct * (vx * vx + vy * vy + vz * vz);
+    t.tc1 = -2.0 * ct * (vx * dx + vy * dy + vz * dz);
+    t.tc2 = ct * (dx * dx + dy * dy + dz * dz);
     // a = 1 - ux^2 = (d1^2 + d2^2) / |d|^2, formed without cancellation
     // (d2 != 0 for an event with crossed slices)
     const double den = g.d1 * g.d1 + g.d2 * g.d2;
@@ -329,7 +333,7 @@ namespace {
 // event's tile listings instead of recomputing it.  A record array of n
 // events holds n heads of 8 float4 (128 bytes, one cache line: all that the
 // common path of a walk reads) and then n tails of 1 float4:
-//   head [ax bx] [ay by] (FP64 pairs) [v0x v0y v0z dvx] [dvy dvz dymax hx]
+//   head [ax bx] [ay by] [tc0 tc1] (FP64 pairs) [tc2 (FP64) dymax hx]
 //        [ux uy thr cw] [ctof s_lo s_hi nx|ny<<16] [A cxy q tm] [x1 y1 z1 value]
 //   tail [x2 y2 z2 f]
 // value is the event's projection in a plan's OSEM step.  An event without a
@@ -356,8 +360,10 @@ __device__ __forceinline__ void pack_walk(const TorParams &p, const RawEvent &r,
     float4 *w = rec + kWalkHead * e;
     w[0] = dpair(t.ax, t.bx);
     w[1] = dpair(t.ay, t.by);
-    w[2] = make_float4(t.v0x, t.v0y, t.v0z, t.dvx);
-    w[3] = make_float4(t.dvy, t.dvz, t.dymax, t.hx);
+    w[2] = dpair(t.tc0, t.tc1);
+    w[3] = dpair(t.tc2, 0.0);
+    w[3].z = t.dymax;
+    w[3].w = t.hx;
     w[4] = make_float4(ev.ux, ev.uy, ev.thr, ev.cw);
     w[5] = make_float4(ev.ctof, __int_as_float(valid ? ev.s_lo : 1),
                        __int_as_float(valid ? ev.s_hi : 0),
@@ -377,12 +383,8 @@ __device__ __forceinline__ float unpack_walk(const TorParams &p, const float4 *_
     const float4 a = w[0], b = w[1], c = w[2], d = w[3], f = w[4], g = w[5], h = w[6], k = w[7];
     undpair(a, t.ax, t.bx);
     undpair(b, t.ay, t.by);
-    t.v0x = c.x;
-    t.v0y = c.y;
-    t.v0z = c.z;
-    t.dvx = c.w;
-    t.dvy = d.x;
-    t.dvz = d.y;
+    undpair(c, t.tc0, t.tc1);
+    t.tc2 = __hiloint2double(__float_as_int(d.y), __float_as_int(d.x));
     t.dymax = d.z;
     t.hx = d.w;
     ev.ux = f.x;
@@ -491,14 +493,13 @@ __device__ __forceinline__ bool inside_exact(const TorParams &p, const Event &ev
 // Per-slice TOF factor tof_bin/sqrt(2 pi s2) * exp(-D/(2 tse)), D the squared
 // distance from the TOF centre to the crossing point; the reference's
 // d2_tof = D - d2 (tor.cu.cc:87) has its -d2 part folded into the per-voxel
-// exponent.
+// exponent.  D / (2 tse) in the log2 domain is the track's quadratic in s,
+// evaluated in FP64 and rounded once.
 __device__ __forceinline__ float tof_exp(const TorParams &p, const Event &ev, const Track &t,
                                          int s)
 {
-    float k = (float)(s - ev.s_lo);
-    float vx = t.v0x - k * t.dvx, vy = t.v0y - k * t.dvy, vz = t.v0z - k * t.dvz;
-    float D = vx * vx + vy * vy + vz * vz;
-    return __builtin_amdgcn_exp2f(D * ev.ctof);
+    const double sd = (double)s;
+    return __builtin_amdgcn_exp2f((float)fma(fma(t.tc2, sd, t.tc1), sd, t.tc0));
 }
 
 __device__ __forceinline__ float tof_factor(const TorParams &p, const Event &ev, const Track &t,
