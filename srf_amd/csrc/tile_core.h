@@ -384,7 +384,10 @@ __device__ __forceinline__ float tile_event(const TorParams &p, const TileGeom &
 // kernel of a plan's OSEM step): LPE lanes per event as tor_kernel<kProj>
 // runs them (the same walk, so the same bits), with the event set-up
 // unpacked from the record head instead of formed again from the LOR row by
-// each of the event's lanes; the value goes into the head's value slot.
+// each of the event's lanes.  The head's value slot receives the scale of
+// the event's backprojection: 1/p for a projection p > 1e-7, else 0 (the
+// event adds nothing, as the reference's ratio skips it), so that each of the
+// event's listings reads it instead of dividing again.
 template <int NB, int LPE>
 __device__ __forceinline__ void record_project(const TorParams &p, float4 *rec, int64_t e,
                                                int sub, bool valid)
@@ -399,7 +402,8 @@ __device__ __forceinline__ void record_project(const TorParams &p, float4 *rec, 
     }
     const float pv = group_sum<LPE>(part);
     if (valid && sub == 0)
-        reinterpret_cast<float *>(rec)[4 * kWalkHead * e + kWalkValue] = pv;
+        reinterpret_cast<float *>(rec)[4 * kWalkHead * e + kWalkValue] =
+            pv > 1e-7 ? 1.0f / pv : 0.0f;
 }
 
 template <int NB, int LPE>
@@ -414,8 +418,8 @@ __global__ __launch_bounds__(256) void record_proj_kernel(TorParams p, float4 *r
 // Scatter of one chunk (modes kBack, kMap, kAdj) into p.out, through the
 // FP64 LDS tile ldsd (S * Wy * Wx doubles).  Every thread of the workgroup
 // calls it; it synchronises the workgroup before and after the flush.
-// VREC: the values are the projections stored in the event records (the OSEM
-// step of a plan), not p.val.
+// VREC (kBack only): the backprojection scales 1/p that the record projection
+// stored in the event records (the OSEM step of a plan), not p.val.
 template <int MODE, int NB, bool VREC = false, bool YF = false>
 __device__ __forceinline__ void scatter_chunk(const TorParams &p, const TileGeom &g,
                                               const TileChunk &ch,
@@ -444,7 +448,11 @@ __device__ __forceinline__ void scatter_chunk(const TorParams &p, const TileGeom
         const float val = VREC ? rv : p.val[e];
         float scale;
         bool go = true;
-        if (MODE == kBack) {
+        static_assert(!VREC || MODE == kBack, "records hold backprojection scales");
+        if (VREC) {
+            scale = rv;
+            go = rv != 0.0f;
+        } else if (MODE == kBack) {
             go = val > 1e-7;
             scale = 1.0f / val;
         } else if (MODE == kMap) {

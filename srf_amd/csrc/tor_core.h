@@ -336,8 +336,9 @@ namespace {
 //   head [ax bx] [ay by] [tc0 tc1] (FP64 pairs) [tc2 (FP64) dymax hx]
 //        [ux uy thr cw] [ctof s_lo s_hi nx|ny<<16] [A cxy q tm] [x1 y1 z1 value]
 //   tail [x2 y2 z2 f]
-// value is the event's projection in a plan's OSEM step.  An event without a
-// crossed slice (or otherwise invalid) has s_lo = 1, s_hi = 0.
+// value is the scale of the event's backprojection in a plan's OSEM step
+// (1/p).  An event without a crossed slice (or otherwise invalid) has
+// s_lo = 1, s_hi = 0.
 constexpr int kWalkF4 = 9;    // float4 per event
 constexpr int kWalkHead = 8;  // float4 per head
 constexpr int kWalkValue = 31; // float index of the value in a head
