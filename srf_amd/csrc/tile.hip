@@ -215,13 +215,17 @@ int plan_build(srfx_tor_plan_s *pl, hipStream_t st, int chunk, bool keep_slots)
         goto done;
     }
     // without the tiled projection's slot tables the listings of a chunk may
-    // be reordered: longest walks first
+    // be reordered: longest walks first, each entry with its listing's slices
+    // where the event index and the slab fit the entry's fields
     if (!keep_slots && pl->nchunks > 0 && chunk <= 16 * 256) {
+        pl->g.spans = pl->g.S <= kSpanMaxSlab && pl->p.n < ((int64_t)1 << kSpanEventBits);
         hipLaunchKernelGGL(sort_chunks, dim3(pl->nchunks), dim3(256),
                            sizeof(int) * (33 + chunk), st, pl->p, pl->g, pl->chunks, pl->list,
                            pl->rec);
-        if ((rc = launch_status()))
+        if ((rc = launch_status())) {
+            pl->g.spans = 0;
             goto done;
+        }
         pl->g.sorted = 1;
     }
 done:

@@ -34,7 +34,15 @@ struct TileGeom {
     int S, Wx, Wy;       // slices per slab, window width along op x and y (voxels)
     int nwx, nwy, nslab; // windows per axis and number of slabs
     int sorted;          // chunks' listings ordered by slice count (sort_chunks)
+    int spans;           // sorted list entries carry their slices (listing_entry)
 };
+
+// A list entry of a plan with spans: the event index in bits 0..23 and the
+// listing's first and last slice in the tile's slab, relative to the slab's
+// first slice, in bits 24..27 and 28..31 (slabs of at most 16 slices, plans
+// of fewer than 2^24 events).  An empty span is stored as (1, 0).
+constexpr int kSpanEventBits = 24;
+constexpr int kSpanMaxSlab = 16;
 
 
 struct TileChunk {
@@ -334,19 +342,27 @@ __device__ __forceinline__ void window_slices(const Track &t, int half, int P, i
     axis_slices(t.ay, t.by, (double)(y0 - P + half - 2), (double)(y1 + half + 2), s0, s1);
 }
 
-// One listing: the event's slices inside the tile, each through the NB x NB
-// candidate box clipped to the window when the event's box fits, else the
-// clipped chord walk.
+// The slices s0..s1 a listing walks: the event's crossed slices in the
+// tile's slab, narrowed to those whose patch can meet the window.
+__device__ __forceinline__ void listing_slices(const TorParams &p, const TileBox &b,
+                                               const Event &ev, const Track &t, int &s0, int &s1)
+{
+    s0 = max(ev.s_lo, b.sa);
+    s1 = min(ev.s_hi, b.sb);
+    window_slices(t, p.half, p.patch, b.x0, b.x1, b.y0, b.y1, s0, s1);
+}
+
+// One listing: its slices s0..s1 inside the tile (listing_slices), each
+// through the NB x NB candidate box clipped to the window when the event's
+// box fits, else the clipped chord walk.
 template <int KIND, int NB, bool YF, typename T>
 __device__ __forceinline__ float tile_event(const TorParams &p, const TileGeom &g,
                                             const TileBox &b, const Event &ev, const Track &t,
-                                            bool tof, float scale, T *lds)
+                                            bool tof, float scale, int s0, int s1, T *lds)
 {
     constexpr int NBB = NB > 1 ? NB : 2;
     const int shape = box_shape<NB>(t);
-    int s0 = max(ev.s_lo, b.sa), s1 = min(ev.s_hi, b.sb);
     const int P = p.patch;
-    window_slices(t, p.half, P, b.x0, b.x1, b.y0, b.y1, s0, s1);
     const float ts = p.tofk * scale;
     float acc = 0.0f;
     for (int s = s0; s <= s1; s++) {
@@ -441,10 +457,18 @@ __device__ __forceinline__ void scatter_chunk(const TorParams &p, const TileGeom
         const int j = g.sorted ? q : (q & 63) * stride + (q >> 6);
         if (j >= ch.count)
             continue;
-        const int64_t e = list[ch.start + j];
+        const int ent = list[ch.start + j];
+        const int64_t e = g.spans ? (ent & ((1 << kSpanEventBits) - 1)) : ent;
         Event ev;
         Track t;
         const float rv = unpack_walk(p, rec, e, ev, t);
+        int s0, s1;
+        if (g.spans) {
+            s0 = b.sa + ((ent >> kSpanEventBits) & 15);
+            s1 = b.sa + (int)((unsigned)ent >> (kSpanEventBits + 4));
+        } else {
+            listing_slices(p, b, ev, t, s0, s1);
+        }
         const float val = VREC ? rv : p.val[e];
         float scale;
         bool go = true;
@@ -462,7 +486,7 @@ __device__ __forceinline__ void scatter_chunk(const TorParams &p, const TileGeom
             go = scale != 0.0f;
         }
         if (go)
-            tile_event<1, NB, YF>(p, g, b, ev, t, tof, scale, ldsd);
+            tile_event<1, NB, YF>(p, g, b, ev, t, tof, scale, s0, s1, ldsd);
     }
     __syncthreads();
     // Flush.  Tile element i = (c * nb + m) * na + f (f along the fastest
@@ -506,9 +530,10 @@ __device__ __forceinline__ void scatter_chunk(const TorParams &p, const TileGeom
 // Plan build, after the fill pass: order each chunk's listings by the number
 // of slices they walk in the tile, longest first (a counting sort in LDS,
 // stable), so that the consecutive listings a wave of the scatter takes walk
-// about as many slices each (the order inside a bucket is the atomics').  One
-// workgroup per chunk; LDS: 33 counters and `chunk` ints.  Needs ch.count <=
-// 16 * blockDim.x.
+// about as many slices each (the order inside a bucket is the atomics').
+// With g.spans each entry also takes its listing's slices.  One workgroup
+// per chunk; LDS: 33 counters and `chunk` ints.  Needs ch.count <= 16 *
+// blockDim.x.
 __global__ __launch_bounds__(256) void sort_chunks(TorParams p, TileGeom g,
                                                    const TileChunk *__restrict__ chunks,
                                                    int *__restrict__ list,
@@ -533,12 +558,15 @@ __global__ __launch_bounds__(256) void sort_chunks(TorParams p, TileGeom g,
             Track t;
             unpack_walk(p, rec, ev[r], e, t);
             int s0, s1;
-            s0 = max(e.s_lo, b.sa);
-            s1 = min(e.s_hi, b.sb);
-            window_slices(t, p.half, p.patch, b.x0, b.x1, b.y0, b.y1, s0, s1);
+            listing_slices(p, b, e, t, s0, s1);
             // longest first: bucket 0 holds the longest walks
             key[r] = kBuckets - 1 - min(max(s1 - s0 + 1, 0), kBuckets - 1);
             atomicAdd(&cnt[key[r]], 1);
+            if (g.spans) {
+                // the span rides in the entry (the host checked the limits)
+                const int a = s0 <= s1 ? s0 - b.sa : 1, z = s0 <= s1 ? s1 - b.sa : 0;
+                ev[r] |= (a << kSpanEventBits) | (z << (kSpanEventBits + 4));
+            }
         }
     }
     __syncthreads();
@@ -600,7 +628,9 @@ __global__ __launch_bounds__(256) void tile_gather(TorParams p, TileGeom g,
         Event ev;
         Track t;
         unpack_walk(p, rec, e, ev, t);
-        const float acc = tile_event<0, NB, false>(p, g, b, ev, t, tof, 0.0f, lds);
+        int s0, s1;
+        listing_slices(p, b, ev, t, s0, s1);
+        const float acc = tile_event<0, NB, false>(p, g, b, ev, t, tof, 0.0f, s0, s1, lds);
         partial[ch.start + j] = acc;
     }
 }
