@@ -174,9 +174,17 @@ int plan_build(srfx_tor_plan_s *pl, hipStream_t st, int chunk, bool keep_slots)
         goto done;
     }
     total32 = (int)total;
-    for (int t = 0; t < ntiles; t++)
-        for (int s = 0; s < cnt[t]; s += chunk)
-            ch.push_back(TileChunk{t, off[t] + s, cnt[t] - s < chunk ? cnt[t] - s : chunk, 0});
+    // chunks window by window, each window's slabs in order: the listings of
+    // one event in consecutive slabs mostly share a window, so its record is
+    // read again while the L2 still holds it
+    {
+        const int nw = g.nwx * g.nwy;
+        for (int w = 0; w < nw; w++)
+            for (int t = w; t < ntiles; t += nw)
+                for (int s = 0; s < cnt[t]; s += chunk)
+                    ch.push_back(
+                        TileChunk{t, off[t] + s, cnt[t] - s < chunk ? cnt[t] - s : chunk, 0});
+    }
     pl->nlist = total;
     pl->nchunks = (int)ch.size();
     if ((rc = dev_alloc(&pl->list, total)) || (rc = dev_alloc(&pl->chunks, ch.size())))
