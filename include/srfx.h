@@ -56,7 +56,9 @@ enum srfx_status {
  * slower; kept for verification. */
 #define SRFX_FULL_PATCH 2u
 /* TOR ops: launch schedule.  Lanes that share one event (its crossed slices
- * interleaved between them): SRFX_LPE_1 (default), SRFX_LPE_4, SRFX_LPE_16.
+ * interleaved between them): SRFX_LPE_1, SRFX_LPE_4, SRFX_LPE_16; without a
+ * flag one lane per event, except the projection of a plan's OSEM step
+ * (srfx_tor_plan_osem_subset), which takes four.
  * Threads per block: SRFX_BLOCK_256 (default), SRFX_BLOCK_64, SRFX_BLOCK_128.
  * Results agree to FP32 rounding across schedules. */
 #define SRFX_LPE_SHIFT 4

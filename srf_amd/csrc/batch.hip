@@ -118,18 +118,31 @@ __global__ __launch_bounds__(256) void batch_project(const BatchJob *__restrict_
         const EvChunk ch = chunks[c];
         const BatchJob &job = jobs[__builtin_amdgcn_readfirstlane(ch.job)];
         float4 *rec = const_cast<float4 *>(job.rec);
-        const bool valid = (int)threadIdx.x < ch.count;
-        const int64_t e = (int64_t)ch.start + threadIdx.x;
-        switch (job.p.box) {
-        case 3:
-            record_project<3, 1>(job.p, rec, e, 0, valid);
-            break;
-        case 4:
-            record_project<4, 1>(job.p, rec, e, 0, valid);
-            break;
-        default:
-            record_project<0, 1>(job.p, rec, e, 0, valid);
-            break;
+        // four lanes per event (the projection schedule of the plan OSEM
+        // step), the chunk's events in four rounds of blockDim / 4
+        const bool frame = job.p.img_bytes > 0; // else too large for 32-bit offsets
+        const int sub = (int)threadIdx.x & 3;
+        for (int r = 0; r < kChunk; r += (int)blockDim.x / 4) {
+            const int i = r + (int)threadIdx.x / 4;
+            const bool valid = i < ch.count;
+            const int64_t e = (int64_t)ch.start + i;
+            switch (job.p.box) {
+            case 3:
+                if (frame)
+                    record_project<3, 4, true>(job.p, rec, e, sub, valid);
+                else
+                    record_project<3, 4, false>(job.p, rec, e, sub, valid);
+                break;
+            case 4:
+                if (frame)
+                    record_project<4, 4, true>(job.p, rec, e, sub, valid);
+                else
+                    record_project<4, 4, false>(job.p, rec, e, sub, valid);
+                break;
+            default:
+                record_project<0, 4>(job.p, rec, e, sub, valid);
+                break;
+            }
         }
     }
 }
@@ -355,6 +368,10 @@ extern "C" int srfx_tor_batch_create(const srfx_recon *recons, int32_t nrecon,
                 job.p.sx = 1;
                 job.p.sy = gg.shape[0];
                 job.p.sz = gg.shape[0] * gg.shape[1];
+                {
+                    const int64_t b = (int64_t)4 * job.p.sz * gg.shape[2];
+                    job.p.img_bytes = b < ((int64_t)1 << 31) ? (int)b : 0;
+                }
                 job.p.out = rc.bp;
                 job.p.val = plan->proj;
                 job.g = plan->g;

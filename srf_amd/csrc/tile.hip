@@ -65,11 +65,25 @@ void launch_pass(const TorParams &p, const TileGeom &g, const PlanArgs &a, hipSt
 }
 
 // Count, scan (host: per tile; device: per event when slots are kept), fill.
+// A read image in contiguous op-frame layout of fewer than 2^31 bytes goes
+// through the frame gathers (buffer loads with 32-bit offsets).
+bool frame_image(const TorParams &p, int &bytes)
+{
+    const int64_t b = (int64_t)4 * p.gx * p.gy * p.gz;
+    bytes = (int)b;
+    return p.sx == 1 && p.sy == p.gx && p.sz == p.gx * p.gy && b > 0 && b < ((int64_t)1 << 31);
+}
+
 template <int NB, int LPE>
 int launch_record_pg(const TorParams &p, float4 *rec, int block, hipStream_t st)
 {
-    hipLaunchKernelGGL((record_proj_kernel<NB, LPE>), dim3(blocks_for(p.n * LPE, block)),
-                       dim3(block), 0, st, p, rec);
+    TorParams q = p;
+    if (frame_image(q, q.img_bytes))
+        hipLaunchKernelGGL((record_proj_kernel<NB, LPE, true>), dim3(blocks_for(q.n * LPE, block)),
+                           dim3(block), 0, st, q, rec);
+    else
+        hipLaunchKernelGGL((record_proj_kernel<NB, LPE, false>),
+                           dim3(blocks_for(q.n * LPE, block)), dim3(block), 0, st, q, rec);
     return launch_status();
 }
 
@@ -94,13 +108,14 @@ int launch_record_project(const TorParams &p, float4 *rec, uint32_t flags, hipSt
         return SRFX_OK;
     static const int kBlock[4] = {256, 64, 128, 256};
     const int block = kBlock[(flags >> SRFX_BLOCK_SHIFT) & 3u];
+    // no lanes-per-event flag: four lanes per event, the fastest measured
     switch ((flags >> SRFX_LPE_SHIFT) & 3u) {
-    case 2:
-        return launch_record_lpe<4>(p, rec, block, st);
+    case 1:
+        return launch_record_lpe<1>(p, rec, block, st);
     case 3:
         return launch_record_lpe<16>(p, rec, block, st);
     default:
-        return launch_record_lpe<1>(p, rec, block, st);
+        return launch_record_lpe<4>(p, rec, block, st);
     }
 }
 

@@ -404,7 +404,7 @@ __device__ __forceinline__ float tile_event(const TorParams &p, const TileGeom &
 // the event's backprojection: 1/p for a projection p > 1e-7, else 0 (the
 // event adds nothing, as the reference's ratio skips it), so that each of the
 // event's listings reads it instead of dividing again.
-template <int NB, int LPE>
+template <int NB, int LPE, bool FRAME = false>
 __device__ __forceinline__ void record_project(const TorParams &p, float4 *rec, int64_t e,
                                                int sub, bool valid)
 {
@@ -414,7 +414,8 @@ __device__ __forceinline__ void record_project(const TorParams &p, float4 *rec, 
         Track t;
         unpack_walk(p, rec, e, ev, t);
         if (ev.s_lo <= ev.s_hi)
-            part = walk_event<0, NB>(p, ev, t, p.use_tof, 0.0f, ev.s_lo + sub, ev.s_hi, LPE);
+            part = walk_event<0, NB, FRAME>(p, ev, t, p.use_tof, 0.0f, ev.s_lo + sub, ev.s_hi,
+                                            LPE);
     }
     const float pv = group_sum<LPE>(part);
     if (valid && sub == 0)
@@ -422,12 +423,13 @@ __device__ __forceinline__ void record_project(const TorParams &p, float4 *rec, 
             pv > 1e-7 ? 1.0f / pv : 0.0f;
 }
 
-template <int NB, int LPE>
+// FRAME: p.img is a contiguous op frame of p.img_bytes (frame gathers).
+template <int NB, int LPE, bool FRAME>
 __global__ __launch_bounds__(256) void record_proj_kernel(TorParams p, float4 *rec)
 {
     const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t e = gid / LPE;
-    record_project<NB, LPE>(p, rec, e, (int)(gid % LPE), e < p.n);
+    record_project<NB, LPE, FRAME>(p, rec, e, (int)(gid % LPE), e < p.n);
 }
 
 

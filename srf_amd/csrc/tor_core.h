@@ -46,6 +46,7 @@ struct TorParams {
     unsigned long long *nnz_total; // count: running total (nullable)
     int gx, gy, gz;
     int sx, sy, sz; // strides of the read image
+    int img_bytes;  // bytes of a read image in contiguous op-frame layout (frame gathers)
     int ox, oy, oz; // strides of the written image
     double inter_x, inter_y, inter_z;
     double rinter_x, rinter_y, rinter_z;
@@ -830,6 +831,84 @@ __device__ __forceinline__ float slice_box(const TorParams &p, const Event &ev, 
     }
 }
 
+// The gather of slice_box<0> from a contiguous op-frame image (unit column
+// stride, rows p.sy apart, p.img_bytes long), read through a buffer
+// resource: every candidate is loaded, a row's columns at immediate offsets
+// from one row offset (a candidate past the image end reads 0; candidate
+// indices are never negative, the box starting inside the grid), and the
+// dropped candidates are zeroed before the sum, so no decision is held as a
+// mask across the loads.  Same decisions, weights and summation order (a
+// dropped candidate adds fma(0, 0, acc) = acc) as slice_box<0>.
+template <int BX, int BY>
+__device__ __forceinline__ float slice_gather_frame(const TorParams &p, const Event &ev,
+                                                    const Track &t, const Cross &c, float amp,
+                                                    int s, int i_lo, int i_hi, int j_lo,
+                                                    int j_hi)
+{
+    constexpr int NC = BX * BY;
+    const float eps = 1.0e-4f;
+    const int ib0 = (int)ceilf((c.dx0 - t.hx) * p.rfinter_x - eps);
+    const int jb0 = (int)ceilf((c.dy0 - t.dymax) * p.rfinter_y - eps);
+    const int ib = max(i_lo, ib0), ie = min(i_hi, ib0 + t.nx - 1);
+    const int jb = max(j_lo, jb0), je = min(j_hi, jb0 + t.ny - 1);
+    float cx2[BX], cux[BX], rt2[BY], ruy[BY];
+#pragma unroll
+    for (int k = 0; k < BX; k++)
+        col_terms(voxel_offset(c.dx0, ib + k, p.finter_x), ev.ux, cx2[k], cux[k]);
+#pragma unroll
+    for (int k = 0; k < BY; k++)
+        row_terms(voxel_offset(c.dy0, jb + k, p.finter_y), ev.uy, ev.thr, rt2[k], ruy[k]);
+    float dt[NC];
+    float gap = INFINITY;
+#pragma unroll
+    for (int jj = 0; jj < BY; jj++) {
+#pragma unroll
+        for (int ii = 0; ii < BX; ii++) {
+            const int k = jj * BX + ii;
+            dt[k] = tube_dt(cx2[ii], cux[ii], rt2[jj], ruy[jj]);
+            gap = fminf(gap, fabsf(dt[k]));
+        }
+    }
+    const __amdgpu_buffer_rsrc_t rs =
+        __builtin_amdgcn_make_buffer_rsrc((void *)p.img, (short)0, p.img_bytes, 0x00020000);
+    const unsigned o0 = 4u * (unsigned)(s * p.sz + (c.iy0 + jb) * p.sy + c.ix0 + ib);
+    float v[NC];
+#pragma unroll
+    for (int jj = 0; jj < BY; jj++) {
+        const unsigned orow = o0 + 4u * (unsigned)(jj * p.sy);
+#pragma unroll
+        for (int ii = 0; ii < BX; ii++)
+            v[jj * BX + ii] =
+                __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs, orow + 4u * ii, 0, 0));
+    }
+    auto ok = [&](int k) { return (ib + k % BX <= ie) & (jb + k / BX <= je); };
+    auto sum = [&](auto keep) {
+        float acc = 0.0f;
+#pragma unroll
+        for (int k = 0; k < NC; k++) {
+            const bool kk = keep(k);
+            const float w = kk ? voxel_weight(ev, amp, dt[k]) : 0.0f;
+            acc = fmaf(w, kk ? v[k] : 0.0f, acc);
+        }
+        return acc;
+    };
+    if (gap > kThrMargin)
+        return sum([&](int k) { return ok(k) & (dt[k] < -kThrMargin); });
+    // rare: the reference's FP64 decision, one voxel at a time
+    uint32_t in = 0, near = 0;
+#pragma unroll
+    for (int k = 0; k < NC; k++) {
+        in |= (uint32_t)(ok(k) & (dt[k] < -kThrMargin)) << k;
+        near |= (uint32_t)(ok(k) & (fabsf(dt[k]) <= kThrMargin)) << k;
+    }
+    for (uint32_t m = (uint32_t)opaque((int)near); m; m &= m - 1) {
+        const int k = __builtin_ctz(m);
+        if (inside_exact(p, ev, s, c.ix0 + ib + k % BX, c.iy0 + jb + k / BX))
+            in |= 1u << k;
+    }
+    return sum([&](int k) { return ((in >> k) & 1u) != 0; });
+}
+
 // Box shape of an event for a compiled box size NB: 1 = NB x (NB - 1)
 // (enough for every event whose ellipse spans at most NB - 1 rows, e.g. all
 // C2 events), 2 = NB x NB, 0 = larger (chord walk).
@@ -843,7 +922,7 @@ __device__ __forceinline__ int box_shape(const Track &t)
 
 // The box walk of one slice over the strided image (patch voxels inside the
 // grid).
-template <int KIND, int NB>
+template <int KIND, int NB, bool FRAME = false>
 __device__ __forceinline__ float slice_box_image(const TorParams &p, const Event &ev,
                                                  const Track &t, const Cross &c, float amp, int s,
                                                  float scale, int shape)
@@ -855,6 +934,11 @@ __device__ __forceinline__ float slice_box_image(const TorParams &p, const Event
     const int i_hi = min(P, p.gx - c.ix0) - 1;
     const int j_lo = c.iy0 < 0 ? -c.iy0 : 0;
     const int j_hi = min(P, p.gy - c.iy0) - 1;
+    if constexpr (KIND == 0 && FRAME) {
+        if (shape == 1)
+            return slice_gather_frame<NB, NB - 1>(p, ev, t, c, amp, s, i_lo, i_hi, j_lo, j_hi);
+        return slice_gather_frame<NB, NB>(p, ev, t, c, amp, s, i_lo, i_hi, j_lo, j_hi);
+    }
     PatchIdx idx;
     if (KIND == 0) {
         idx.base = s * p.sz + c.ix0 * p.sx + c.iy0 * p.sy;
@@ -875,7 +959,7 @@ __device__ __forceinline__ float slice_box_image(const TorParams &p, const Event
 // Walk the crossed slices s_first, s_first + rstep, ... <= s_last of an event
 // (the caller clips to [s_lo, s_hi]).  NB > 0: slices go through the NB x NB
 // candidate box when the event's box fits, else through the chord walk.
-template <int KIND, int NB>
+template <int KIND, int NB, bool FRAME = false>
 __device__ __forceinline__ float walk_event(const TorParams &p, const Event &ev, const Track &t,
                                             bool tof, float scale, int s_first, int s_last,
                                             int rstep)
@@ -892,7 +976,7 @@ __device__ __forceinline__ float walk_event(const TorParams &p, const Event &ev,
         float amp = KIND == 1 ? (tof ? tof_exp(p, ev, t, s) * ts : scale)
                               : (tof ? tof_factor(p, ev, t, s) : 1.0f);
         if (NB > 0 && shape)
-            acc += slice_box_image<KIND, NBB>(p, ev, t, c, amp, s, scale, shape);
+            acc += slice_box_image<KIND, NBB, FRAME>(p, ev, t, c, amp, s, scale, shape);
         else if (p.full_patch)
             acc += patch_walk<KIND, true>(p, ev, t, c, amp, s, scale);
         else

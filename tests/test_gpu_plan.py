@@ -200,3 +200,42 @@ def test_plan_osem_op_frame_copy(cuda, case, axis):
         plan.osem_subset(frame, out=torch.zeros_like(world), op_frame=True,
                          tiled_projection=True)
     plan.close()
+
+
+def test_frame_gather_far_corner(cuda):
+    """Events through the op frame's last voxel rows: the record projection
+    reads a contiguous op frame with buffer loads whose candidate boxes run
+    past the image end (those candidates read 0 and are dropped).  Voxels
+    kept next to them must still read their values: the plan's OSEM step
+    equals the untiled fused step (strided gathers), and both match the FP64
+    oracle."""
+    from oracle import lib as O
+    from srf_amd import ops
+    rng = np.random.default_rng(21)
+    c = np.array(H.TOR_CENTER, np.float64)
+    s = np.array(H.TOR_SIZE, np.float64)
+    lo, hi = c - s / 2, c + s / 2
+    n = 512
+    p1 = np.empty((n, 3))
+    p1[:, 0] = rng.uniform(hi[0] - 3.0, hi[0], n)   # last voxel columns along x
+    p1[:, 1] = rng.uniform(hi[1] - 3.0, hi[1], n)   # last rows along y
+    p1[:, 2] = lo[2] - 5.0
+    p2 = p1.copy()
+    p2[:, 0] += rng.uniform(-1.0, 1.0, n)
+    p2[:, 1] += rng.uniform(-1.0, 1.0, n)
+    p2[:, 2] = hi[2] + 5.0
+    pc = p1 + rng.uniform(0.8, 0.97, n)[:, None] * (p2 - p1)  # TOF centres in the last slices
+    f = rng.uniform(0.5, 1.5, n)
+    lors = np.vstack([p1.T, p2.T, pc.T, f[None, :]]).astype(np.float32)
+    img = H.image(H.TOR_GRID, seed=9)
+    L, X = _t(lors, cuda), _t(img, cuda)
+    args = (H.TOR_GRID, H.TOR_CENTER, H.TOR_SIZE, H.KW, False, H.TOF_BIN, H.TOF_SIGMA2)
+    plan = _plan(L, False, True, (8, 16, 16), 256, tiled_projection=False)
+    got = plan.osem_subset(X).cpu().numpy()
+    ref = ops.osem_subset(L, X, *args).cpu().numpy()
+    plan.close()
+    assert float(np.abs(ref[-1, -1, :]).sum()) > 0  # the last row of the last slice is hit
+    assert H.rel_l2(got, ref) <= 1e-6
+    p = ops.projection(L, X, *args).cpu().numpy()
+    pref = O.tor_project(lors, img, *args, prec="f64")
+    assert H.rel_l2(p, pref) <= TOL64
