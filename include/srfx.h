@@ -86,7 +86,8 @@ enum srfx_status {
  * gather instead of the untiled kernel (selected by the schedule bits). */
 #define SRFX_TILED_PROJECTION (1u << 10)
 /* srfx_tor_plan_osem_subset in two calls (e.g. to time the two kernels):
- * SRFX_OSEM_PROJECT_ONLY runs only the projection into the plan's event
+ * SRFX_OSEM_PROJECT_ONLY runs only the projection, which stores each
+ * event's backprojection scale 1/p (0 for p <= 1e-7) in the plan's event
  * records, SRFX_OSEM_SCATTER_ONLY only the tiled backprojection of 1/p from
  * them (zeroing bp unless SRFX_ACCUMULATE).  Not with SRFX_TILED_PROJECTION. */
 #define SRFX_OSEM_PROJECT_ONLY (1u << 11)
