@@ -64,7 +64,6 @@ void launch_pass(const TorParams &p, const TileGeom &g, const PlanArgs &a, hipSt
         hipLaunchKernelGGL((plan_pass<FILL, false>), grid, block, 0, st, p, g, a);
 }
 
-// Count, scan (host: per tile; device: per event when slots are kept), fill.
 // A read image in contiguous op-frame layout of fewer than 2^31 bytes goes
 // through the frame gathers (buffer loads with 32-bit offsets).
 bool frame_image(const TorParams &p, int &bytes)
@@ -119,6 +118,8 @@ int launch_record_project(const TorParams &p, float4 *rec, uint32_t flags, hipSt
     }
 }
 
+// Count, scan (host: per tile; device: per event when slots are kept), fill,
+// and without slot tables the listing sort.
 int plan_build(srfx_tor_plan_s *pl, hipStream_t st, int chunk, bool keep_slots)
 {
     TorParams &p = pl->p;
@@ -223,6 +224,9 @@ int plan_build(srfx_tor_plan_s *pl, hipStream_t st, int chunk, bool keep_slots)
     a.list = pl->list;
     a.ev_slot = keep_slots ? pl->ev_slot : nullptr;
     a.err = cnt_d + 2 * ntiles;
+    // without the tiled projection's slot tables each list entry also carries
+    // its listing's slices where the event index and the slab fit its fields
+    pl->g.spans = !keep_slots && g.S <= kSpanMaxSlab && n < ((int64_t)1 << kSpanEventBits);
     if (n > 0)
         launch_pass<true>(p, g, a, st);
     if ((rc = launch_status()))
@@ -238,17 +242,13 @@ int plan_build(srfx_tor_plan_s *pl, hipStream_t st, int chunk, bool keep_slots)
         goto done;
     }
     // without the tiled projection's slot tables the listings of a chunk may
-    // be reordered: longest walks first, each entry with its listing's slices
-    // where the event index and the slab fit the entry's fields
+    // be reordered: longest walks first
     if (!keep_slots && pl->nchunks > 0 && chunk <= 16 * 256) {
-        pl->g.spans = pl->g.S <= kSpanMaxSlab && pl->p.n < ((int64_t)1 << kSpanEventBits);
         hipLaunchKernelGGL(sort_chunks, dim3(pl->nchunks), dim3(256),
                            sizeof(int) * (33 + chunk), st, pl->p, pl->g, pl->chunks, pl->list,
                            pl->rec);
-        if ((rc = launch_status())) {
-            pl->g.spans = 0;
+        if ((rc = launch_status()))
             goto done;
-        }
         pl->g.sorted = 1;
     }
 done:

@@ -34,7 +34,7 @@ struct TileGeom {
     int S, Wx, Wy;       // slices per slab, window width along op x and y (voxels)
     int nwx, nwy, nslab; // windows per axis and number of slabs
     int sorted;          // chunks' listings ordered by slice count (sort_chunks)
-    int spans;           // sorted list entries carry their slices (listing_entry)
+    int spans;           // list entries carry their listing's slices (kSpanEventBits)
 };
 
 // A list entry of a plan with spans: the event index in bits 0..23 and the
@@ -94,6 +94,82 @@ __device__ __forceinline__ void for_each_tile(const TorParams &p, const TileGeom
             for (int wx = wx0; wx <= wx1; wx++)
                 f((sl * g.nwy + wy) * g.nwx + wx);
     }
+}
+
+} // namespace
+
+struct TileBox {
+    int sa, sb, x0, x1, y0, y1;
+};
+
+namespace {
+
+// Tile bounds of a chunk: slices sa..sb, op-frame voxels x0..x1, y0..y1.
+__device__ __forceinline__ TileBox tile_box(const TorParams &p, const TileGeom &g, int tile)
+{
+    TileBox b;
+    const int per = g.nwx * g.nwy;
+    const int sl = tile / per, r = tile - sl * per;
+    const int wy = r / g.nwx, wx = r - wy * g.nwx;
+    b.sa = sl * g.S;
+    b.sb = min(b.sa + g.S, p.gz) - 1;
+    b.x0 = wx * g.Wx;
+    b.x1 = min(b.x0 + g.Wx, p.gx) - 1;
+    b.y0 = wy * g.Wy;
+    b.y1 = min(b.y0 + g.Wy, p.gy) - 1;
+    return b;
+}
+
+// Whether a scatter into p.out should use the y-fastest LDS tile.
+__host__ __device__ inline bool flush_y_fastest(const TorParams &p)
+{
+    return (p.oy < 0 ? -p.oy : p.oy) < (p.ox < 0 ? -p.ox : p.ox);
+}
+
+// Index of voxel (ix, iy) of slice s in the LDS tile of box b: x fastest, or
+// (YF) y fastest, so that a flush along the tile's fastest axis writes
+// consecutive addresses of an output whose unit stride is along op y.
+
+// Narrow [s0, s1] to a superset of the slices whose patch can meet the
+// window columns x0..x1 and rows y0..y1.  The patch origin is trunc(u) - half
+// (or the exact origin, one voxel away) with u(s) = ax + s bx, so the patch
+// meets the columns only if x0 - P + half - 2 < u(s) < x1 + half + 2; the
+// bounds on s get one more slice of slack on each side.  The quotients are
+// formed in FP32 (approximate reciprocal): their error, a few 1e-7 of the
+// slice index, is far inside that slack, and slices outside the bounds
+// only cost work (the walk clips every voxel to the window).
+__device__ __forceinline__ void axis_slices(double a, double b, double lo, double hi, int &s0,
+                                            int &s1)
+{
+    if (fabs(b) < 1.0e-9) {
+        // u is constant to 1e-9 voxel per slice: test it at both ends
+        const double u0 = a + (double)s0 * b, u1 = a + (double)s1 * b;
+        if (fmax(u0, u1) <= lo - 1.0 || fmin(u0, u1) >= hi + 1.0)
+            s1 = s0 - 1;
+        return;
+    }
+    const float r = __builtin_amdgcn_rcpf((float)b);
+    const float t1 = (float)(lo - a) * r, t2 = (float)(hi - a) * r;
+    const float tl = fmaxf(fminf(t1, t2), -1.0e8f), th = fminf(fmaxf(t1, t2), 1.0e8f);
+    s0 = max(s0, (int)floorf(tl) - 1);
+    s1 = min(s1, (int)ceilf(th) + 1);
+}
+
+__device__ __forceinline__ void window_slices(const Track &t, int half, int P, int x0, int x1,
+                                              int y0, int y1, int &s0, int &s1)
+{
+    axis_slices(t.ax, t.bx, (double)(x0 - P + half - 2), (double)(x1 + half + 2), s0, s1);
+    axis_slices(t.ay, t.by, (double)(y0 - P + half - 2), (double)(y1 + half + 2), s0, s1);
+}
+
+// The slices s0..s1 a listing walks: the event's crossed slices in the
+// tile's slab, narrowed to those whose patch can meet the window.
+__device__ __forceinline__ void listing_slices(const TorParams &p, const TileBox &b,
+                                               const Event &ev, const Track &t, int &s0, int &s1)
+{
+    s0 = max(ev.s_lo, b.sa);
+    s1 = min(ev.s_hi, b.sb);
+    window_slices(t, p.half, p.patch, b.x0, b.x1, b.y0, b.y1, s0, s1);
 }
 
 struct PlanArgs {
@@ -205,7 +281,17 @@ __global__ __launch_bounds__(256) void plan_pass(TorParams p, TileGeom g, PlanAr
             fits = fits && rank < a.tile_cnt[tile] && (!a.ev_slot || k0 + k < k1);
             if (fits) {
                 int slot = a.tile_off[tile] + rank;
-                a.list[slot] = (int)e;
+                int ent = (int)e;
+                if (g.spans) {
+                    // the listing's slices ride in the entry (the host checked
+                    // the limits)
+                    const TileBox b = tile_box(p, g, tile);
+                    int s0, s1;
+                    listing_slices(p, b, ev, t, s0, s1);
+                    const int lo = s0 <= s1 ? s0 - b.sa : 1, hi = s0 <= s1 ? s1 - b.sa : 0;
+                    ent |= (lo << kSpanEventBits) | (hi << (kSpanEventBits + 4));
+                }
+                a.list[slot] = ent;
                 if (a.ev_slot)
                     a.ev_slot[k0 + k] = slot;
             } else {
@@ -224,39 +310,6 @@ __global__ __launch_bounds__(256) void fill_int(int *v, int64_t n, int value)
         v[i] = value;
 }
 
-// Tile bounds of a chunk: slices sa..sb, op-frame voxels x0..x1, y0..y1.
-} // namespace
-
-struct TileBox {
-    int sa, sb, x0, x1, y0, y1;
-};
-
-namespace {
-
-__device__ __forceinline__ TileBox tile_box(const TorParams &p, const TileGeom &g, int tile)
-{
-    TileBox b;
-    const int per = g.nwx * g.nwy;
-    const int sl = tile / per, r = tile - sl * per;
-    const int wy = r / g.nwx, wx = r - wy * g.nwx;
-    b.sa = sl * g.S;
-    b.sb = min(b.sa + g.S, p.gz) - 1;
-    b.x0 = wx * g.Wx;
-    b.x1 = min(b.x0 + g.Wx, p.gx) - 1;
-    b.y0 = wy * g.Wy;
-    b.y1 = min(b.y0 + g.Wy, p.gy) - 1;
-    return b;
-}
-
-// Whether a scatter into p.out should use the y-fastest LDS tile.
-__host__ __device__ inline bool flush_y_fastest(const TorParams &p)
-{
-    return (p.oy < 0 ? -p.oy : p.oy) < (p.ox < 0 ? -p.ox : p.ox);
-}
-
-// Index of voxel (ix, iy) of slice s in the LDS tile of box b: x fastest, or
-// (YF) y fastest, so that a flush along the tile's fastest axis writes
-// consecutive addresses of an output whose unit stride is along op y.
 template <bool YF>
 __device__ __forceinline__ int lds_index(const TileGeom &g, const TileBox &b, int s, int ix, int iy)
 {
@@ -308,48 +361,6 @@ __device__ __forceinline__ float window_walk(const TorParams &p, const TileGeom 
         }
     }
     return acc;
-}
-
-// Narrow [s0, s1] to a superset of the slices whose patch can meet the
-// window columns x0..x1 and rows y0..y1.  The patch origin is trunc(u) - half
-// (or the exact origin, one voxel away) with u(s) = ax + s bx, so the patch
-// meets the columns only if x0 - P + half - 2 < u(s) < x1 + half + 2; the
-// bounds on s get one more slice of slack on each side.  The quotients are
-// formed in FP32 (approximate reciprocal): their error, a few 1e-7 of the
-// slice index, is far inside that slack, and slices outside the bounds
-// only cost work (the walk clips every voxel to the window).
-__device__ __forceinline__ void axis_slices(double a, double b, double lo, double hi, int &s0,
-                                            int &s1)
-{
-    if (fabs(b) < 1.0e-9) {
-        // u is constant to 1e-9 voxel per slice: test it at both ends
-        const double u0 = a + (double)s0 * b, u1 = a + (double)s1 * b;
-        if (fmax(u0, u1) <= lo - 1.0 || fmin(u0, u1) >= hi + 1.0)
-            s1 = s0 - 1;
-        return;
-    }
-    const float r = __builtin_amdgcn_rcpf((float)b);
-    const float t1 = (float)(lo - a) * r, t2 = (float)(hi - a) * r;
-    const float tl = fmaxf(fminf(t1, t2), -1.0e8f), th = fminf(fmaxf(t1, t2), 1.0e8f);
-    s0 = max(s0, (int)floorf(tl) - 1);
-    s1 = min(s1, (int)ceilf(th) + 1);
-}
-
-__device__ __forceinline__ void window_slices(const Track &t, int half, int P, int x0, int x1,
-                                              int y0, int y1, int &s0, int &s1)
-{
-    axis_slices(t.ax, t.bx, (double)(x0 - P + half - 2), (double)(x1 + half + 2), s0, s1);
-    axis_slices(t.ay, t.by, (double)(y0 - P + half - 2), (double)(y1 + half + 2), s0, s1);
-}
-
-// The slices s0..s1 a listing walks: the event's crossed slices in the
-// tile's slab, narrowed to those whose patch can meet the window.
-__device__ __forceinline__ void listing_slices(const TorParams &p, const TileBox &b,
-                                               const Event &ev, const Track &t, int &s0, int &s1)
-{
-    s0 = max(ev.s_lo, b.sa);
-    s1 = min(ev.s_hi, b.sb);
-    window_slices(t, p.half, p.patch, b.x0, b.x1, b.y0, b.y1, s0, s1);
 }
 
 // One listing: its slices s0..s1 inside the tile (listing_slices), each
@@ -533,8 +544,9 @@ __device__ __forceinline__ void scatter_chunk(const TorParams &p, const TileGeom
 // of slices they walk in the tile, longest first (a counting sort in LDS,
 // stable), so that the consecutive listings a wave of the scatter takes walk
 // about as many slices each (the order inside a bucket is the atomics').
-// With g.spans each entry also takes its listing's slices.  One workgroup
-// per chunk; LDS: 33 counters and `chunk` ints.  Needs ch.count <= 16 *
+// With g.spans the slice counts come from the entries (the fill pass wrote
+// them) instead of the walk records.  One workgroup per chunk; LDS: 33
+// counters and `chunk` ints.  Needs ch.count <= 16 *
 // blockDim.x.
 __global__ __launch_bounds__(256) void sort_chunks(TorParams p, TileGeom g,
                                                    const TileChunk *__restrict__ chunks,
@@ -556,19 +568,20 @@ __global__ __launch_bounds__(256) void sort_chunks(TorParams p, TileGeom g,
         key[r] = -1;
         if (j < ch.count) {
             ev[r] = list[ch.start + j];
-            Event e;
-            Track t;
-            unpack_walk(p, rec, ev[r], e, t);
             int s0, s1;
-            listing_slices(p, b, e, t, s0, s1);
+            if (g.spans) {
+                // the fill pass put the listing's slices in the entry
+                s0 = (ev[r] >> kSpanEventBits) & 15;
+                s1 = (int)((unsigned)ev[r] >> (kSpanEventBits + 4));
+            } else {
+                Event e;
+                Track t;
+                unpack_walk(p, rec, ev[r], e, t);
+                listing_slices(p, b, e, t, s0, s1);
+            }
             // longest first: bucket 0 holds the longest walks
             key[r] = kBuckets - 1 - min(max(s1 - s0 + 1, 0), kBuckets - 1);
             atomicAdd(&cnt[key[r]], 1);
-            if (g.spans) {
-                // the span rides in the entry (the host checked the limits)
-                const int a = s0 <= s1 ? s0 - b.sa : 1, z = s0 <= s1 ? s1 - b.sa : 0;
-                ev[r] |= (a << kSpanEventBits) | (z << (kSpanEventBits + 4));
-            }
         }
     }
     __syncthreads();
