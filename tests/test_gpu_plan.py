@@ -73,15 +73,19 @@ def test_plan_project_backproject(cuda, case, tile, chunk, geo, tof):
     plan.close()
 
 
-@pytest.mark.parametrize("tiled_projection", [False, True])
-def test_plan_osem_subset(cuda, case, tiled_projection):
+# Plans without the tiled projection's slot tables sort their chunks and
+# carry each listing's slices in its list entry when the slab has at most 16
+# slices; a 20-slice slab sorts without them.
+@pytest.mark.parametrize("tiled_projection,tile", [(False, (4, 16, 16)), (True, (4, 16, 16)),
+                                                   (False, (20, 8, 8))])
+def test_plan_osem_subset(cuda, case, tiled_projection, tile):
     import torch
     from oracle import lib as O
     from srf_amd import ops
     lors, img = case
     args = (H.TOR_GRID, H.TOR_CENTER, H.TOR_SIZE, H.KW, True, H.TOF_BIN, H.TOF_SIGMA2)
     L, X = _t(lors, cuda), _t(img, cuda)
-    plan = _plan(L, True, True, (4, 16, 16), 512, tiled_projection=tiled_projection)
+    plan = _plan(L, True, True, tile, 512, tiled_projection=tiled_projection)
     f = plan.osem_subset(X, tiled_projection=tiled_projection).cpu().numpy()
     ref = O.tor_project(lors, img, *args, prec="f64")
     keep = ref > 1e-7
