@@ -639,7 +639,9 @@ __global__ __launch_bounds__(256) void tile_gather(TorParams p, TileGeom g,
     __syncthreads();
     const bool tof = p.use_tof;
     for (int j = threadIdx.x; j < ch.count; j += blockDim.x) {
-        const int64_t e = list[ch.start + j];
+        // plans with slot tables keep plain event indices (g.spans 0)
+        const int ent = list[ch.start + j];
+        const int64_t e = g.spans ? (ent & ((1 << kSpanEventBits) - 1)) : ent;
         Event ev;
         Track t;
         unpack_walk(p, rec, e, ev, t);
