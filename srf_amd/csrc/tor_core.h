@@ -836,9 +836,9 @@ __device__ __forceinline__ float slice_box(const TorParams &p, const Event &ev, 
 // resource: every candidate is loaded, a row's columns at immediate offsets
 // from one row offset (a candidate past the image end reads 0; candidate
 // indices are never negative, the box starting inside the grid), and the
-// dropped candidates are zeroed before the sum, so no decision is held as a
-// mask across the loads.  Same decisions, weights and summation order (a
-// dropped candidate adds fma(0, 0, acc) = acc) as slice_box<0>.
+// dropped candidates' values are zeroed before the sum, so no decision is
+// held as a mask across the loads.  Same decisions, weights and summation
+// order (a dropped candidate adds fma(w, 0, acc) = acc) as slice_box<0>.
 template <int BX, int BY>
 __device__ __forceinline__ float slice_gather_frame(const TorParams &p, const Event &ev,
                                                     const Track &t, const Cross &c, float amp,
@@ -882,14 +882,13 @@ __device__ __forceinline__ float slice_gather_frame(const TorParams &p, const Ev
                 __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs, orow + 4u * ii, 0, 0));
     }
     auto ok = [&](int k) { return (ib + k % BX <= ie) & (jb + k / BX <= je); };
+    // a dropped candidate's weight is finite (dt >= -thr), so zeroing its
+    // value alone makes its term fma(w, 0, acc) = acc
     auto sum = [&](auto keep) {
         float acc = 0.0f;
 #pragma unroll
-        for (int k = 0; k < NC; k++) {
-            const bool kk = keep(k);
-            const float w = kk ? voxel_weight(ev, amp, dt[k]) : 0.0f;
-            acc = fmaf(w, kk ? v[k] : 0.0f, acc);
-        }
+        for (int k = 0; k < NC; k++)
+            acc = fmaf(voxel_weight(ev, amp, dt[k]), keep(k) ? v[k] : 0.0f, acc);
         return acc;
     };
     if (gap > kThrMargin)
