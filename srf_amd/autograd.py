@@ -39,7 +39,7 @@ class _TorProject(torch.autograd.Function):
             if n:
                 src, g = model._source(img, a)  # op-frame copy when the model stages
                 _lib.call("srfx_tor_project", _ptr(lors), n, ld, _ptr(src), g, model._model(),
-                          _ptr(p), 0, _stream())
+                          _ptr(p), model._flags("projection"), _stream())
             outs.append(p)
         ctx.model = model
         ctx.plans = plans
