@@ -322,6 +322,10 @@ def batched_c4(args):
     bat = BatchedOSEM(recons, model, nb_subsets=K)
     torch.cuda.synchronize()
     log(f"batch plan ({R * 3 * K} tile plans) in {time.time() - t0:.2f}s")
+    t0 = time.time()
+    tune = bat.autotune(reps=2)
+    log(f"projection schedules (lanes per event, events per item) tuned in "
+        f"{time.time() - t0:.2f}s: {bat.schedule}")
     bat.capture()
     reset()
     bat.replay(1)  # warm-up
@@ -356,7 +360,9 @@ def batched_c4(args):
         total * iters / (ms_b / 1e3) / 1e6, "Mevents/s", iters, 1, ms_b / iters,
         {"workload": "batched_c4", "reconstructions": R, "events": total, "subsets": K,
          "iterations": iters, "image_sizes": [list(k[0]) for k in kinds],
-         "step": "one OSEM iteration of all reconstructions"},
+         "step": "one OSEM iteration of all reconstructions",
+         "projection_schedule": list(bat.schedule),
+         "autotune_iteration_ms": {f"{l}x{e}": round(ms, 2) for (l, e), ms in tune.items()}},
         batched_vs_serial={"batched_graph_ms": round(ms_b, 2), "serial_ms": round(ms_s, 2),
                            "speedup": round(ms_s / ms_b, 3),
                            "serial_Mevents_per_s": round(total * iters / (ms_s / 1e3) / 1e6, 2),

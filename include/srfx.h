@@ -292,6 +292,14 @@ int srfx_tor_batch_create(const srfx_recon *recons, int32_t nrecon, int32_t nb_s
  * graph.  Calls on one plan must not overlap. */
 int srfx_tor_batch_subset(srfx_tor_batch *plan, int32_t subset, void *stream);
 
+/* Schedule of the plan's persistent projection: lanes per event (1, 4 or 16)
+ * and events per work item a wave pulls (64, 128 or 256); default 4 and 64.
+ * SRFX_EINVAL for any other pair (the schedule is then unchanged).  Set it
+ * before capturing subsets in a HIP graph: a graph keeps the kernel it
+ * recorded.  No reference counterpart (launch tuning of the batch). */
+int srfx_tor_batch_set_schedule(srfx_tor_batch *plan, int32_t lanes_per_event,
+                                int32_t events_per_item);
+
 int srfx_tor_batch_destroy(srfx_tor_batch *plan);
 
 /* ------------------------------------------------------------- Update --- */

@@ -132,7 +132,8 @@ def load():
 
 
 # bound in srf_amd/batch.py (they take the batch structs)
-_BATCH = ["srfx_tor_batch_create", "srfx_tor_batch_subset", "srfx_tor_batch_destroy"]
+_BATCH = ["srfx_tor_batch_create", "srfx_tor_batch_subset", "srfx_tor_batch_set_schedule",
+          "srfx_tor_batch_destroy"]
 _PSF = ["srfx_psf_apply"]  # bound in psf.py
 
 

@@ -22,6 +22,11 @@ from .ops import _check_dev, _stream
 from .recon import OSEM
 
 
+# Schedules of the persistent projection: lanes per event x events per work item.
+LANES = (1, 4, 16)
+ITEM_EVENTS = (64, 128, 256)
+
+
 class SrfxRecon(ctypes.Structure):
     _fields_ = [("shape", ctypes.c_int32 * 3), ("center", ctypes.c_float * 3),
                 ("size", ctypes.c_float * 3), ("image", ctypes.c_void_p),
@@ -39,6 +44,9 @@ def _bind():
         lib.srfx_tor_batch_create.restype = ctypes.c_int
         lib.srfx_tor_batch_subset.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p]
         lib.srfx_tor_batch_subset.restype = ctypes.c_int
+        lib.srfx_tor_batch_set_schedule.argtypes = [ctypes.c_void_p, ctypes.c_int32,
+                                                    ctypes.c_int32]
+        lib.srfx_tor_batch_set_schedule.restype = ctypes.c_int
         lib.srfx_tor_batch_destroy.argtypes = [ctypes.c_void_p]
         lib.srfx_tor_batch_destroy.restype = ctypes.c_int
         lib._batch_bound = True
@@ -88,6 +96,48 @@ class BatchedOSEM:
                                               ctypes.byref(handle)))
         self.handle = handle
         self.graph = None
+        self.schedule = (4, 64)
+
+    def set_schedule(self, lanes_per_event, events_per_item):
+        """Schedule of the persistent projection (srfx_tor_batch_set_schedule):
+        lanes per event in LANES, events per work item in ITEM_EVENTS.  Set it
+        before capture(): a graph keeps the kernel it recorded."""
+        _check(self.lib, "srfx_tor_batch_set_schedule",
+               self.lib.srfx_tor_batch_set_schedule(self.handle, int(lanes_per_event),
+                                                    int(events_per_item)))
+        self.schedule = (int(lanes_per_event), int(events_per_item))
+
+    def autotune(self, reps=2):
+        """Time one OSEM iteration (all subsets) for every schedule in
+        LANES x ITEM_EVENTS, keep the fastest and restore every image to its
+        value before the call.  Returns {(lanes, events): ms}, the minimum of
+        ``reps`` timed iterations after one warm-up each."""
+        saved = [rc["image"].data.clone() for rc in self.recons]
+
+        def restore():
+            for rc, x in zip(self.recons, saved):
+                rc["image"].data.copy_(x)
+
+        table = {}
+        for lanes in LANES:
+            for events in ITEM_EVENTS:
+                self.set_schedule(lanes, events)
+                best = float("inf")
+                for rep in range(reps + 1):
+                    restore()
+                    a = torch.cuda.Event(enable_timing=True)
+                    b = torch.cuda.Event(enable_timing=True)
+                    a.record()
+                    self.iterate(1)
+                    b.record()
+                    torch.cuda.synchronize()
+                    if rep:
+                        best = min(best, a.elapsed_time(b))
+                table[(lanes, events)] = best
+        restore()
+        self.set_schedule(*min(table, key=table.get))
+        torch.cuda.synchronize()
+        return table
 
     def run_subset(self, k):
         _check(self.lib, "srfx_tor_batch_subset",

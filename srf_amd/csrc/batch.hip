@@ -117,38 +117,42 @@ __device__ __forceinline__ int next_wave_item(unsigned *counter)
     return __builtin_amdgcn_readfirstlane(item);
 }
 
-// Waves pull quarters of the event chunks (64 events each, walked in four
-// rounds of 16 events at four lanes per event, the projection schedule of
-// the plan OSEM step) without synchronising with the other waves of their
-// block, so a wave with long walks holds up no other.  The gathers are the
-// strided ones: the frame gathers (record_project<.., true>), faster for one
-// plan, measured slower in this kernel (C4: 657 against 751 Mevents/s).
+// Waves pull work items of T events from the event chunks (kChunk / T items
+// per chunk), walked in rounds of 64 / LPE events at LPE lanes per event,
+// without synchronising with the other waves of their block, so a wave with
+// long walks holds up no other.  The default schedule, LPE = 4 and T = 64
+// (quarter chunks), is the projection schedule of the plan OSEM step.  The
+// gathers are the strided ones: the frame gathers (record_project<.., true>),
+// faster for one plan, measured slower in this kernel (C4: 657 against 751
+// Mevents/s).
+template <int LPE, int T>
 __global__ __launch_bounds__(256) void batch_project(const BatchJob *__restrict__ jobs,
                                                      const EvChunk *__restrict__ chunks,
                                                      int nchunks, unsigned *counter)
 {
-    constexpr int kQuarter = kChunk / 4;
+    constexpr int kParts = kChunk / T, kRound = 64 / LPE;
+    static_assert(kChunk % T == 0 && T % kRound == 0, "items split chunks into whole rounds");
     for (;;) {
         const int q = next_wave_item(counter);
-        if (q >= 4 * nchunks)
+        if (q >= kParts * nchunks)
             break;
-        const EvChunk ch = chunks[q >> 2];
+        const EvChunk ch = chunks[q / kParts];
         const BatchJob &job = jobs[__builtin_amdgcn_readfirstlane(ch.job)];
         float4 *rec = const_cast<float4 *>(job.rec);
-        const int lane = (int)__lane_id(), sub = lane & 3;
-        for (int r = 0; r < kQuarter; r += 16) {
-            const int i = (q & 3) * kQuarter + r + lane / 4;
+        const int lane = (int)__lane_id(), sub = lane % LPE;
+        for (int r = 0; r < T; r += kRound) {
+            const int i = (q % kParts) * T + r + lane / LPE;
             const bool valid = i < ch.count;
             const int64_t e = (int64_t)ch.start + i;
             switch (job.p.box) {
             case 3:
-                record_project<3, 4>(job.p, rec, e, sub, valid);
+                record_project<3, LPE>(job.p, rec, e, sub, valid);
                 break;
             case 4:
-                record_project<4, 4>(job.p, rec, e, sub, valid);
+                record_project<4, LPE>(job.p, rec, e, sub, valid);
                 break;
             default:
-                record_project<0, 4>(job.p, rec, e, sub, valid);
+                record_project<0, LPE>(job.p, rec, e, sub, valid);
                 break;
             }
         }
@@ -264,9 +268,53 @@ struct srfx_tor_batch {
     long long frame_blocks = 0;
     unsigned scatter_lds = 0;
     int proj_blocks = 0, scatter_blocks = 0, elem_blocks = 0;
+    int cus = 0;
+    int proj_lanes = 4, proj_events = 64; // projection schedule (batch_project<LPE, T>)
 };
 
 using namespace srfx;
+
+// The persistent projection of schedule (LPE, T): with subset < 0 sets the
+// plan's grid from the kernel's occupancy, else enqueues it for that subset.
+template <int LPE, int T>
+static int project_launch(srfx_tor_batch *b, int subset, hipStream_t st)
+{
+    if (subset < 0) {
+        int per_cu = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_project<LPE, T>, 256, 0) !=
+            hipSuccess)
+            return SRFX_ELAUNCH;
+        b->proj_blocks = std::max(1, b->cus * std::max(1, per_cu));
+        return SRFX_OK;
+    }
+    hipLaunchKernelGGL((batch_project<LPE, T>), dim3(b->proj_blocks), dim3(256), 0, st, b->d_jobs,
+                       b->d_ev[subset], b->nev[subset], b->d_counters);
+    return SRFX_OK;
+}
+
+// Lanes per event {1, 4, 16} x events per work item {64, 128, 256}.
+static int project_schedule(srfx_tor_batch *b, int lanes, int events, int subset, hipStream_t st)
+{
+    if (lanes == 1 && events == 64)
+        return project_launch<1, 64>(b, subset, st);
+    if (lanes == 1 && events == 128)
+        return project_launch<1, 128>(b, subset, st);
+    if (lanes == 1 && events == 256)
+        return project_launch<1, 256>(b, subset, st);
+    if (lanes == 4 && events == 64)
+        return project_launch<4, 64>(b, subset, st);
+    if (lanes == 4 && events == 128)
+        return project_launch<4, 128>(b, subset, st);
+    if (lanes == 4 && events == 256)
+        return project_launch<4, 256>(b, subset, st);
+    if (lanes == 16 && events == 64)
+        return project_launch<16, 64>(b, subset, st);
+    if (lanes == 16 && events == 128)
+        return project_launch<16, 128>(b, subset, st);
+    if (lanes == 16 && events == 256)
+        return project_launch<16, 256>(b, subset, st);
+    return SRFX_EINVAL;
+}
 
 static void free_plan(srfx_tor_batch *b)
 {
@@ -421,18 +469,17 @@ extern "C" int srfx_tor_batch_create(const srfx_recon *recons, int32_t nrecon,
         b->nwork.push_back((int)work[k].size());
     }
     b->scatter_lds = (unsigned)(sizeof(double) * kBatchTile.slices * kBatchTile.wx * kBatchTile.wy);
-    int dev = 0, cus = 0, per_cu_p = 0, per_cu_s = 0;
+    int dev = 0, cus = 0, per_cu_s = 0;
     ok = ok && hipGetDevice(&dev) == hipSuccess &&
          hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
-         hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu_p, batch_project, 256, 0) ==
-             hipSuccess &&
          hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu_s, batch_scatter, 256,
                                                       b->scatter_lds) == hipSuccess;
+    b->cus = cus;
+    ok = ok && project_schedule(b, b->proj_lanes, b->proj_events, -1, nullptr) == SRFX_OK;
     if (!ok) {
         free_plan(b);
         return SRFX_ELAUNCH;
     }
-    b->proj_blocks = std::max(1, cus * std::max(1, per_cu_p));
     b->scatter_blocks = std::max(1, cus * std::max(1, per_cu_s));
     b->elem_blocks = std::max(1, cus * 4);
     *out = b;
@@ -450,14 +497,27 @@ extern "C" int srfx_tor_batch_subset(srfx_tor_batch *b, int32_t subset, void *st
         hipLaunchKernelGGL(batch_frames, dim3((unsigned)b->frame_blocks), dim3(256), 0, st,
                            b->d_frames, b->nframes);
     if (b->nev[subset] > 0)
-        hipLaunchKernelGGL(batch_project, dim3(b->proj_blocks), dim3(256), 0, st, b->d_jobs,
-                           b->d_ev[subset], b->nev[subset], b->d_counters);
+        project_schedule(b, b->proj_lanes, b->proj_events, subset, st);
     if (b->nwork[subset] > 0)
         hipLaunchKernelGGL(batch_scatter, dim3(b->scatter_blocks), dim3(256), b->scatter_lds, st,
                            b->d_jobs, b->d_work[subset], b->nwork[subset], b->d_counters + 1);
     hipLaunchKernelGGL(batch_update, dim3(b->elem_blocks), dim3(256), 0, st, b->d_vox, b->nrec,
                        b->total_vox);
     return launch_status();
+}
+
+extern "C" int srfx_tor_batch_set_schedule(srfx_tor_batch *b, int32_t lanes_per_event,
+                                           int32_t events_per_item)
+{
+    if (!b)
+        return SRFX_EINVAL;
+    // the grid changes only when the schedule exists and its occupancy query succeeds
+    const int rc = project_schedule(b, lanes_per_event, events_per_item, -1, nullptr);
+    if (rc == SRFX_OK) {
+        b->proj_lanes = lanes_per_event;
+        b->proj_events = events_per_item;
+    }
+    return rc;
 }
 
 extern "C" int srfx_tor_batch_destroy(srfx_tor_batch *b)

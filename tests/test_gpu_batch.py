@@ -88,3 +88,44 @@ def test_graph_replays_match_eager(cuda):
         assert torch.isfinite(junk)
     gb.close()
     eb.close()
+
+
+def test_projection_schedules_equal_default(cuda):
+    """Every schedule of the persistent projection (lanes per event x events
+    per work item) gives the default schedule's images: only the order of the
+    lanes' partial sums of one event differs.  The autotuner keeps one of
+    them and restores the images it iterated."""
+    from srf_amd.batch import ITEM_EVENTS, LANES, BatchedOSEM
+    recons, sc = _recons(cuda, nrec=4, seed=2)
+    model = _model(sc)
+    ref = _clone(recons)
+    rb = BatchedOSEM(ref, model, nb_subsets=2)
+    rb.iterate(1)
+    torch.cuda.synchronize()
+    for lanes in LANES:
+        for events in ITEM_EVENTS:
+            got = _clone(recons)
+            b = BatchedOSEM(got, model, nb_subsets=2)
+            b.set_schedule(lanes, events)
+            b.iterate(1)
+            torch.cuda.synchronize()
+            for rg, rr in zip(got, ref):
+                e = H.rel_l2(rg["image"].data.cpu().numpy(), rr["image"].data.cpu().numpy())
+                assert e <= 1e-5, (lanes, events, e)
+            b.close()
+    with pytest.raises(RuntimeError):
+        rb.set_schedule(2, 64)
+    assert rb.schedule == (4, 64)
+    tuned = _clone(recons)
+    tb = BatchedOSEM(tuned, model, nb_subsets=2)
+    table = tb.autotune(reps=1)
+    assert set(table) == {(l, e) for l in LANES for e in ITEM_EVENTS}
+    assert tb.schedule == min(table, key=table.get)
+    for rt, r0 in zip(tuned, recons):
+        assert torch.equal(rt["image"].data, r0["image"].data)
+    tb.iterate(1)
+    torch.cuda.synchronize()
+    for rt, rr in zip(tuned, ref):
+        assert H.rel_l2(rt["image"].data.cpu().numpy(), rr["image"].data.cpu().numpy()) <= 1e-5
+    tb.close()
+    rb.close()
