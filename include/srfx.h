@@ -163,6 +163,16 @@ int srfx_tor_count_nnz(const float *lors, int64_t n, int64_t ld, srfx_grid grid,
                        srfx_tor_model model, int32_t *nnz_per_event, uint64_t *nnz_total,
                        void *stream);
 
+
+/* The reference's TOR patch width for a grid and kernel width, P =
+ * ceil((sqrt(2) kernel_width + inter_z) / inter_x) (tor.cu.cc:369-375), and
+ * the candidate box the kernels compile for it: 3 or 4 (NB x NB voxels per
+ * slice hold every kept voxel of an event whose slice axis is its dominant
+ * direction), or 0 when the ellipse of such an event can span more than 4
+ * voxels (the generic chord walk).  Host only: no device work.  Either
+ * output pointer may be NULL.  SRFX_EUNSUPPORTED for P > 16, as the ops. */
+int srfx_tor_patch_info(srfx_grid grid, float kernel_width, int32_t *patch, int32_t *box);
+
 /* ------------------------------------------------------------- Siddon --- */
 
 /* Replaces op "Projection" of siddon.so: siddon.cc:14-26, 88-136;
@@ -255,6 +265,14 @@ int srfx_tor_plan_maplors(srfx_tor_plan plan, const float *lors_value, float *im
  * of 1/p into bp.  Calls on one plan must not overlap. */
 int srfx_tor_plan_osem_subset(srfx_tor_plan plan, const float *image, srfx_grid bp_grid,
                               float *bp, uint32_t flags, void *stream);
+
+/* The backprojection scales the last OSEM step's projection stored in the
+ * plan's event records, out[e] = 1 / p_e for p_e > 1e-7, else 0 (the
+ * reference's ratio threshold, tor.cu.cc:203-206), in event order (n floats).
+ * Lets a caller read the projection of the untiled OSEM step without a
+ * second projection.  No reference counterpart (the reference's projection
+ * output is the tensor of op "Projection", tor.cc:122-128). */
+int srfx_tor_plan_scales(srfx_tor_plan plan, float *out, void *stream);
 
 /* ------------------------------------------------------------ Batched --- */
 

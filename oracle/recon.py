@@ -10,6 +10,8 @@ project_split / backproject_split  the SplitLoRsModel projection /
 osem                          ReconStep + mlem_update (recon_step.py:33-49)
                               over contiguous subsets (worker.py:79-85), or
                               PSFReconStep (recon_step.py:56-115) with psf.
+siddon_osem                   the same over CompleteLoRsModel (Siddon,
+                              complete_lors.py:59-98).
 """
 import numpy as np
 
@@ -101,4 +103,29 @@ def osem(groups, x0, emap, center, size, kw, geo, tof_bin, tof_sigma2, nb_iterat
             if psf is not None:
                 bp = OPSF.psf_backward(bp, *psf)
             x = x * emap * bp
+    return x
+
+
+def siddon_osem(lors, x0, emap, center, size, tof_bin, tof_sigma2, nb_iterations, nb_subsets,
+                weights=None, prec="f64"):
+    """CompleteLoRsModel + ReconStep + mlem_update (complete_lors.py:59-98,
+    recon_step.py:33-49) over contiguous subsets of the [7, N] Siddon rows.
+    The op sees the world image (nx, ny, nz) transposed [2, 1, 0], i.e. laid
+    out [gz][gy][gx] with grid (nx, ny, nz) and the world center and size
+    (complete_lors.py:61-70 transposes the image and reverses the reversed
+    geometry).  weights (counts per LOR, optional): lors_value = p / w, so the
+    backprojection adds w_i a_ij / p_i; without them lors_value = p."""
+    x = np.array(x0, np.float64)
+    grid = list(x.shape)
+    n = lors.shape[1]
+    for _ in range(nb_iterations):
+        for k in range(nb_subsets):
+            lo, hi = subset_bounds(n, k, nb_subsets)
+            sub = np.ascontiguousarray(lors[:, lo:hi])
+            xt = np.ascontiguousarray(np.transpose(x, (2, 1, 0))).astype(np.float32)
+            p = O.siddon_project(sub, xt, grid, center, size, tof_bin, tof_sigma2, prec=prec)
+            v = p if weights is None else p / np.asarray(weights[lo:hi], np.float64)
+            bp = O.siddon_backproject(sub, v.astype(np.float32), grid, center, size, tof_bin,
+                                      tof_sigma2, prec=prec)
+            x = x * emap * np.transpose(bp, (2, 1, 0))
     return x

@@ -89,6 +89,7 @@ _PROTOS = {
     "srfx_tor_maplors": [_P, _I64, _I64, _P, SrfxGrid, _F, _I32, _P, _U32, _P],
     "srfx_tor_osem_subset": [_P, _I64, _I64, _P, SrfxGrid, SrfxGrid, SrfxTorModel, _P, _U32, _P],
     "srfx_tor_count_nnz": [_P, _I64, _I64, SrfxGrid, SrfxTorModel, _P, _P, _P],
+    "srfx_tor_patch_info": [SrfxGrid, _F, ctypes.POINTER(_I32), ctypes.POINTER(_I32)],
     "srfx_siddon_project": [_P, _I64, _I64, _P, SrfxGrid, _F, _F, _P, _U32, _P],
     "srfx_siddon_backproject": [_P, _I64, _I64, _P, SrfxGrid, _F, _F, _P, _U32, _P],
     "srfx_siddon_backproject_adjoint": [_P, _I64, _I64, _P, SrfxGrid, _F, _F, _P, _U32, _P],
@@ -106,6 +107,7 @@ _PROTOS = {
     "srfx_tor_plan_backproject_adjoint": [_P, _P, _P, _U32, _P],
     "srfx_tor_plan_maplors": [_P, _P, _P, _U32, _P],
     "srfx_tor_plan_osem_subset": [_P, _P, SrfxGrid, _P, _U32, _P],
+    "srfx_tor_plan_scales": [_P, _P, _P],
 }
 
 _lib = None
@@ -166,3 +168,13 @@ def make_tor_model(kernel_width, geo_sigma2_flag, tof_bin, tof_sigma2):
     m.tof_bin = float(tof_bin)
     m.tof_sigma2 = float(tof_sigma2)
     return m
+
+
+def tor_patch_info(grid, size, kernel_width):
+    """(P, box) for a TOR op frame (grid (gx, gy, gz), full size in mm): the
+    reference's patch width P (tor.cu.cc:375) and the candidate box the
+    kernels run (3, 4, or 0 for the generic chord walk).  Host only."""
+    p, b = _I32(), _I32()
+    call("srfx_tor_patch_info", make_grid(grid, (0, 0, 0), size), float(kernel_width),
+         ctypes.byref(p), ctypes.byref(b))
+    return int(p.value), int(b.value)

@@ -100,12 +100,15 @@ class BatchedOSEM:
 
     def set_schedule(self, lanes_per_event, events_per_item):
         """Schedule of the persistent projection (srfx_tor_batch_set_schedule):
-        lanes per event in LANES, events per work item in ITEM_EVENTS.  Set it
-        before capture(): a graph keeps the kernel it recorded."""
+        lanes per event in LANES, events per work item in ITEM_EVENTS.  A
+        graph keeps the kernel and grid it recorded, so a captured graph is
+        dropped: replay() then raises until capture() records the new
+        schedule."""
         _check(self.lib, "srfx_tor_batch_set_schedule",
                self.lib.srfx_tor_batch_set_schedule(self.handle, int(lanes_per_event),
                                                     int(events_per_item)))
         self.schedule = (int(lanes_per_event), int(events_per_item))
+        self.graph = None
 
     def autotune(self, reps=2):
         """Time one OSEM iteration (all subsets) for every schedule in
@@ -160,6 +163,8 @@ class BatchedOSEM:
         return g
 
     def replay(self, nb_iterations=1):
+        if self.graph is None:
+            raise RuntimeError("no captured graph for the current schedule: call capture()")
         for _ in range(nb_iterations):
             self.graph.replay()
 

@@ -469,6 +469,28 @@ extern "C" int srfx_tor_plan_maplors(srfx_tor_plan plan, const float *lors_value
     return plan_scatter_entry(plan, kMap, lors_value, image, flags, stream);
 }
 
+namespace {
+// out[e] = the value slot of event e's walk record.
+__global__ __launch_bounds__(256) void record_values(const float4 *__restrict__ rec, int64_t n,
+                                                     float *__restrict__ out)
+{
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < n)
+        out[e] = reinterpret_cast<const float *>(rec)[4 * kWalkHead * e + kWalkValue];
+}
+} // namespace
+
+extern "C" int srfx_tor_plan_scales(srfx_tor_plan plan, float *out, void *stream)
+{
+    if (!plan || plan->cols7 || (plan->p.n > 0 && !out))
+        return SRFX_EINVAL;
+    if (plan->p.n == 0)
+        return SRFX_OK;
+    hipLaunchKernelGGL(record_values, dim3(blocks_for(plan->p.n, 256)), dim3(256), 0,
+                       (hipStream_t)stream, plan->rec, plan->p.n, out);
+    return launch_status();
+}
+
 extern "C" int srfx_tor_plan_osem_subset(srfx_tor_plan plan, const float *image,
                                          srfx_grid bp_grid, float *bp, uint32_t flags,
                                          void *stream)

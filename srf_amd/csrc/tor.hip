@@ -101,18 +101,22 @@ extern "C" int srfx_tor_maplors(const float *lors, int64_t n, int64_t ld, const 
                               model_no_tof(kernel_width, geo_sigma2_flag), image, flags, stream);
 }
 
-extern "C" int srfx_tor_osem_subset(const float *lors, int64_t n, int64_t ld, const float *image,
-                                    srfx_grid image_grid, srfx_grid bp_grid, srfx_tor_model m,
-                                    float *bp, uint32_t flags, void *stream)
+// Arguments of srfx_tor_osem_subset checked and set up, bp zeroed unless
+// accumulating.  nslab = 0 when the flags name no slab schedule.
+static int osem_subset_setup(TorParams &p, const float *lors, int64_t n, int64_t ld,
+                             const float *image, const srfx_grid &image_grid,
+                             const srfx_grid &bp_grid, const srfx_tor_model &m, float *bp,
+                             uint32_t flags, hipStream_t st, int &nslab)
 {
-    TorParams p;
     int rc = setup(p, lors, n, ld, image_grid, m.kernel_width, m.geo_sigma2_flag, m.tof_bin,
                    m.tof_sigma2);
     if (rc)
         return rc;
     if (!grid_ok(bp_grid) || !same_frame(image_grid, bp_grid) || !bp || (n > 0 && !image))
         return SRFX_EINVAL;
-    hipStream_t st = (hipStream_t)stream;
+    int inter;
+    if (!slab_schedule(flags, nslab, inter))
+        nslab = 0;
     rc = zero_grid(bp, bp_grid, flags, st);
     if (rc)
         return rc;
@@ -123,21 +127,40 @@ extern "C" int srfx_tor_osem_subset(const float *lors, int64_t n, int64_t ld, co
     p.full_patch = (flags & SRFX_FULL_PATCH) ? 1 : 0;
     p.img = image;
     p.out = bp;
-    int nslab, inter;
-    if (!slab_schedule(flags, nslab, inter) || n == 0)
+    return SRFX_OK;
+}
+
+// Slab schedule of the OSEM pass: the per-slab projections into part (nslab
+// x n floats, every one written before it is read), then the scatter with
+// their sum.
+static int osem_subset_slabs(const TorParams &p, float *part, uint32_t flags, hipStream_t st)
+{
+    int rc = launch_slab<kProj>(p, part, flags, st);
+    return rc ? rc : launch_slab<kOsem>(p, part, flags, st);
+}
+
+extern "C" int srfx_tor_osem_subset(const float *lors, int64_t n, int64_t ld, const float *image,
+                                    srfx_grid image_grid, srfx_grid bp_grid, srfx_tor_model m,
+                                    float *bp, uint32_t flags, void *stream)
+{
+    TorParams p;
+    hipStream_t st = (hipStream_t)stream;
+    int nslab = 0;
+    int rc = osem_subset_setup(p, lors, n, ld, image, image_grid, bp_grid, m, bp, flags, st,
+                               nslab);
+    if (rc)
+        return rc;
+    if (nslab == 0 || n == 0)
         return launch<kOsem>(p, flags, st);
-    // Slab schedule: two passes, the per-slab projections into a workspace,
-    // then the scatter with their sum.
     void *ws = nullptr;
     rc = ws_alloc(&ws, sizeof(float) * (size_t)nslab * (size_t)n, st);
     if (rc)
         return rc;
-    rc = launch_slab<kProj>(p, (float *)ws, flags, st);
-    if (!rc)
-        rc = launch_slab<kOsem>(p, (float *)ws, flags, st);
+    rc = osem_subset_slabs(p, (float *)ws, flags, st);
     int rf = ws_free(ws, st);
     return rc ? rc : rf;
 }
+
 
 extern "C" int srfx_tor_count_nnz(const float *lors, int64_t n, int64_t ld, srfx_grid grid,
                                   srfx_tor_model m, int32_t *nnz_per_event, uint64_t *nnz_total,
@@ -151,4 +174,19 @@ extern "C" int srfx_tor_count_nnz(const float *lors, int64_t n, int64_t ld, srfx
     p.nnz_total = (unsigned long long *)nnz_total;
     p.full_patch = 0;
     return launch<kCount>(p, 0u, (hipStream_t)stream);
+}
+
+extern "C" int srfx_tor_patch_info(srfx_grid grid, float kernel_width, int32_t *patch,
+                                   int32_t *box)
+{
+    TorParams p;
+    const float one = 1.0f;
+    int rc = setup(p, &one, 0, 0, grid, kernel_width, 0, 0.0f, 1.0e6f);
+    if (rc)
+        return rc;
+    if (patch)
+        *patch = p.patch;
+    if (box)
+        *box = p.box;
+    return SRFX_OK;
 }

@@ -89,6 +89,21 @@ def _image_like(image, grid):
     return image.contiguous()
 
 
+def _out_image(out, grid, like):
+    """A caller-supplied output image: a float32 device tensor on the input's
+    device, contiguous, with exactly the grid's voxel count (the kernels write
+    gx * gy * gz floats from its first element)."""
+    _check_dev(out, "out")
+    n = grid[0] * grid[1] * grid[2]
+    if out.device != like.device:
+        raise ValueError(f"out is on {out.device}, the inputs on {like.device}")
+    if not out.is_contiguous():
+        raise ValueError("out must be contiguous")
+    if out.numel() != n:
+        raise ValueError(f"out has {out.numel()} voxels, grid {grid} needs {n}")
+    return out
+
+
 def _ptr(t):
     return ctypes_void(t.data_ptr()) if t is not None else None
 
@@ -170,6 +185,8 @@ def osem_subset(lors, image, grid, center, size, kernel_width, geo_sigma2_flag=F
     if out is None:
         out = torch.empty_like(image, memory_format=torch.contiguous_format)
         accumulate = False
+    else:
+        out = _out_image(out, g, image)
     m = _lib.make_tor_model(kernel_width, geo_sigma2_flag, tof_bin, tof_sigma2)
     _lib.call("srfx_tor_osem_subset", _ptr(lors), n, ld, _ptr(image), cg, cg, m, _ptr(out),
               (_lib.SRFX_ACCUMULATE if accumulate else 0) | _lib.schedule_flags(schedule),
@@ -246,6 +263,8 @@ def siddon_osem_subset(lors, image, grid, center, size, tof_bin=1.0e4, tof_sigma
     if out is None:
         out = torch.empty_like(image, memory_format=torch.contiguous_format)
         accumulate = False
+    else:
+        out = _out_image(out, g, image)
     _lib.call("srfx_siddon_osem_subset", _ptr(lors), n, ld, _ptr(image), cg, float(tof_bin),
               float(tof_sigma2), _ptr(out), _lib.SRFX_ACCUMULATE if accumulate else 0, _stream())
     return out

@@ -331,6 +331,20 @@ class CompleteLoRsModel:
                       self.tof_sigma2, _ptr(out), _lib.SRFX_ACCUMULATE, _stream())
         return Image(out, image.center, image.size)
 
+    def osem_subset(self, image, data, out=None):
+        """Projection + ratio backprojection (recon_step.py:33-41 over
+        complete_lors.py:59-98) in one fused kernel (srfx_siddon_osem_subset):
+        out += A^T (1 / A x), events with A x = 0 left out as the reference's
+        backprojection does (siddon.cu.cc:363-371)."""
+        x = _world(image)
+        if out is None:
+            out = torch.zeros_like(x)
+        lors, n, ld = _soa(data.soa, 7)
+        if n:
+            _lib.call("srfx_siddon_osem_subset", _ptr(lors), n, ld, _ptr(x), self._grid(image),
+                      self.tof_bin, self.tof_sigma2, _ptr(out), _lib.SRFX_ACCUMULATE, _stream())
+        return Image(out, image.center, image.size)
+
     def maplors(self, data, image, out=None):
         x = _world(image)
         if out is None:

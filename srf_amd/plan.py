@@ -129,6 +129,14 @@ class TorPlan:
                   _stream())
         return out
 
+    def scales(self):
+        """The backprojection scales the last OSEM step stored in the event
+        records: 1/p for p > 1e-7, else 0 (float32 [n])."""
+        out = torch.empty(self.n, dtype=torch.float32,
+                          device=self.lors.device)
+        _lib.call("srfx_tor_plan_scales", self._h, _ptr(out), _stream())
+        return out
+
     def close(self):
         if getattr(self, "_h", None) and self._h.value:
             _lib.call("srfx_tor_plan_destroy", self._h)

@@ -20,7 +20,7 @@ OSEM take it as ``psf=``.
 import torch
 
 from . import ops
-from .data import Image, ListModeDataSplit
+from .data import Image, ListModeData, ListModeDataSplit
 
 
 def subset_bounds(n, k, nb_subsets):
@@ -29,8 +29,19 @@ def subset_bounds(n, k, nb_subsets):
     return (k * n) // nb_subsets, ((k + 1) * n) // nb_subsets
 
 
+def _subset_part(lm, k, nb_subsets):
+    lo, hi = subset_bounds(len(lm), k, nb_subsets)
+    part = lm.replace_values(lm.values[lo:hi] if lm.values is not None else None)
+    part.soa = lm.soa[:, lo:hi]
+    return part
+
+
 def subset_view(data, k, nb_subsets):
-    """Column views (no copy) of each group's SoA LORs for subset k."""
+    """Column views (no copy) of each group's SoA LORs for subset k (a
+    ListModeDataSplit of the TOR model), or of the LORs of a single
+    ListModeData (the Siddon model's complete LORs)."""
+    if isinstance(data, ListModeData):
+        return _subset_part(data, k, nb_subsets)
     parts = []
     for a in ("x", "y", "z"):
         lm = data[a]

@@ -129,3 +129,78 @@ def test_projection_schedules_equal_default(cuda):
         assert H.rel_l2(rt["image"].data.cpu().numpy(), rr["image"].data.cpu().numpy()) <= 1e-5
     tb.close()
     rb.close()
+
+
+# The batched workload's three image kinds (kernel_width 4: box 3) and the
+# large frame's pitch (box 4), each a separate reconstruction of its own
+# events, as the reference reconstructs one image per run
+# (graph/reconstruction/local_reconstruction.py:44-51).
+KINDS_KW4 = [((96, 96, 48), (384.0, 384.0, 216.0), 150.0),
+             ((128, 128, 64), (384.0, 384.0, 217.6), 150.0),
+             ((200, 200, 100), (400.0, 400.0, 220.0), 150.0),
+             ((64, 64, 40), (102.4, 102.4, 68.0), 45.0)]
+
+
+def _recons_kw4(cuda, seed=7):
+    from srf_amd import preprocess, synth
+    from srf_amd.data import Image, split_from_groups
+    from srf_amd.scanner import CylindricalScanner
+    sc = CylindricalScanner()
+    rng = np.random.default_rng(seed)
+    out = []
+    for r, (shape, size, radius) in enumerate(KINDS_KW4 + KINDS_KW4[:1]):
+        n = int(rng.integers(6000, 16000))
+        ph = synth.Phantom(radius, 0.45 * size[2] if radius < 100 else sc.axial_length / 2,
+                           [((0.3 * radius, -0.2 * radius, 0.0), 0.15 * radius, 4.0)])
+        ev, _ = synth.tof_events(n, ph, seed=30 + r, scanner=sc, device=cuda)
+        groups = preprocess.recon_process(ev, sc.tof_res_ps)
+        data = split_from_groups(groups)
+        x = torch.ones(shape, device=cuda)
+        e = torch.from_numpy(rng.uniform(0.5, 1.5, shape).astype(np.float32)).to(cuda)
+        out.append({"image": Image(x, (0, 0, 0), size), "emap": Image(e, (0, 0, 0), size),
+                    "data": data, "groups": {a: groups[a].cpu().numpy() for a in "xyz"}})
+    return out, sc
+
+
+def test_batched_graph_vs_oracle(cuda):
+    """Mixed reconstructions at the batched workload's pitches and the large
+    frame's, kernel_width 4, 2 iterations x 8 subsets, run as one captured HIP
+    graph per iteration: each image equals the oracle's OSEM of that
+    reconstruction alone (rel L2 <= 1e-4)."""
+    from oracle import recon as R
+    from srf_amd import _lib
+    from srf_amd.batch import BatchedOSEM
+    recons, sc = _recons_kw4(cuda)
+    tb, ts2 = sc.tof_info()
+    boxes = {_lib.tor_patch_info(*_frame(rc), 4.0)[1] for rc in recons}
+    assert boxes == {3, 4}
+    model = _model_kw(sc, 4.0)
+    start = [rc["image"].data.clone() for rc in recons]
+    bat = BatchedOSEM(recons, model, nb_subsets=8)
+    bat.capture()
+    for rc, x0 in zip(recons, start):  # capture records without running
+        rc["image"].data.copy_(x0)
+    bat.replay(2)
+    torch.cuda.synchronize()
+    for r, (rc, x0) in enumerate(zip(recons, start)):
+        ref = R.osem(rc["groups"], x0.cpu().numpy(), rc["emap"].data.cpu().numpy().astype(
+            np.float64), (0, 0, 0), rc["image"].size, 4.0, False, tb, ts2, 2, 8)
+        e = H.rel_l2(rc["image"].data.cpu().numpy(), ref)
+        n = sum(rc["groups"][a].shape[1] for a in "xyz")
+        print(f"batched recon {r} {tuple(rc['image'].data.shape)}: {n} events, "
+              f"rel_l2 vs oracle {e:.2e}")
+        assert e <= 1e-4
+    bat.close()
+
+
+def _frame(rc):
+    from srf_amd import physics
+    x = rc["image"].data
+    g, c, s, _ = physics.group_frame(x.shape, (0, 0, 0), rc["image"].size, (1, 1, 1), "x")
+    return g, s
+
+
+def _model_kw(sc, kw):
+    from srf_amd import physics
+    tb, ts2 = sc.tof_info()
+    return physics.SplitLoRsModel(kw, False, tof_sigma2=ts2, tof_bin=tb)

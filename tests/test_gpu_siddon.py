@@ -101,3 +101,31 @@ def test_osem_fused(cuda, case):
     b = ops.siddon_backprojection(X, L, p, *GEO, 6.0, 1.0e6)
     f = ops.siddon_osem_subset(L, X, *GEO, 6.0, 1.0e6)
     assert H.rel_l2(f.cpu().numpy(), b.cpu().numpy()) <= 1e-6
+
+
+@pytest.mark.parametrize("fused", [False, True])
+def test_osem_vs_oracle(cuda, fused):
+    """Non-TOF Siddon OSEM, 2 iterations x 4 subsets, through the
+    reconstruction driver on the Siddon model (complete_lors.py:59-98 +
+    recon_step.py:33-49: world image, [2, 1, 0] frame, contiguous subsets)
+    against the oracle's OSEM: rel L2 <= 1e-4."""
+    import torch
+    from oracle import recon as R
+    from srf_amd import physics
+    from srf_amd.data import Image, ListModeData
+    from srf_amd.recon import OSEM
+    grid, center, size = (32, 32, 16), (0.5, -1.0, 0.25), (96.0, 96.0, 54.4)  # 3x3x3.4 mm
+    lors = H.siddon_lors(12000, seed=17, grid=grid, center=center, size=size)
+    rng = np.random.default_rng(18)
+    emap = rng.uniform(0.5, 1.5, grid).astype(np.float32)
+    x0 = np.ones(grid, np.float32)
+    model = physics.CompleteLoRsModel("siddon", tof_sigma2=1.0e6, tof_bin=6.0)
+    data = ListModeData(_t(lors, cuda), soa=True)
+    x = _t(x0, cuda)
+    osem = OSEM(model, data, Image(_t(emap, cuda), center, size), nb_subsets=4, fused=fused)
+    osem.iterate(Image(x, center, size), 2)
+    torch.cuda.synchronize()
+    ref = R.siddon_osem(lors, x0, emap.astype(np.float64), center, size, 6.0, 1.0e6, 2, 4)
+    e = H.rel_l2(x.cpu().numpy(), ref)
+    print(f"Siddon OSEM 2 it x 4 subsets fused={fused}: {lors.shape[1]} LORs, rel_l2 {e:.2e}")
+    assert e <= 1e-4

@@ -172,3 +172,68 @@ def test_fp32_port_error_level():
     b = O.tor_project(lors, x, *args, prec="f32")
     err = H.rel_l2(b, a)
     assert 1e-7 < err < 1e-4
+
+
+def brute_tor_count(lors, grid, center, size, kw, geo):
+    """Per-event number of kept (event, voxel) pairs from the brute-force
+    geometry above (parametric crossing, cross-product distance): strictly
+    crossed slice, voxel in the grid and in the P x P patch, d^2 < 9 sigma_c^2."""
+    g = np.array(grid)
+    c = np.array(center, np.float32).astype(np.float64)
+    s = np.array(size, np.float32).astype(np.float64)
+    inter = s / g
+    lb = c - s / 2
+    kw = float(np.float32(kw))
+    sig2 = kw * kw / 36
+    P = math.ceil((math.sqrt(2) * kw + inter[2]) / inter[0])
+    zs = c[2] - (s[2] - inter[2]) / 2 + np.arange(g[2]) * inter[2]
+    out = np.zeros(lors.shape[1], np.int64)
+    for e in range(lors.shape[1]):
+        col = lors[:, e].astype(np.float64)
+        p1, p2, f = col[0:3], col[3:6], col[9]
+        d = p2 - p1
+        u = d / np.linalg.norm(d)
+        sc = sig2 * (f if geo else 1.0)
+        if d[2] == 0:
+            continue
+        for z in zs:
+            t = (z - p1[2]) / d[2]
+            if not (0 < t < 1):
+                continue
+            X = p1 + t * d
+            ix0 = int((X[0] - lb[0]) / inter[0]) - P // 2
+            iy0 = int((X[1] - lb[1]) / inter[1]) - P // 2
+            for iy in range(max(iy0, 0), min(iy0 + P, g[1])):
+                for ix in range(max(ix0, 0), min(ix0 + P, g[0])):
+                    m = np.array([lb[0] + (ix + 0.5) * inter[0], lb[1] + (iy + 0.5) * inter[1], z])
+                    if float(np.sum(np.cross(m - p1, u) ** 2)) < 9 * sc:
+                        out[e] += 1
+    return out
+
+
+# the SMALL box at 2.0 x 2.0 x 2.2 mm (patch 4) and at 1.6 x 1.6 x 1.7 mm (patch 5)
+SMALL_16 = ((12, 10, 8), (0.5, -0.25, 0.3), (19.2, 16.0, 13.6))
+
+
+@pytest.mark.parametrize("box", [SMALL, SMALL_16])
+@pytest.mark.parametrize("geo", [False, True])
+def test_count_brute_force_support(box, geo):
+    """tor_count_f64 equals the brute-force support, event by event, and
+    every counted pair carries a non-zero weight in the projection of a
+    ones image (the support of the projection)."""
+    grid, center, size = box
+    lors = small_lors(120, seed=31)
+    if box is SMALL_16:
+        # the same LOR shapes scaled into the smaller box
+        c = np.array(center, np.float32)[:, None]
+        k = np.array([0.8, 0.8, 0.7727], np.float32)[:, None]
+        for r in (0, 3, 6):
+            lors[r:r + 3] = (lors[r:r + 3] - c) * k + c
+    ref = brute_tor_count(lors, grid, center, size, 4.0, geo)
+    got = O.tor_count(lors, grid, center, size, 4.0, geo, prec="f64")
+    assert ref.sum() > 500
+    np.testing.assert_array_equal(got, ref)
+    # an event with a zero count projects a ones image to zero, and back
+    ones = np.ones(grid[::-1], np.float32)
+    p = O.tor_project(lors, ones, grid, center, size, 4.0, geo, 3.0, 1.0e6, prec="f64")
+    assert np.all((p > 0) == (ref > 0))
