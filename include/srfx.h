@@ -47,7 +47,8 @@ enum srfx_status {
     SRFX_EINVAL = 1,      /* bad argument: null pointer, n < 0, ld < n, grid <= 0, size <= 0 */
     SRFX_ELAUNCH = 2,     /* a HIP call or kernel launch failed */
     SRFX_EUNSUPPORTED = 3, /* valid but unsupported (e.g. TOR patch wider than 16 voxels) */
-    SRFX_ENOMEM = 4        /* device workspace allocation failed */
+    SRFX_ENOMEM = 4,       /* device workspace allocation failed */
+    SRFX_ETOOLARGE = 5     /* input larger than a no-allocation entry point's static scratch */
 };
 
 #define SRFX_ACCUMULATE 1u
@@ -163,6 +164,21 @@ int srfx_tor_count_nnz(const float *lors, int64_t n, int64_t ld, srfx_grid grid,
                        srfx_tor_model model, int32_t *nnz_per_event, uint64_t *nnz_total,
                        void *stream);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): srfx_tor_osem_subset under a slab schedule (the flags'
+ * SRFX_SLAB_* field, SRFX_SLAB_XCD8 when it names none), with the per-slab
+ * partial projections in a static scratch of the library (16 MiB per device,
+ * 2^22 floats, kept until the process ends) instead of a workspace from the
+ * device's memory pool.  Calls no allocating routine.  Limit: slabs x n <=
+ * 2^22 (n <= 524288 events at 8 slabs, 262144 at 16); larger inputs return
+ * SRFX_ETOOLARGE before any device work (split them and pass
+ * SRFX_ACCUMULATE).  Every partial is written before it is read, so nothing
+ * needs clearing between calls; calls that overlap on one device (other
+ * streams or host threads) share the scratch and race. */
+int srfx_tor_osem_subset_noalloc(const float *lors, int64_t n, int64_t ld, const float *image,
+                                 srfx_grid image_grid, srfx_grid bp_grid, srfx_tor_model model,
+                                 float *bp, uint32_t flags, void *stream);
 
 /* The reference's TOR patch width for a grid and kernel width, P =
  * ceil((sqrt(2) kernel_width + inter_z) / inter_x) (tor.cu.cc:369-375), and

@@ -15,6 +15,9 @@ SRFX_EINVAL = 1
 SRFX_ELAUNCH = 2
 SRFX_EUNSUPPORTED = 3
 SRFX_ENOMEM = 4
+SRFX_ETOOLARGE = 5
+# floats of the static scratch of srfx_tor_osem_subset_noalloc (slabs x events)
+NOALLOC_SCRATCH_FLOATS = 1 << 22
 SRFX_ACCUMULATE = 1
 SRFX_FULL_PATCH = 2
 SRFX_LPE_SHIFT = 4
@@ -89,6 +92,8 @@ _PROTOS = {
     "srfx_tor_maplors": [_P, _I64, _I64, _P, SrfxGrid, _F, _I32, _P, _U32, _P],
     "srfx_tor_osem_subset": [_P, _I64, _I64, _P, SrfxGrid, SrfxGrid, SrfxTorModel, _P, _U32, _P],
     "srfx_tor_count_nnz": [_P, _I64, _I64, SrfxGrid, SrfxTorModel, _P, _P, _P],
+    "srfx_tor_osem_subset_noalloc": [_P, _I64, _I64, _P, SrfxGrid, SrfxGrid, SrfxTorModel, _P,
+                                     _U32, _P],
     "srfx_tor_patch_info": [SrfxGrid, _F, ctypes.POINTER(_I32), ctypes.POINTER(_I32)],
     "srfx_siddon_project": [_P, _I64, _I64, _P, SrfxGrid, _F, _F, _P, _U32, _P],
     "srfx_siddon_backproject": [_P, _I64, _I64, _P, SrfxGrid, _F, _F, _P, _U32, _P],

@@ -1,5 +1,7 @@
 // TOF tube-of-response (TOR) projector entry points (C ABI, include/srfx.h).
 // The device code is in tor_core.h.
+#include <mutex>
+
 #include "tor_core.h"
 
 using namespace srfx;
@@ -161,6 +163,65 @@ extern "C" int srfx_tor_osem_subset(const float *lors, int64_t n, int64_t ld, co
     return rc ? rc : rf;
 }
 
+// Scratch of srfx_tor_osem_subset_noalloc: the per-slab partial projections,
+// one copy per device (a module global), 16 MiB.  Host code never passes the
+// array's name to a kernel (on the host it is only a handle of the symbol);
+// it passes the device address hipGetSymbolAddress returns, looked up once
+// per device on the entry point's first call there.
+constexpr int64_t kScratchFloats = (int64_t)1 << 22;
+__device__ __align__(256) float scratch[kScratchFloats];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex scratch_mu;
+float *scratch_dev[kMaxDevices];
+
+// Device address of the scratch on the current device.
+int scratch_address(float **out)
+{
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess)
+        return SRFX_ELAUNCH;
+    if (dev < 0 || dev >= kMaxDevices)
+        return SRFX_EUNSUPPORTED;
+    std::lock_guard<std::mutex> lock(scratch_mu);
+    if (!scratch_dev[dev]) {
+        void *ptr = nullptr;
+        if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr) {
+            (void)hipGetLastError();
+            return SRFX_ELAUNCH;
+        }
+        scratch_dev[dev] = (float *)ptr;
+    }
+    *out = scratch_dev[dev];
+    return SRFX_OK;
+}
+} // namespace
+
+extern "C" int srfx_tor_osem_subset_noalloc(const float *lors, int64_t n, int64_t ld,
+                                            const float *image, srfx_grid image_grid,
+                                            srfx_grid bp_grid, srfx_tor_model m, float *bp,
+                                            uint32_t flags, void *stream)
+{
+    // the slab schedule of the flags, SRFX_SLAB_XCD8 when they name none
+    if (((flags >> SRFX_SLAB_SHIFT) & 3u) == 0)
+        flags |= SRFX_SLAB_XCD8;
+    int nslab = 0, inter = 0;
+    (void)slab_schedule(flags, nslab, inter);
+    // size checks before any device work
+    if (n > 0 && (int64_t)nslab * n > kScratchFloats)
+        return SRFX_ETOOLARGE;
+    float *part = nullptr;
+    int rc = scratch_address(&part);
+    if (rc)
+        return rc;
+    TorParams p;
+    hipStream_t st = (hipStream_t)stream;
+    rc = osem_subset_setup(p, lors, n, ld, image, image_grid, bp_grid, m, bp, flags, st, nslab);
+    if (rc || n == 0)
+        return rc;
+    return osem_subset_slabs(p, part, flags, st);
+}
 
 extern "C" int srfx_tor_count_nnz(const float *lors, int64_t n, int64_t ld, srfx_grid grid,
                                   srfx_tor_model m, int32_t *nnz_per_event, uint64_t *nnz_total,

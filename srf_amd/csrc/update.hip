@@ -130,6 +130,8 @@ extern "C" const char *srfx_strerror(int status)
         return "unsupported configuration";
     case SRFX_ENOMEM:
         return "device workspace allocation failed";
+    case SRFX_ETOOLARGE:
+        return "input larger than the entry point's static scratch";
     default:
         return "unknown status";
     }
