@@ -39,28 +39,36 @@ def _newer(target, deps):
     return any(os.path.getmtime(d) > t for d in deps)
 
 
-def build(force=False, verbose=False):
-    os.makedirs(BUILD, exist_ok=True)
-    headers = [os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".h")]
+def build(force=False, verbose=False, csrc=CSRC, lib=LIB, jobs=None):
+    """Compile every .hip file of ``csrc`` (in parallel, at most ``jobs``
+    compilers at once) and link them into ``lib``."""
+    from concurrent.futures import ThreadPoolExecutor
+    build_dir = os.path.join(csrc, "_build")
+    os.makedirs(build_dir, exist_ok=True)
+    headers = [os.path.join(csrc, f) for f in os.listdir(csrc) if f.endswith(".h")]
     headers.append(os.path.join(INCLUDE, "srfx.h"))
-    objs = []
-    for src in sources():
-        s = os.path.join(CSRC, src)
-        o = os.path.join(BUILD, src[:-4] + ".o")
+    objs, cmds = [], []
+    for src in sorted(f for f in os.listdir(csrc) if f.endswith(".hip")):
+        s = os.path.join(csrc, src)
+        o = os.path.join(build_dir, src[:-4] + ".o")
         objs.append(o)
         if force or _newer(o, [s] + headers):
-            cmd = [HIPCC] + HIP_FLAGS + ["-c", s, "-o", o]
-            if verbose:
-                print(" ".join(cmd), flush=True)
-            subprocess.run(cmd, check=True)
-    if force or _newer(LIB, objs):
-        tmp = LIB + ".tmp"
-        cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs
+            cmds.append([HIPCC] + HIP_FLAGS + ["-c", s, "-o", o])
+
+    def run(cmd):
         if verbose:
             print(" ".join(cmd), flush=True)
         subprocess.run(cmd, check=True)
-        os.replace(tmp, LIB)
-    return LIB
+
+    if cmds:
+        jobs = jobs or min(len(cmds), max(1, min(8, os.cpu_count() or 1)))
+        with ThreadPoolExecutor(max_workers=jobs) as pool:
+            list(pool.map(run, cmds))
+    if force or _newer(lib, objs):
+        tmp = lib + ".tmp"
+        run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs)
+        os.replace(tmp, lib)
+    return lib
 
 
 if __name__ == "__main__":

@@ -56,6 +56,25 @@ __device__ __forceinline__ int trunc_clamped(double u)
     return (int)fmin(fmax(u, -1.0e9), 1.0e9);
 }
 
+// Slack (voxels) on the candidate extents below: the walks form their
+// offsets in FP32, ~1e-6 voxel from the FP64 track.
+constexpr double kExtentSlack = 1.0e-3;
+
+// Half extents, in voxels, of the padded tube ellipse of a slice around the
+// crossing point (u, w): a voxel the walks keep (d2 < 9 sigma_c^2) or decide
+// in FP64 (within 1e-4 mm^2 of the cut-off) has its centre inside that
+// ellipse (cut-off tm, make_track), so its column i and row j satisfy
+// |i + 0.5 - u| <= hx / inter_x and |j + 0.5 - w| <= dymax / inter_y.
+// Voxels of the candidate box outside the ellipse add nothing.
+__device__ __forceinline__ double col_extent(const TorParams &p, const Track &t)
+{
+    return (double)t.hx * p.rinter_x + kExtentSlack;
+}
+__device__ __forceinline__ double row_extent(const TorParams &p, const Track &t)
+{
+    return (double)t.dymax * p.rinter_y + kExtentSlack;
+}
+
 // Windows wx0..wx1, wy0..wy1 holding every voxel the event can visit in the
 // slices s0..s1.  The patch origin of slice s is (int)u(s) - half with u
 // affine in s (cross_at), or the reference's exact origin, which differs from
@@ -69,6 +88,17 @@ __device__ __forceinline__ bool footprint(const TorParams &p, const TileGeom &g,
     int b1 = trunc_clamped(t.ay + (double)s1 * t.by) - p.half;
     int x0 = max(min(a0, a1) - 1, 0), x1 = min(max(a0, a1) + p.patch, p.gx - 1);
     int y0 = max(min(b0, b1) - 1, 0), y1 = min(max(b0, b1) + p.patch, p.gy - 1);
+    // and the columns / rows the padded ellipse covers over the slab (the
+    // only ones the walks add to; u and w are affine in s)
+    {
+        const double u0 = t.ax + (double)s0 * t.bx, u1 = t.ax + (double)s1 * t.bx;
+        const double w0 = t.ay + (double)s0 * t.by, w1 = t.ay + (double)s1 * t.by;
+        const double ex = col_extent(p, t), ey = row_extent(p, t);
+        x0 = max(x0, trunc_clamped(ceil(fmin(u0, u1) - 0.5 - ex)));
+        x1 = min(x1, trunc_clamped(floor(fmax(u0, u1) - 0.5 + ex)));
+        y0 = max(y0, trunc_clamped(ceil(fmin(w0, w1) - 0.5 - ey)));
+        y1 = min(y1, trunc_clamped(floor(fmax(w0, w1) - 0.5 + ey)));
+    }
     if (x0 > x1 || y0 > y1)
         return false;
     wx0 = x0 / g.Wx;
@@ -130,46 +160,49 @@ __host__ __device__ inline bool flush_y_fastest(const TorParams &p)
 // (YF) y fastest, so that a flush along the tile's fastest axis writes
 // consecutive addresses of an output whose unit stride is along op y.
 
-// Narrow [s0, s1] to a superset of the slices whose patch can meet the
-// window columns x0..x1 and rows y0..y1.  The patch origin is trunc(u) - half
-// (or the exact origin, one voxel away) with u(s) = ax + s bx, so the patch
-// meets the columns only if x0 - P + half - 2 < u(s) < x1 + half + 2; the
-// bounds on s get one more slice of slack on each side.  The quotients are
-// formed in FP32 (approximate reciprocal): their error, a few 1e-7 of the
-// slice index, is far inside that slack, and slices outside the bounds
-// only cost work (the walk clips every voxel to the window).
+// Narrow [s0, s1] to the slices s with lo <= u(s) <= hi, u(s) = a + s b
+// (FP64; the bounds on s are widened by 1e-6 slice, so the result is a
+// superset).
 __device__ __forceinline__ void axis_slices(double a, double b, double lo, double hi, int &s0,
                                             int &s1)
 {
-    if (fabs(b) < 1.0e-9) {
-        // u is constant to 1e-9 voxel per slice: test it at both ends
+    if (fabs(b) < 1.0e-12) {
+        // u is constant to 1e-12 voxel per slice: test it at both ends
         const double u0 = a + (double)s0 * b, u1 = a + (double)s1 * b;
-        if (fmax(u0, u1) <= lo - 1.0 || fmin(u0, u1) >= hi + 1.0)
+        if (fmax(u0, u1) < lo - 1.0e-6 || fmin(u0, u1) > hi + 1.0e-6)
             s1 = s0 - 1;
         return;
     }
-    const float r = __builtin_amdgcn_rcpf((float)b);
-    const float t1 = (float)(lo - a) * r, t2 = (float)(hi - a) * r;
-    const float tl = fmaxf(fminf(t1, t2), -1.0e8f), th = fminf(fmaxf(t1, t2), 1.0e8f);
-    s0 = max(s0, (int)floorf(tl) - 1);
-    s1 = min(s1, (int)ceilf(th) + 1);
+    const double t1 = (lo - a) / b, t2 = (hi - a) / b;
+    const double tl = fmax(fmin(t1, t2), -1.0e8), th = fmin(fmax(t1, t2), 1.0e8);
+    s0 = max(s0, (int)ceil(tl - 1.0e-6));
+    s1 = min(s1, (int)floor(th + 1.0e-6));
 }
 
-__device__ __forceinline__ void window_slices(const Track &t, int half, int P, int x0, int x1,
+// Narrow [s0, s1] to a superset of the slices at which the walk can add to
+// a voxel of columns x0..x1 and rows y0..y1: u(s) within the column extent
+// of the window, [x0 + 0.5 - hx', x1 + 0.5 + hx'], and w(s) within the row
+// extent; and the patch [trunc(u) - half, trunc(u) - half + P) meeting it
+// (2 voxels of slack for the exact origin).
+__device__ __forceinline__ void window_slices(const TorParams &p, const Track &t, int x0, int x1,
                                               int y0, int y1, int &s0, int &s1)
 {
+    const double ex = col_extent(p, t), ey = row_extent(p, t);
+    axis_slices(t.ax, t.bx, x0 + 0.5 - ex, x1 + 0.5 + ex, s0, s1);
+    axis_slices(t.ay, t.by, y0 + 0.5 - ey, y1 + 0.5 + ey, s0, s1);
+    const int P = p.patch, half = p.half;
     axis_slices(t.ax, t.bx, (double)(x0 - P + half - 2), (double)(x1 + half + 2), s0, s1);
     axis_slices(t.ay, t.by, (double)(y0 - P + half - 2), (double)(y1 + half + 2), s0, s1);
 }
 
 // The slices s0..s1 a listing walks: the event's crossed slices in the
-// tile's slab, narrowed to those whose patch can meet the window.
+// tile's slab, narrowed to those at which the walk can visit the window.
 __device__ __forceinline__ void listing_slices(const TorParams &p, const TileBox &b,
                                                const Event &ev, const Track &t, int &s0, int &s1)
 {
     s0 = max(ev.s_lo, b.sa);
     s1 = min(ev.s_hi, b.sb);
-    window_slices(t, p.half, p.patch, b.x0, b.x1, b.y0, b.y1, s0, s1);
+    window_slices(p, t, b.x0, b.x1, b.y0, b.y1, s0, s1);
 }
 
 struct PlanArgs {
