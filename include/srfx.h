@@ -255,6 +255,20 @@ int srfx_tor_plan_create(const float *lors, int64_t n, int64_t ld, int32_t ncols
                          srfx_grid grid, srfx_tor_model model, srfx_tile tile,
                          srfx_tor_plan *plan, void *stream);
 
+/* The plans of consecutive event ranges of one LOR array, built together:
+ * plans[k] is the plan srfx_tor_plan_create would build for events
+ * bounds[k] .. bounds[k + 1] - 1 (bounds: nplans + 1 non-decreasing host
+ * values, 0 <= bounds[0], bounds[nplans] <= n), e.g. the OSEM subsets of one
+ * axis group (contiguous rows, graph/reconstruction/worker.py:79-85).  One
+ * count pass and one fill pass over all events, two host synchronisations in
+ * all and two shared device blocks, instead of per-plan passes and
+ * allocations; each plan is destroyed on its own.  No reference
+ * counterpart. */
+int srfx_tor_plan_create_subsets(const float *lors, int64_t n, int64_t ld, int32_t ncols,
+                                 srfx_grid grid, srfx_tor_model model, srfx_tile tile,
+                                 int32_t nplans, const int64_t *bounds, srfx_tor_plan *plans,
+                                 void *stream);
+
 /* Listings (event, tile pairs) and workgroup chunks of a plan. */
 int srfx_tor_plan_info(srfx_tor_plan plan, int64_t *nlistings, int32_t *nchunks);
 

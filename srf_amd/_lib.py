@@ -105,6 +105,9 @@ _PROTOS = {
     "srfx_frame_copy": [_P, SrfxGrid, _P, _P],
     "srfx_tor_plan_create": [_P, _I64, _I64, _I32, SrfxGrid, SrfxTorModel, SrfxTile,
                              ctypes.POINTER(ctypes.c_void_p), _P],
+    "srfx_tor_plan_create_subsets": [_P, _I64, _I64, _I32, SrfxGrid, SrfxTorModel, SrfxTile,
+                                     _I32, ctypes.POINTER(_I64), ctypes.POINTER(ctypes.c_void_p),
+                                     _P],
     "srfx_tor_plan_info": [_P, ctypes.POINTER(_I64), ctypes.POINTER(_I32)],
     "srfx_tor_plan_destroy": [_P],
     "srfx_tor_plan_project": [_P, _P, _P, _U32, _P],

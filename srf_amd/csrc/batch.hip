@@ -407,17 +407,29 @@ extern "C" int srfx_tor_batch_create(const srfx_recon *recons, int32_t nrecon,
                 b->frame_blocks += (long long)f.tx * f.ty * f.gz;
                 frames.push_back(f);
             }
+            // the group's subsets, planned together
+            std::vector<int64_t> bounds(nb_subsets + 1);
+            for (int k = 0; k <= nb_subsets; k++)
+                bounds[k] = k * n / nb_subsets;
+            std::vector<srfx_tor_plan> sub(nb_subsets, nullptr);
+            rc_status = srfx_tor_plan_create_subsets(rc.lors[a], n, rc.ld[a], 10,
+                                                     group_grid(rc, a), m, kBatchTile,
+                                                     nb_subsets, bounds.data(), sub.data(),
+                                                     nullptr);
+            if (rc_status)
+                break;
+            // the batch owns every non-empty plan from here (free_plan)
             for (int k = 0; k < nb_subsets; k++) {
-                const int64_t lo = k * n / nb_subsets, hi = (k + 1) * n / nb_subsets;
+                if (bounds[k + 1] > bounds[k])
+                    b->plans.push_back(sub[k]);
+                else
+                    srfx_tor_plan_destroy(sub[k]);
+            }
+            for (int k = 0; k < nb_subsets; k++) {
+                const int64_t lo = bounds[k], hi = bounds[k + 1];
+                srfx_tor_plan plan = sub[k];
                 if (hi <= lo)
                     continue;
-                srfx_tor_plan plan = nullptr;
-                rc_status = srfx_tor_plan_create(rc.lors[a] + lo, hi - lo, rc.ld[a], 10,
-                                                 group_grid(rc, a), m, kBatchTile, &plan,
-                                                 nullptr);
-                if (rc_status)
-                    break;
-                b->plans.push_back(plan);
                 BatchJob job;
                 job.p = plan->p;
                 job.p.img = frame; // contiguous op frame

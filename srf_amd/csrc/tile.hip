@@ -29,10 +29,19 @@ void plan_free(srfx_tor_plan_s *pl)
 {
     if (!pl)
         return;
-    (void)hipFree(pl->list);
-    (void)hipFree(pl->rec);
-    (void)hipFree(pl->chunks);
-    (void)hipFree(pl->proj);
+    if (pl->arena) {
+        // records, projection buffer, list and chunks live in the set's blocks
+        if (pl->arena->refs.fetch_sub(1) == 1) {
+            (void)hipFree(pl->arena->mem[0]);
+            (void)hipFree(pl->arena->mem[1]);
+            delete pl->arena;
+        }
+    } else {
+        (void)hipFree(pl->list);
+        (void)hipFree(pl->rec);
+        (void)hipFree(pl->chunks);
+        (void)hipFree(pl->proj);
+    }
     (void)hipFree(pl->ev_off);
     (void)hipFree(pl->ev_slot);
     (void)hipFree(pl->partial);
@@ -261,6 +270,200 @@ done:
     return rc;
 }
 
+size_t align256(size_t b)
+{
+    return (b + 255) & ~(size_t)255;
+}
+
+// Whether plans of this tile shape and event count can be built as a set
+// (their list entries carry spans).
+bool set_buildable(const TileGeom &g, int64_t n, bool keep_slots)
+{
+    return !keep_slots && g.S <= kSpanMaxSlab && n < ((int64_t)1 << kSpanEventBits);
+}
+
+// Build plans pls[0..K) together; plan k holds events first[k] ..
+// first[k + 1] - 1 of the array at lors (row stride ld), and pls[k]->p is
+// already set up for its own events.  One count pass and one fill pass over
+// all events, two host synchronisations instead of two per plan, and two
+// device blocks shared by the plans (records and projection buffers; lists
+// and chunks).  Every plan's entries carry their spans; chunks are sorted.
+int plan_build_set(srfx_tor_plan_s *const *pls, int K, const int64_t *first, const float *lors,
+                   int chunk, hipStream_t st)
+{
+    TileGeom g = pls[0]->g;
+    g.spans = 1;
+    const int nt = g.nslab * g.nwx * g.nwy;
+    auto *arena = new srfx_tor_plan_s::Arena;
+    arena->mem[0] = arena->mem[1] = nullptr;
+    arena->refs.store(K);
+    for (int k = 0; k < K; k++)
+        pls[k]->arena = arena;
+    // block 0: records and projection buffers
+    std::vector<size_t> rec_off(K), proj_off(K);
+    size_t b0 = 0;
+    for (int k = 0; k < K; k++) {
+        const size_t n = (size_t)std::max<int64_t>(first[k + 1] - first[k], 1);
+        rec_off[k] = b0;
+        b0 += align256(sizeof(float4) * kWalkF4 * n);
+        proj_off[k] = b0;
+        b0 += align256(sizeof(float) * n);
+    }
+    // temporaries: cnt [K nt], fill [K nt], off [K nt], err; first [K + 1];
+    // record and list pointers [K] each
+    const size_t cnt_b = align256(sizeof(int) * ((size_t)3 * K * nt + 1));
+    const size_t first_b = align256(sizeof(int64_t) * (K + 1));
+    const size_t ptr_b = align256(sizeof(void *) * K);
+    char *tmp = nullptr;
+    std::vector<int> cnt((size_t)K * nt), off((size_t)K * nt);
+    std::vector<int64_t> totals(K);
+    std::vector<TileChunk> ch;
+    std::vector<int> nch(K);
+    std::vector<void *> ptrs(K);
+    int mismatch = 0, rc = SRFX_OK;
+    int *cnt_d, *fill_d, *off_d, *err_d;
+    int64_t *first_d;
+    void **rec_d, **list_d;
+    PlanSetArgs a{};
+    const unsigned lds_cnt = 2u * sizeof(int) * nt, lds_fill = 6u * sizeof(int) * nt;
+    const bool lds = lds_fill <= 65536u;
+    const dim3 grid(blocks_for(first[K], 256)), block(256);
+    TorParams p = pls[0]->p;
+    p.lors = lors;
+    char *blk0 = nullptr, *blk1 = nullptr;
+    if ((rc = dev_alloc(&blk0, b0)) || (rc = dev_alloc(&tmp, cnt_b + first_b + 2 * ptr_b)))
+        goto done;
+    arena->mem[0] = blk0;
+    cnt_d = (int *)tmp;
+    fill_d = cnt_d + (size_t)K * nt;
+    off_d = fill_d + (size_t)K * nt;
+    err_d = off_d + (size_t)K * nt;
+    first_d = (int64_t *)(tmp + cnt_b);
+    rec_d = (void **)(tmp + cnt_b + first_b);
+    list_d = (void **)(tmp + cnt_b + first_b + ptr_b);
+    for (int k = 0; k < K; k++) {
+        pls[k]->rec = (float4 *)(blk0 + rec_off[k]);
+        pls[k]->proj = (float *)(blk0 + proj_off[k]);
+        ptrs[k] = pls[k]->rec;
+    }
+    hipLaunchKernelGGL(fill_int, dim3(blocks_for(3 * (int64_t)K * nt + 1, 256)), dim3(256), 0, st,
+                       cnt_d, 3 * (int64_t)K * nt + 1, 0);
+    if (hipMemcpyAsync(first_d, first, sizeof(int64_t) * (K + 1), hipMemcpyHostToDevice, st) !=
+            hipSuccess ||
+        hipMemcpyAsync(rec_d, ptrs.data(), sizeof(void *) * K, hipMemcpyHostToDevice, st) !=
+            hipSuccess) {
+        rc = SRFX_ELAUNCH;
+        goto done;
+    }
+    a.cols7 = pls[0]->cols7;
+    a.ntiles = nt;
+    a.nplans = K;
+    a.lds = lds;
+    a.first = first_d;
+    a.rec = (float4 *const *)rec_d;
+    a.cnt = cnt_d;
+    a.off = off_d;
+    a.fill = fill_d;
+    a.list = (int *const *)list_d;
+    a.err = err_d;
+    if (first[K] > 0)
+        hipLaunchKernelGGL(plan_set_pass<false>, grid, block, lds ? lds_cnt : 0u, st, p, g, a);
+    if ((rc = launch_status()))
+        goto done;
+    if (hipMemcpyAsync(cnt.data(), cnt_d, sizeof(int) * cnt.size(), hipMemcpyDeviceToHost, st) !=
+            hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess) {
+        rc = SRFX_ELAUNCH;
+        goto done;
+    }
+    // per plan: slot offsets per tile, then chunks window by window, each
+    // window's slabs in order (an event's listings in consecutive slabs mostly
+    // share a window, so its record is read again while the L2 holds it)
+    for (int k = 0; k < K; k++) {
+        const int *c = cnt.data() + (size_t)k * nt;
+        int *o = off.data() + (size_t)k * nt;
+        int64_t tot = 0;
+        for (int t = 0; t < nt; t++) {
+            o[t] = (int)tot;
+            tot += c[t];
+            if (tot > (int64_t)INT32_MAX - 1) {
+                rc = SRFX_EUNSUPPORTED;
+                goto done;
+            }
+        }
+        totals[k] = tot;
+        const size_t before = ch.size();
+        const int nw = g.nwx * g.nwy;
+        for (int w = 0; w < nw; w++)
+            for (int t = w; t < nt; t += nw)
+                for (int s = 0; s < c[t]; s += chunk)
+                    ch.push_back(TileChunk{t, o[t] + s, c[t] - s < chunk ? c[t] - s : chunk, 0});
+        nch[k] = (int)(ch.size() - before);
+    }
+    {
+        // block 1: every plan's chunks, contiguous, then the lists
+        const size_t chunk_b = align256(sizeof(TileChunk) * std::max<size_t>(ch.size(), 1));
+        size_t b1 = chunk_b;
+        std::vector<size_t> list_off(K);
+        for (int k = 0; k < K; k++) {
+            list_off[k] = b1;
+            b1 += align256(sizeof(int) * (size_t)std::max<int64_t>(totals[k], 1));
+        }
+        if ((rc = dev_alloc(&blk1, b1)))
+            goto done;
+        arena->mem[1] = blk1;
+        size_t c0 = 0;
+        for (int k = 0; k < K; k++) {
+            pls[k]->chunks = (TileChunk *)blk1 + c0;
+            pls[k]->nchunks = nch[k];
+            c0 += nch[k];
+            pls[k]->list = (int *)(blk1 + list_off[k]);
+            pls[k]->nlist = totals[k];
+            pls[k]->g = g;
+            ptrs[k] = pls[k]->list;
+        }
+    }
+    if (hipMemcpyAsync(off_d, off.data(), sizeof(int) * off.size(), hipMemcpyHostToDevice, st) !=
+            hipSuccess ||
+        hipMemcpyAsync(list_d, ptrs.data(), sizeof(void *) * K, hipMemcpyHostToDevice, st) !=
+            hipSuccess ||
+        (!ch.empty() && hipMemcpyAsync(blk1, ch.data(), sizeof(TileChunk) * ch.size(),
+                                       hipMemcpyHostToDevice, st) != hipSuccess)) {
+        rc = SRFX_ELAUNCH;
+        goto done;
+    }
+    if (first[K] > 0)
+        hipLaunchKernelGGL(plan_set_pass<true>, grid, block, lds ? lds_fill : 0u, st, p, g, a);
+    if ((rc = launch_status()))
+        goto done;
+    if (hipMemcpyAsync(&mismatch, err_d, sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess) {
+        rc = SRFX_ELAUNCH;
+        goto done;
+    }
+    if (mismatch) {
+        rc = SRFX_ELAUNCH; // count and fill passes disagreed: no valid plan
+        goto done;
+    }
+    // listings of each chunk ordered by slice count, longest first
+    if (chunk <= 16 * 256) {
+        for (int k = 0; k < K; k++) {
+            srfx_tor_plan_s *pl = pls[k];
+            if (pl->nchunks > 0)
+                hipLaunchKernelGGL(sort_chunks, dim3(pl->nchunks), dim3(256),
+                                   sizeof(int) * (33 + chunk), st, pl->p, pl->g, pl->chunks,
+                                   pl->list, pl->rec);
+            pl->g.sorted = 1;
+        }
+        rc = launch_status();
+    }
+done:
+    // host buffers above are read by the copies: finish them before returning
+    (void)hipStreamSynchronize(st);
+    (void)hipFree(tmp);
+    return rc;
+}
+
 template <int MODE, bool VREC, bool YF>
 void launch_scatter_l(const TorParams &p, const srfx_tor_plan_s *pl, dim3 grid, dim3 block,
                       unsigned sh, hipStream_t st)
@@ -361,12 +564,12 @@ int plan_project(srfx_tor_plan_s *pl, const float *image, float *proj, hipStream
 
 } // namespace
 
-extern "C" int srfx_tor_plan_create(const float *lors, int64_t n, int64_t ld, int32_t ncols,
-                                    srfx_grid grid, srfx_tor_model m, srfx_tile tile,
-                                    srfx_tor_plan *out, void *stream)
+namespace {
+
+// A plan object for n events of one group frame, set up but not built.
+int plan_new(const float *lors, int64_t n, int64_t ld, int32_t ncols, const srfx_grid &grid,
+             const srfx_tor_model &m, const srfx_tile &tile, srfx_tor_plan_s **out)
 {
-    if (!out)
-        return SRFX_EINVAL;
     *out = nullptr;
     if (ncols != 7 && ncols != 10)
         return SRFX_EINVAL;
@@ -400,12 +603,80 @@ extern "C" int srfx_tor_plan_create(const float *lors, int64_t n, int64_t ld, in
         delete pl;
         return SRFX_EUNSUPPORTED;
     }
-    rc = plan_build(pl, (hipStream_t)stream, tile.chunk, tile.projection != 0);
+    *out = pl;
+    return SRFX_OK;
+}
+
+} // namespace
+
+extern "C" int srfx_tor_plan_create(const float *lors, int64_t n, int64_t ld, int32_t ncols,
+                                    srfx_grid grid, srfx_tor_model m, srfx_tile tile,
+                                    srfx_tor_plan *out, void *stream)
+{
+    if (!out)
+        return SRFX_EINVAL;
+    srfx_tor_plan_s *pl = nullptr;
+    int rc = plan_new(lors, n, ld, ncols, grid, m, tile, &pl);
+    if (rc)
+        return rc;
+    const bool keep_slots = tile.projection != 0;
+    if (set_buildable(pl->g, n, keep_slots)) {
+        const int64_t first[2] = {0, n};
+        rc = plan_build_set(&pl, 1, first, lors, tile.chunk, (hipStream_t)stream);
+    } else {
+        rc = plan_build(pl, (hipStream_t)stream, tile.chunk, keep_slots);
+    }
     if (rc) {
         plan_free(pl);
         return rc;
     }
     *out = pl;
+    return SRFX_OK;
+}
+
+extern "C" int srfx_tor_plan_create_subsets(const float *lors, int64_t n, int64_t ld,
+                                            int32_t ncols, srfx_grid grid, srfx_tor_model m,
+                                            srfx_tile tile, int32_t nplans, const int64_t *bounds,
+                                            srfx_tor_plan *plans, void *stream)
+{
+    if (!plans || !bounds || nplans < 1 || n < 0)
+        return SRFX_EINVAL;
+    for (int k = 0; k < nplans; k++)
+        plans[k] = nullptr;
+    if (bounds[0] < 0 || bounds[nplans] > n)
+        return SRFX_EINVAL;
+    for (int k = 0; k < nplans; k++)
+        if (bounds[k + 1] < bounds[k])
+            return SRFX_EINVAL;
+    std::vector<srfx_tor_plan_s *> pls(nplans, nullptr);
+    std::vector<int64_t> first(nplans + 1);
+    int rc = SRFX_OK;
+    bool together = tile.projection == 0;
+    for (int k = 0; k < nplans && !rc; k++) {
+        const int64_t nk = bounds[k + 1] - bounds[k];
+        rc = plan_new(lors ? lors + bounds[k] : nullptr, nk, ld, ncols, grid, m, tile, &pls[k]);
+        if (!rc)
+            together = together && set_buildable(pls[k]->g, nk, false);
+        first[k] = bounds[k] - bounds[0];
+    }
+    first[nplans] = bounds[nplans] - bounds[0];
+    hipStream_t st = (hipStream_t)stream;
+    if (!rc) {
+        if (together) {
+            rc = plan_build_set(pls.data(), nplans, first.data(), lors + bounds[0], tile.chunk,
+                                st);
+        } else {
+            for (int k = 0; k < nplans && !rc; k++)
+                rc = plan_build(pls[k], st, tile.chunk, tile.projection != 0);
+        }
+    }
+    if (rc) {
+        for (srfx_tor_plan_s *pl : pls)
+            plan_free(pl);
+        return rc;
+    }
+    for (int k = 0; k < nplans; k++)
+        plans[k] = pls[k];
     return SRFX_OK;
 }
 

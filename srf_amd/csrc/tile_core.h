@@ -26,6 +26,8 @@
 // clipped to the window.
 #pragma once
 
+#include <atomic>
+
 #include "tor_core.h"
 
 namespace srfx {
@@ -122,7 +124,7 @@ __device__ __forceinline__ void for_each_tile(const TorParams &p, const TileGeom
             continue;
         for (int wy = wy0; wy <= wy1; wy++)
             for (int wx = wx0; wx <= wx1; wx++)
-                f((sl * g.nwy + wy) * g.nwx + wx);
+                f((sl * g.nwy + wy) * g.nwx + wx, sl, wx, wy);
     }
 }
 
@@ -134,13 +136,12 @@ struct TileBox {
 
 namespace {
 
-// Tile bounds of a chunk: slices sa..sb, op-frame voxels x0..x1, y0..y1.
-__device__ __forceinline__ TileBox tile_box(const TorParams &p, const TileGeom &g, int tile)
+// Tile bounds of slab sl, window (wx, wy): slices sa..sb, op-frame voxels
+// x0..x1, y0..y1.
+__device__ __forceinline__ TileBox tile_box_at(const TorParams &p, const TileGeom &g, int sl,
+                                               int wx, int wy)
 {
     TileBox b;
-    const int per = g.nwx * g.nwy;
-    const int sl = tile / per, r = tile - sl * per;
-    const int wy = r / g.nwx, wx = r - wy * g.nwx;
     b.sa = sl * g.S;
     b.sb = min(b.sa + g.S, p.gz) - 1;
     b.x0 = wx * g.Wx;
@@ -148,6 +149,15 @@ __device__ __forceinline__ TileBox tile_box(const TorParams &p, const TileGeom &
     b.y0 = wy * g.Wy;
     b.y1 = min(b.y0 + g.Wy, p.gy) - 1;
     return b;
+}
+
+// Tile bounds of a chunk's tile index.
+__device__ __forceinline__ TileBox tile_box(const TorParams &p, const TileGeom &g, int tile)
+{
+    const int per = g.nwx * g.nwy;
+    const int sl = tile / per, r = tile - sl * per;
+    const int wy = r / g.nwx, wx = r - wy * g.nwx;
+    return tile_box_at(p, g, sl, wx, wy);
 }
 
 // Whether a scatter into p.out should use the y-fastest LDS tile.
@@ -160,11 +170,11 @@ __host__ __device__ inline bool flush_y_fastest(const TorParams &p)
 // (YF) y fastest, so that a flush along the tile's fastest axis writes
 // consecutive addresses of an output whose unit stride is along op y.
 
-// Narrow [s0, s1] to the slices s with lo <= u(s) <= hi, u(s) = a + s b
-// (FP64; the bounds on s are widened by 1e-6 slice, so the result is a
-// superset).
-__device__ __forceinline__ void axis_slices(double a, double b, double lo, double hi, int &s0,
-                                            int &s1)
+// Narrow [s0, s1] to the slices s with lo <= u(s) <= hi, u(s) = a + s b,
+// rb = 1 / b (FP64; the bounds on s are widened by 1e-6 slice, so the result
+// is a superset).
+__device__ __forceinline__ void axis_slices(double a, double b, double rb, double lo, double hi,
+                                            int &s0, int &s1)
 {
     if (fabs(b) < 1.0e-12) {
         // u is constant to 1e-12 voxel per slice: test it at both ends
@@ -173,36 +183,51 @@ __device__ __forceinline__ void axis_slices(double a, double b, double lo, doubl
             s1 = s0 - 1;
         return;
     }
-    const double t1 = (lo - a) / b, t2 = (hi - a) / b;
+    const double t1 = (lo - a) * rb, t2 = (hi - a) * rb;
     const double tl = fmax(fmin(t1, t2), -1.0e8), th = fmin(fmax(t1, t2), 1.0e8);
     s0 = max(s0, (int)ceil(tl - 1.0e-6));
     s1 = min(s1, (int)floor(th + 1.0e-6));
+}
+
+// 1 / b for axis_slices (any value where |b| < 1e-12: not used there).
+__device__ __forceinline__ double slope_rcp(double b)
+{
+    return fabs(b) < 1.0e-12 ? 0.0 : 1.0 / b;
 }
 
 // Narrow [s0, s1] to a superset of the slices at which the walk can add to
 // a voxel of columns x0..x1 and rows y0..y1: u(s) within the column extent
 // of the window, [x0 + 0.5 - hx', x1 + 0.5 + hx'], and w(s) within the row
 // extent; and the patch [trunc(u) - half, trunc(u) - half + P) meeting it
-// (2 voxels of slack for the exact origin).
-__device__ __forceinline__ void window_slices(const TorParams &p, const Track &t, int x0, int x1,
-                                              int y0, int y1, int &s0, int &s1)
+// (2 voxels of slack for the exact origin).  rbx, rby: slope_rcp of the
+// track's slopes.
+__device__ __forceinline__ void window_slices(const TorParams &p, const Track &t, double rbx,
+                                              double rby, int x0, int x1, int y0, int y1, int &s0,
+                                              int &s1)
 {
     const double ex = col_extent(p, t), ey = row_extent(p, t);
-    axis_slices(t.ax, t.bx, x0 + 0.5 - ex, x1 + 0.5 + ex, s0, s1);
-    axis_slices(t.ay, t.by, y0 + 0.5 - ey, y1 + 0.5 + ey, s0, s1);
+    axis_slices(t.ax, t.bx, rbx, x0 + 0.5 - ex, x1 + 0.5 + ex, s0, s1);
+    axis_slices(t.ay, t.by, rby, y0 + 0.5 - ey, y1 + 0.5 + ey, s0, s1);
     const int P = p.patch, half = p.half;
-    axis_slices(t.ax, t.bx, (double)(x0 - P + half - 2), (double)(x1 + half + 2), s0, s1);
-    axis_slices(t.ay, t.by, (double)(y0 - P + half - 2), (double)(y1 + half + 2), s0, s1);
+    axis_slices(t.ax, t.bx, rbx, (double)(x0 - P + half - 2), (double)(x1 + half + 2), s0, s1);
+    axis_slices(t.ay, t.by, rby, (double)(y0 - P + half - 2), (double)(y1 + half + 2), s0, s1);
 }
 
 // The slices s0..s1 a listing walks: the event's crossed slices in the
 // tile's slab, narrowed to those at which the walk can visit the window.
 __device__ __forceinline__ void listing_slices(const TorParams &p, const TileBox &b,
-                                               const Event &ev, const Track &t, int &s0, int &s1)
+                                               const Event &ev, const Track &t, double rbx,
+                                               double rby, int &s0, int &s1)
 {
     s0 = max(ev.s_lo, b.sa);
     s1 = min(ev.s_hi, b.sb);
-    window_slices(p, t, b.x0, b.x1, b.y0, b.y1, s0, s1);
+    window_slices(p, t, rbx, rby, b.x0, b.x1, b.y0, b.y1, s0, s1);
+}
+
+__device__ __forceinline__ void listing_slices(const TorParams &p, const TileBox &b,
+                                               const Event &ev, const Track &t, int &s0, int &s1)
+{
+    listing_slices(p, b, ev, t, slope_rcp(t.bx), slope_rcp(t.by), s0, s1);
 }
 
 struct PlanArgs {
@@ -270,7 +295,7 @@ __global__ __launch_bounds__(256) void plan_pass(TorParams p, TileGeom g, PlanAr
     if (!FILL) {
         int k = 0;
         if (ok)
-            for_each_tile(p, g, ev, t, [&](int tile) {
+            for_each_tile(p, g, ev, t, [&](int tile, int, int, int) {
                 atomicAdd(LDS ? &sh[tile] : &a.tile_cnt[tile], 1);
                 k++;
             });
@@ -288,7 +313,7 @@ __global__ __launch_bounds__(256) void plan_pass(TorParams p, TileGeom g, PlanAr
     }
     if (LDS) {
         if (ok)
-            for_each_tile(p, g, ev, t, [&](int tile) { atomicAdd(&sh[tile], 1); });
+            for_each_tile(p, g, ev, t, [&](int tile, int, int, int) { atomicAdd(&sh[tile], 1); });
         __syncthreads();
         for (int i = threadIdx.x; i < nt; i += blockDim.x) {
             int c = sh[i];
@@ -298,9 +323,10 @@ __global__ __launch_bounds__(256) void plan_pass(TorParams p, TileGeom g, PlanAr
     }
     const int k0 = (valid && a.ev_slot) ? a.ev_off[e] : 0;
     const int k1 = (valid && a.ev_slot) ? a.ev_off[e + 1] : 0;
+    const double rbx = slope_rcp(t.bx), rby = slope_rcp(t.by);
     int k = 0;
     if (ok)
-        for_each_tile(p, g, ev, t, [&](int tile) {
+        for_each_tile(p, g, ev, t, [&](int tile, int sl, int wx, int wy) {
             int rank;
             bool fits;
             if (LDS) {
@@ -318,9 +344,9 @@ __global__ __launch_bounds__(256) void plan_pass(TorParams p, TileGeom g, PlanAr
                 if (g.spans) {
                     // the listing's slices ride in the entry (the host checked
                     // the limits)
-                    const TileBox b = tile_box(p, g, tile);
+                    const TileBox b = tile_box_at(p, g, sl, wx, wy);
                     int s0, s1;
-                    listing_slices(p, b, ev, t, s0, s1);
+                    listing_slices(p, b, ev, t, rbx, rby, s0, s1);
                     const int lo = s0 <= s1 ? s0 - b.sa : 1, hi = s0 <= s1 ? s1 - b.sa : 0;
                     ent |= (lo << kSpanEventBits) | (hi << (kSpanEventBits + 4));
                 }
@@ -334,6 +360,137 @@ __global__ __launch_bounds__(256) void plan_pass(TorParams p, TileGeom g, PlanAr
         });
     if (valid && a.ev_slot && k0 + k != k1)
         atomicAdd(a.err, 1);
+}
+
+// A set of plans over consecutive event ranges of one LOR array (one plan per
+// OSEM subset of an axis group): plan k holds events [first[k], first[k+1]),
+// indexed from 0 in its own records and list.  Tables are per (plan, tile),
+// plan-major.
+struct PlanSetArgs {
+    int cols7, ntiles, nplans;
+    int lds;                // the per-workgroup counters fit in LDS (2 plans)
+    const int64_t *first;   // [nplans + 1]
+    float4 *const *rec;     // [nplans] walk records
+    int *cnt;               // count pass: listings per (plan, tile); fill pass: read
+    const int *off;         // fill pass: first slot of each (plan, tile) in the plan's list
+    int *fill;              // fill pass: slots taken per (plan, tile)
+    int *const *list;       // fill pass: [nplans] list entries
+    int *err;               // fill pass: listings that did not fit
+};
+
+// The plan holding event e of the set's array.
+__device__ __forceinline__ int plan_of(const PlanSetArgs &a, int64_t e)
+{
+    int lo = 0, hi = a.nplans; // first[lo] <= e < first[hi]
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (a.first[mid] <= e)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+// plan_pass for a plan set, over all events of the set in one launch.  A
+// workgroup whose events belong to at most two plans counts in LDS (two
+// per-plan slices of counters), as plan_pass does for one plan; otherwise
+// (tiny plans) it uses the global counters directly.  List entries carry the
+// listing's slices (the set's plans always have spans).
+template <bool FILL>
+__global__ __launch_bounds__(256) void plan_set_pass(TorParams p, TileGeom g, PlanSetArgs a)
+{
+    extern __shared__ int sh[]; // [2 plans][counts (FILL: + bases, ranks)][ntiles]
+    const int nt = a.ntiles;
+    const int per = FILL ? 3 * nt : nt;
+    const int64_t total = a.first[a.nplans];
+    const int64_t e0 = (int64_t)blockIdx.x * blockDim.x;
+    const int64_t e = e0 + threadIdx.x;
+    const bool valid = e < total;
+    const int kf = plan_of(a, e0), kl = plan_of(a, min(e0 + (int64_t)blockDim.x, total) - 1);
+    const bool lds = a.lds && kl - kf <= 1;
+    if (lds) {
+        for (int i = threadIdx.x; i < 2 * per; i += blockDim.x)
+            sh[i] = 0;
+        __syncthreads();
+    }
+    const int k = valid ? plan_of(a, e) : kf;
+    const int64_t le = e - a.first[k];
+    TorParams q = p; // plan k's events
+    q.lors = p.lors + a.first[k];
+    q.n = a.first[k + 1] - a.first[k];
+    int *lcnt = sh + (k - kf) * per;
+    int *gcnt = a.cnt + (int64_t)k * nt;
+    Event ev{};
+    Track t{};
+    bool ok = false;
+    if (valid) {
+        if (FILL) {
+            unpack_walk(q, a.rec[k], le, ev, t);
+            ok = ev.s_lo <= ev.s_hi;
+        } else {
+            RawEvent r;
+            if (a.cols7)
+                fetch_event<kMap>(q, le, r);
+            else
+                fetch_event<kProj>(q, le, r);
+            const ExactGeom eg = exact_geom(q, r);
+            ok = plan_init(q, a.cols7, r, eg, ev);
+            if (ok)
+                make_track(q, ev, eg, r, t);
+            pack_walk(q, r, ev, t, ok, a.rec[k], le);
+        }
+    }
+    if (!FILL || lds) {
+        if (ok)
+            for_each_tile(q, g, ev, t, [&](int tile, int, int, int) {
+                atomicAdd(lds ? &lcnt[tile] : &gcnt[tile], 1);
+            });
+        if (lds) {
+            __syncthreads();
+            for (int i = threadIdx.x; i < 2 * nt; i += blockDim.x) {
+                const int slot = i >= nt, tile = i - slot * nt;
+                const int c = sh[slot * per + tile];
+                if (c && kf + slot <= kl) {
+                    if (FILL)
+                        sh[slot * per + nt + tile] =
+                            atomicAdd(&a.fill[(int64_t)(kf + slot) * nt + tile], c);
+                    else
+                        atomicAdd(&a.cnt[(int64_t)(kf + slot) * nt + tile], c);
+                }
+            }
+            __syncthreads();
+        }
+    }
+    if (!FILL)
+        return;
+    const double rbx = slope_rcp(t.bx), rby = slope_rcp(t.by);
+    const int *goff = a.off + (int64_t)k * nt;
+    int *gfill = a.fill + (int64_t)k * nt;
+    int *list = a.list[k];
+    if (ok)
+        for_each_tile(q, g, ev, t, [&](int tile, int sl, int wx, int wy) {
+            int rank;
+            bool fits;
+            if (lds) {
+                const int loc = atomicAdd(&lcnt[2 * nt + tile], 1);
+                rank = lcnt[nt + tile] + loc;
+                fits = loc < lcnt[tile];
+            } else {
+                rank = atomicAdd(&gfill[tile], 1);
+                fits = true;
+            }
+            if (fits && rank < gcnt[tile]) {
+                const TileBox b = tile_box_at(q, g, sl, wx, wy);
+                int s0, s1;
+                listing_slices(q, b, ev, t, rbx, rby, s0, s1);
+                const int lo = s0 <= s1 ? s0 - b.sa : 1, hi = s0 <= s1 ? s1 - b.sa : 0;
+                list[goff[tile] + rank] =
+                    (int)le | (lo << kSpanEventBits) | (hi << (kSpanEventBits + 4));
+            } else {
+                atomicAdd(a.err, 1);
+            }
+        });
 }
 
 __global__ __launch_bounds__(256) void fill_int(int *v, int64_t n, int value)
@@ -718,4 +875,11 @@ struct srfx_tor_plan_s {
     float *proj;           // projection buffer of the OSEM step
     int *ev_off, *ev_slot; // tiled projection only: each event's slots
     float *partial;        // tiled projection only: per-slot partial sums
+    // Plans built together (srfx_tor_plan_create_subsets) share two device
+    // blocks holding their records, projection buffers, lists and chunks;
+    // the last plan destroyed frees them.  Null for a plan built alone.
+    struct Arena {
+        void *mem[2];
+        std::atomic<int> refs;
+    } *arena;
 };

@@ -146,6 +146,36 @@ class SplitLoRsModel:
                                axis=a)
         return plans
 
+    def make_subset_plans(self, image, data, nb_subsets, ncols=10):
+        """make_plans for every OSEM subset of ``data`` (contiguous rows of
+        each group, recon.subset_bounds), each group's subsets built together
+        (srfx_tor_plan_create_subsets).  Returns a list over subsets of
+        {axis: TorPlan or None}."""
+        from .plan import DEFAULT_CHUNK, DEFAULT_TILE, TorPlan
+        from .recon import subset_bounds
+        x = _world(image)
+        K = int(nb_subsets)
+        out = [dict() for _ in range(K)]
+        for a in self.AXIS:
+            lm = data[a]
+            n = len(lm)
+            bounds = [subset_bounds(n, k, K)[0] for k in range(K)] + [n]
+            if n == 0:
+                for k in range(K):
+                    out[k][a] = None
+                continue
+            g, c, sz, st = group_frame(x.shape, image.center, image.size, x.stride(), a)
+            plans = TorPlan.subsets(lm.soa, bounds, g, c, sz, self.kernel_width,
+                                    self.geo_sigma2_flag, self.tof_bin, self.tof_sigma2,
+                                    strides=st, ncols=ncols, tile=self.tile or DEFAULT_TILE,
+                                    chunk=self.chunk or DEFAULT_CHUNK, axis=a)
+            for k, pl in enumerate(plans):
+                if pl.n == 0:
+                    pl.close()
+                    pl = None
+                out[k][a] = pl
+        return out
+
     @staticmethod
     def _check_plan_layout(plan, t):
         if list(plan.strides) != [int(v) for v in group_frame(

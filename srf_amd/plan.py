@@ -47,6 +47,44 @@ class TorPlan:
                   t, ctypes.byref(h), _stream())
         self._h = h
 
+    @classmethod
+    def subsets(cls, lors, bounds, grid, center, size, kernel_width, geo_sigma2_flag=False,
+                tof_bin=1.0e4, tof_sigma2=1.0e4, strides=(0, 0, 0), ncols=10,
+                tile=DEFAULT_TILE, chunk=DEFAULT_CHUNK, block=256, axis=None):
+        """Plans of the event ranges bounds[k]:bounds[k + 1] of one LOR tensor
+        (the OSEM subsets of one axis group), built together by
+        srfx_tor_plan_create_subsets.  Returns one TorPlan per range."""
+        if ncols not in (7, 10):
+            raise ValueError("ncols must be 10 (TOF rows) or 7 (Maplors rows)")
+        full, n, ld = _soa(lors, ncols)
+        bounds = [int(b) for b in bounds]
+        K = len(bounds) - 1
+        if K < 1 or bounds[0] < 0 or bounds[-1] > n or any(
+                b1 < b0 for b0, b1 in zip(bounds, bounds[1:])):
+            raise ValueError("bounds must be non-decreasing event offsets within the LORs")
+        g, cgrid = _geom(grid, center, size, strides)
+        model = _lib.make_tor_model(kernel_width, geo_sigma2_flag, tof_bin, tof_sigma2)
+        t = _lib.SrfxTile(*[int(v) for v in tile], int(chunk), int(block), 0)
+        hs = (ctypes.c_void_p * K)()
+        bs = (ctypes.c_int64 * (K + 1))(*bounds)
+        _lib.call("srfx_tor_plan_create_subsets", _ptr(full), n, ld, ncols, cgrid, model, t, K,
+                  bs, hs, _stream())
+        out = []
+        for k in range(K):
+            pl = cls.__new__(cls)
+            pl.lors = full[:, bounds[k]:bounds[k + 1]]
+            pl.n = bounds[k + 1] - bounds[k]
+            pl.grid, pl._cgrid = g, cgrid
+            pl.center, pl.size = [float(c) for c in center], [float(s) for s in size]
+            pl.strides = tuple(int(s) for s in strides)
+            pl.axis = axis
+            pl.ncols = ncols
+            pl.tile = tuple(int(v) for v in tile)
+            pl.tiled_projection = False
+            pl._h = ctypes.c_void_p(hs[k])
+            out.append(pl)
+        return out
+
     def info(self):
         """(number of (event, tile) listings, number of workgroup chunks)."""
         a, b = ctypes.c_int64(), ctypes.c_int32()

@@ -120,7 +120,11 @@ class OSEM:
     def plan(self, image):
         """Build the tile plans of every subset for images laid out like image."""
         self.close()
-        self.plans = [self.model.make_plans(image, sub) for sub in self.subsets]
+        if hasattr(self.model, "make_subset_plans"):
+            # every subset of a group in one build (two synchronisations)
+            self.plans = self.model.make_subset_plans(image, self.data, self.nb_subsets)
+        else:
+            self.plans = [self.model.make_plans(image, sub) for sub in self.subsets]
         return self.plans
 
     def close(self):

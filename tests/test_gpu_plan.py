@@ -243,3 +243,47 @@ def test_frame_gather_far_corner(cuda):
     p = ops.projection(L, X, *args).cpu().numpy()
     pref = O.tor_project(lors, img, *args, prec="f64")
     assert H.rel_l2(p, pref) <= TOL64
+
+
+# (tile, chunk): the default tile (per-workgroup counters in LDS) and a tile
+# so small that the counters of two plans exceed the LDS (global counters)
+@pytest.mark.parametrize("tile,chunk", [((8, 32, 20), 2048), ((1, 2, 2), 50)])
+def test_plan_subsets_equal_single_plans(cuda, case, tile, chunk):
+    """Plans of consecutive event ranges built together
+    (srfx_tor_plan_create_subsets) equal plans built one by one: the same
+    listings, the same OSEM step and backprojection (to FP32 rounding of the
+    LDS and flush order), and the FP64 oracle.  The ranges include an empty
+    one and ranges smaller than a workgroup (a workgroup then spans more
+    than two plans)."""
+    from oracle import lib as O
+    from srf_amd.plan import TorPlan
+    lors, img = case
+    L, X = _t(lors, cuda), _t(img, cuda)
+    n = lors.shape[1]
+    bounds = [0, 3, 3, 100, 2500, 2600, n]
+    args = (H.TOR_GRID, H.TOR_CENTER, H.TOR_SIZE, H.KW, True, H.TOF_BIN, H.TOF_SIGMA2)
+    plans = TorPlan.subsets(L, bounds, *args, tile=tile, chunk=chunk)
+    assert len(plans) == len(bounds) - 1
+    v = np.random.default_rng(8).uniform(0.5, 2.0, n).astype(np.float32)
+    for k, pl in enumerate(plans):
+        lo, hi = bounds[k], bounds[k + 1]
+        assert pl.n == hi - lo
+        one = TorPlan(L[:, lo:hi].contiguous(), *args, tile=tile, chunk=chunk)
+        assert pl.info() == one.info(), (k, pl.info(), one.info())
+        if hi == lo:
+            one.close()
+            continue
+        a, b = pl.osem_subset(X).cpu().numpy(), one.osem_subset(X).cpu().numpy()
+        assert H.rel_l2(a, b) <= 1e-6
+        sub = np.ascontiguousarray(lors[:, lo:hi])
+        p = O.tor_project(sub, img, *args, prec="f64")
+        keep = p > 1e-7
+        inv = np.where(keep, 1.0 / np.where(keep, p, 1.0), 0.0).astype(np.float32)
+        ref = O.tor_backproject_adjoint(sub, inv, *args)
+        assert H.rel_l2(a, ref) <= TOL64
+        bp = pl.backprojection(_t(v[lo:hi], cuda)).cpu().numpy()
+        rb = O.tor_backproject(sub, v[lo:hi], *args, prec="f64")
+        assert H.rel_l2(bp, rb) <= TOL64
+        one.close()
+    for pl in plans[::-1]:  # the shared blocks go with the last plan
+        pl.close()
