@@ -610,14 +610,24 @@ __device__ __forceinline__ void record_project(const TorParams &p, float4 *rec, 
                                                int sub, bool valid)
 {
     float part = 0.0f;
+    Event ev;
+    Track t;
+    bool any = false;
     if (valid) {
-        Event ev;
-        Track t;
         unpack_walk(p, rec, e, ev, t);
-        if (ev.s_lo <= ev.s_hi)
-            part = walk_event<0, NB, FRAME>(p, ev, t, p.use_tof, 0.0f, ev.s_lo + sub, ev.s_hi,
-                                            LPE);
+        any = ev.s_lo <= ev.s_hi;
     }
+    // The lanes of a wave walk the same slices at the same time (from the
+    // wave's first crossed slice, lane sub of an event on slices base + sub
+    // + k LPE), so that the gathers of neighbouring events (the plan's event
+    // order is a locality order) hit the same cache lines.
+    int base = any ? ev.s_lo : 0x7fffffff;
+#pragma unroll
+    for (int m = 1; m < 64; m <<= 1)
+        base = min(base, __shfl_xor(base, m, 64));
+    if (any)
+        part = walk_event<0, NB, FRAME>(p, ev, t, p.use_tof, 0.0f, base + sub, ev.s_hi, LPE,
+                                        ev.s_lo);
     const float pv = group_sum<LPE>(part);
     if (valid && sub == 0)
         reinterpret_cast<float *>(rec)[4 * kWalkHead * e + kWalkValue] =
@@ -625,6 +635,17 @@ __device__ __forceinline__ void record_project(const TorParams &p, float4 *rec, 
 }
 
 // FRAME: p.img is a contiguous op frame of p.img_bytes (frame gathers).
+// Workgroup b runs on XCD b % 8 (the dispatcher deals workgroups out round
+// robin over the chip's 8 XCDs); remap so that each XCD takes a contiguous
+// eighth of the workgroups (of the events, in their locality order), and its
+// L2 serves a compact part of the image.  A bijection of [0, gridDim.x).
+__device__ __forceinline__ unsigned xcd_block()
+{
+    const unsigned b = blockIdx.x, n = gridDim.x;
+    const unsigned q = n / 8u, r = n % 8u, x = b % 8u;
+    return x * q + min(x, r) + b / 8u;
+}
+
 template <int NB, int LPE, bool FRAME>
 __global__ __launch_bounds__(256) void record_proj_kernel(TorParams p, float4 *rec)
 {

@@ -961,13 +961,15 @@ __device__ __forceinline__ float slice_box_image(const TorParams &p, const Event
 template <int KIND, int NB, bool FRAME = false>
 __device__ __forceinline__ float walk_event(const TorParams &p, const Event &ev, const Track &t,
                                             bool tof, float scale, int s_first, int s_last,
-                                            int rstep)
+                                            int rstep, int s_min = -1)
 {
     constexpr int NBB = NB > 1 ? NB : 2;
     const int shape = p.full_patch ? 0 : box_shape<NB>(t);
     const float ts = p.tofk * scale;
     float acc = 0.0f;
     for (int s = s_first; s <= s_last; s += rstep) {
+        if (s < s_min)
+            continue; // a slice before the event's first (aligned walks)
         Cross c;
         cross_at(p, ev, t, s, c);
         // scatters: the scale rides in amp (one product per slice, not per
