@@ -556,6 +556,112 @@ __device__ __forceinline__ float window_walk(const TorParams &p, const TileGeom 
 // One listing: its slices s0..s1 inside the tile (listing_slices), each
 // through the NB x NB candidate box clipped to the window when the event's
 // box fits, else the clipped chord walk.
+// The scatter of one listing (slices s0..s1) through its BX x BY candidate box
+// into a y-fastest FP64 LDS tile: the box walk of slice_box<1, BX, BY, 2>
+// with less work per slice.
+//   * The box's clip to the patch [0, P) and the window as float bounds
+//     (lo, hi) on its first column / row, so the offsets need no integer
+//     round trip; clipped candidates get an infinite term (neither kept nor
+//     near).
+//   * The scale, the TOF constant and the slice's TOF factor folded into each
+//     kept candidate's exponent, exp2(cw dt + thr cw + log2 |scale tofk| +
+//     TOF quadratic), times the sign of the scale unless POS (a positive
+//     scale: the ratio backprojection).
+//   * LDS byte offsets carried from slice to slice.
+// The same candidates and decisions as slice_box (its box contains every kept
+// voxel whatever the last-bit rounding of its start).
+template <int BX, int BY, bool POS>
+__device__ __forceinline__ void scatter_box_yf(const TorParams &p, const TileGeom &g,
+                                               const TileBox &b, const Event &ev, const Track &t,
+                                               bool tof, float scale, int s0, int s1,
+                                               double *lds)
+{
+    constexpr int NC = BX * BY;
+    const float eps = 1.0e-4f;
+    const float hxr = fmaf(t.hx, p.rfinter_x, eps), dyr = fmaf(t.dymax, p.rfinter_y, eps);
+    const float pm = (float)(p.patch - 1);
+    const float nxm = (float)(t.nx - 1), nym = (float)(t.ny - 1);
+    const float wx0 = (float)b.x0, wx1 = (float)b.x1, wy0 = (float)b.y0, wy1 = (float)b.y1;
+    const float sgn = POS ? 1.0f : (scale < 0.0f ? -1.0f : 1.0f);
+    const float cthr = ev.thr * ev.cw + __log2f(fabsf(tof ? p.tofk * scale : scale));
+    // LDS byte offset of voxel (X, Y) of slice s (y fastest): lsl(s) + 8 (X Wy + Y)
+    const int wy8 = 8 * g.Wy;
+    const int sstep = 8 * g.Wx * g.Wy;
+    int lsl = (s0 - b.sa) * sstep - 8 * (b.x0 * g.Wy + b.y0);
+    for (int s = s0; s <= s1; s++, lsl += sstep) {
+        Cross c;
+        cross_at(p, ev, t, s, c);
+        const double sd = (double)s;
+        const float cexp = tof ? cthr + (float)fma(fma(t.tc2, sd, t.tc1), sd, t.tc0) : cthr;
+        const float fx0 = (float)c.ix0, fy0 = (float)c.iy0;
+        const float ib0 = ceilf(fmaf(c.dx0, p.rfinter_x, -hxr));
+        const float jb0 = ceilf(fmaf(c.dy0, p.rfinter_y, -dyr));
+        const float ilo = fmaxf(fmaxf(ib0, wx0 - fx0), 0.0f);
+        const float ihi = fminf(fminf(ib0 + nxm, wx1 - fx0), pm);
+        const float jlo = fmaxf(fmaxf(jb0, wy0 - fy0), 0.0f);
+        const float jhi = fminf(fminf(jb0 + nym, wy1 - fy0), pm);
+        const float ni = ihi - ilo, nj = jhi - jlo;
+        float cx2[BX], cux[BX], rt2[BY], ruy[BY];
+        {
+            float dx = fmaf(-ilo, p.finter_x, c.dx0);
+#pragma unroll
+            for (int k = 0; k < BX; k++) {
+                col_terms(dx, ev.ux, cx2[k], cux[k]);
+                cx2[k] = ni >= (float)k ? cx2[k] : INFINITY;
+                dx -= p.finter_x;
+            }
+            float dy = fmaf(-jlo, p.finter_y, c.dy0);
+#pragma unroll
+            for (int k = 0; k < BY; k++) {
+                row_terms(dy, ev.uy, ev.thr, rt2[k], ruy[k]);
+                rt2[k] = nj >= (float)k ? rt2[k] : INFINITY;
+                dy -= p.finter_y;
+            }
+        }
+        float dt[NC];
+        float gap = INFINITY;
+#pragma unroll
+        for (int jj = 0; jj < BY; jj++) {
+#pragma unroll
+            for (int ii = 0; ii < BX; ii++) {
+                const int k = jj * BX + ii;
+                dt[k] = tube_dt(cx2[ii], cux[ii], rt2[jj], ruy[jj]);
+                gap = fminf(gap, fabsf(dt[k]));
+            }
+        }
+        const int ib = (int)ilo, jb = (int)jlo;
+        // LDS address of the box's first voxel; a clipped-away candidate is
+        // never added, so its address may lie outside the tile
+        const int a0 = opaque(lsl + __mul24(c.ix0 + ib, wy8) + 8 * (c.iy0 + jb));
+        char *const base = reinterpret_cast<char *>(lds);
+        if (gap <= kThrMargin) {
+            // rare: the reference's FP64 decision, one voxel at a time, for the
+            // candidates within the margin of the cut-off; a kept one is added
+            // here (the common pass below only adds dt < -margin)
+            uint32_t near = 0;
+#pragma unroll
+            for (int k = 0; k < NC; k++)
+                near |= (uint32_t)(fabsf(dt[k]) <= kThrMargin) << k;
+            for (uint32_t m = (uint32_t)opaque((int)near); m; m &= m - 1) {
+                const int k = __builtin_ctz(m);
+                const int i = k % BX, j = k / BX;
+                if (inside_exact(p, ev, s, c.ix0 + ib + i, c.iy0 + jb + j)) {
+                    const float w = sgn * __builtin_amdgcn_exp2f(fmaf(dt[k], ev.cw, cexp));
+                    atomicAdd(reinterpret_cast<double *>(base + a0 + i * wy8 + 8 * j), (double)w);
+                }
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < NC; k++) {
+            if (dt[k] < -kThrMargin) {
+                const float w = sgn * __builtin_amdgcn_exp2f(fmaf(dt[k], ev.cw, cexp));
+                atomicAdd(reinterpret_cast<double *>(base + a0 + (k % BX) * wy8 + 8 * (k / BX)),
+                          (double)w);
+            }
+        }
+    }
+}
+
 template <int KIND, int NB, bool YF, typename T>
 __device__ __forceinline__ float tile_event(const TorParams &p, const TileGeom &g,
                                             const TileBox &b, const Event &ev, const Track &t,
@@ -709,8 +815,17 @@ __device__ __forceinline__ void scatter_chunk(const TorParams &p, const TileGeom
             scale = val;
             go = scale != 0.0f;
         }
-        if (go)
-            tile_event<1, NB, YF>(p, g, b, ev, t, tof, scale, s0, s1, ldsd);
+        if (go) {
+            constexpr int NBB = NB > 1 ? NB : 2;
+            constexpr bool POS = MODE == kBack; // scale 1/v with v > 1e-7
+            const int shape = box_shape<NB>(t);
+            if (YF && NB > 0 && shape == 1)
+                scatter_box_yf<NBB, NBB - 1, POS>(p, g, b, ev, t, tof, scale, s0, s1, ldsd);
+            else if (YF && NB > 0 && shape == 2)
+                scatter_box_yf<NBB, NBB, POS>(p, g, b, ev, t, tof, scale, s0, s1, ldsd);
+            else
+                tile_event<1, NB, YF>(p, g, b, ev, t, tof, scale, s0, s1, ldsd);
+        }
     }
     __syncthreads();
     // Flush.  Tile element i = (c * nb + m) * na + f (f along the fastest
@@ -816,9 +931,11 @@ __global__ __launch_bounds__(256) void sort_chunks(TorParams p, TileGeom g,
 
 // One workgroup per chunk.  The LDS tile is FP64: LDS FP64 atomic adds run
 // at several lanes per cycle where FP32 ones are serialised lane by lane,
-// and the tile sums gain precision.
+// and the tile sums gain precision.  Four waves per SIMD where the walk fits
+// in 128 VGPRs without spilling (the LDS atomics need the latency cover);
+// the 4 x 4 box walk needs more registers and runs three.
 template <int MODE, int NB, bool VREC, bool YF>
-__global__ __launch_bounds__(256) void tile_scatter(TorParams p, TileGeom g,
+__global__ __launch_bounds__(256, NB == 4 ? 3 : 4) void tile_scatter(TorParams p, TileGeom g,
                                                     const TileChunk *__restrict__ chunks,
                                                     const int *__restrict__ list,
                                                     const float4 *__restrict__ rec)
