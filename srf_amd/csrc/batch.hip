@@ -244,9 +244,9 @@ srfx_grid group_grid(const srfx_recon &r, int a)
     return g;
 }
 
-// Tile of the batch's plans: 8 slices x 32 x 20 voxels, <= 2048 events per
-// workgroup (the FP64 LDS tile is 40 KB: four workgroups per CU).
-constexpr srfx_tile kBatchTile = {8, 32, 20, 2048, 256, 0};
+// Tile of the batch's plans: 12 slices x 32 x 12 voxels, <= 2048 events per
+// workgroup (the FP64 LDS tile is 36 KB: four workgroups per CU).
+constexpr srfx_tile kBatchTile = {12, 32, 12, 2048, 256, 0};
 
 } // namespace
 } // namespace srfx

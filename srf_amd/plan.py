@@ -20,7 +20,7 @@ import torch
 from . import _lib
 from .ops import _check_dev, _geom, _ptr, _soa, _stream, _vec
 
-DEFAULT_TILE = (8, 32, 20)
+DEFAULT_TILE = (12, 32, 12)
 DEFAULT_CHUNK = 2048
 
 

@@ -30,7 +30,7 @@ def workload(cuda):
     from srf_amd.data import split_from_groups
     from srf_amd.plan import DEFAULT_CHUNK, DEFAULT_TILE
     from srf_amd.scanner import CylindricalScanner
-    assert DEFAULT_TILE == (8, 32, 20) and DEFAULT_CHUNK == 2048
+    assert DEFAULT_TILE == (12, 32, 12) and DEFAULT_CHUNK == 2048
     sc = CylindricalScanner()
     _, ev, meta = synth.c2(n_events=N_EVENTS, seed=2, scanner=sc, device=cuda)
     groups = preprocess.recon_process(ev, sc.tof_res_ps)
