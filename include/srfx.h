@@ -304,6 +304,55 @@ int srfx_tor_plan_osem_subset(srfx_tor_plan plan, const float *image, srfx_grid 
  * output is the tensor of op "Projection", tor.cc:122-128). */
 int srfx_tor_plan_scales(srfx_tor_plan plan, float *out, void *stream);
 
+/* Tiled Siddon backprojection over a fixed LOR set (no counterpart in the
+ * reference, whose BackRayTracing adds every segment to the image with its
+ * own atomic, siddon.cu.cc:339-381; the ops compute what
+ * srfx_siddon_backproject / _backproject_adjoint / _osem_subset compute for
+ * the same LORs, with the same voxels and segment values, summed in another
+ * order).  A plan cuts each LOR's walk into runs, one per image tile of
+ * tx x ty x tz voxels it crosses (tx * ty * tz <= 8192: the scatter's tile is
+ * FP64 in LDS), and lists the runs by tile; a scatter workgroup takes up to
+ * `chunk` runs of one tile (0 = 2048), accumulates them in LDS and flushes
+ * the tile with one atomic per non-zero voxel.  `block`: 64, 128 or 256
+ * (0 = 256).  Grid dimensions below 65536. */
+typedef struct srfx_siddon_tile {
+    int32_t tx, ty, tz, chunk, block;
+} srfx_siddon_tile;
+
+typedef struct srfx_siddon_plan_s *srfx_siddon_plan;
+
+/* Build a plan for n LORs (7 columns x1..z2, tof, as srfx_siddon_project) of
+ * an image grid (the op's read and written images share its strides).  Keeps
+ * the lors pointer: the LORs must stay unchanged and alive while the plan is
+ * used.  Allocates the plan's tables and synchronises `stream`. */
+int srfx_siddon_plan_create(const float *lors, int64_t n, int64_t ld, srfx_grid grid,
+                            float tof_bin, float tof_sigma2, srfx_siddon_tile tile,
+                            srfx_siddon_plan *plan, void *stream);
+
+/* Runs (LOR, tile pairs) and workgroup chunks of a plan. */
+int srfx_siddon_plan_info(srfx_siddon_plan plan, int64_t *nruns, int32_t *nchunks);
+
+/* Synchronises the device, then frees the plan. */
+int srfx_siddon_plan_destroy(srfx_siddon_plan plan);
+
+/* Backprojection image += a / v for v > 0 (siddon.cc:28-43) and the plain
+ * adjoint image += a * g of the plan's LORs. */
+int srfx_siddon_plan_backproject(srfx_siddon_plan plan, const float *lors_value, float *image,
+                                 uint32_t flags, void *stream);
+int srfx_siddon_plan_backproject_adjoint(srfx_siddon_plan plan, const float *g, float *image,
+                                         uint32_t flags, void *stream);
+
+/* OSEM subset step of the plan's LORs, as srfx_siddon_osem_subset: the
+ * projection (srfx_siddon_project) into a plan buffer, then the tiled
+ * backprojection of 1/p for p > 0 into bp.  Calls on one plan must not
+ * overlap. */
+int srfx_siddon_plan_osem_subset(srfx_siddon_plan plan, const float *image, float *bp,
+                                 uint32_t flags, void *stream);
+
+/* The projection of the last srfx_siddon_plan_osem_subset call (n floats, LOR
+ * order), copied to out.  No reference counterpart. */
+int srfx_siddon_plan_projection(srfx_siddon_plan plan, float *out, void *stream);
+
 /* ------------------------------------------------------------ Batched --- */
 
 /* One list-mode TOF reconstruction of a batch.  The image, the inverse

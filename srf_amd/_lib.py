@@ -72,6 +72,13 @@ class SrfxTile(ctypes.Structure):
                 ("projection", ctypes.c_int32)]
 
 
+class SrfxSiddonTile(ctypes.Structure):
+    """srfx_siddon_tile: image tiles of `tx` x `ty` x `tz` voxels, at most
+    `chunk` runs per workgroup, `block` threads per workgroup."""
+    _fields_ = [("tx", ctypes.c_int32), ("ty", ctypes.c_int32), ("tz", ctypes.c_int32),
+                ("chunk", ctypes.c_int32), ("block", ctypes.c_int32)]
+
+
 class SrfxError(RuntimeError):
     def __init__(self, name, status, detail):
         super().__init__(f"{name} failed with status {status} ({detail})")
@@ -116,6 +123,14 @@ _PROTOS = {
     "srfx_tor_plan_maplors": [_P, _P, _P, _U32, _P],
     "srfx_tor_plan_osem_subset": [_P, _P, SrfxGrid, _P, _U32, _P],
     "srfx_tor_plan_scales": [_P, _P, _P],
+    "srfx_siddon_plan_create": [_P, _I64, _I64, SrfxGrid, _F, _F, SrfxSiddonTile,
+                                ctypes.POINTER(ctypes.c_void_p), _P],
+    "srfx_siddon_plan_info": [_P, ctypes.POINTER(_I64), ctypes.POINTER(_I32)],
+    "srfx_siddon_plan_destroy": [_P],
+    "srfx_siddon_plan_backproject": [_P, _P, _P, _U32, _P],
+    "srfx_siddon_plan_backproject_adjoint": [_P, _P, _P, _U32, _P],
+    "srfx_siddon_plan_osem_subset": [_P, _P, _P, _U32, _P],
+    "srfx_siddon_plan_projection": [_P, _P, _P],
 }
 
 _lib = None

@@ -349,10 +349,17 @@ class CompleteLoRsModel:
                       self.tof_bin, self.tof_sigma2, _ptr(p), 0, _stream())
         return data.replace_values(p)
 
-    def backprojection(self, data, image, out=None):
+    def backprojection(self, data, image, out=None, plans=None):
+        """Backprojection (complete_lors.py:72-98); with plans (make_plans of
+        the same LORs) through the plan's tiled scatter."""
         x = _world(image)
         if out is None:
             out = torch.zeros_like(x)
+        if plans is not None:
+            if plans["siddon"] is not None:
+                v = _vec(data.values, plans["siddon"].n, "lors_value")
+                plans["siddon"].backprojection(v, out=out, accumulate=True)
+            return Image(out, image.center, image.size)
         lors, n, ld = _soa(data.soa, 7)
         if n:
             v = _vec(data.values, n, "lors_value")
@@ -361,19 +368,38 @@ class CompleteLoRsModel:
                       self.tof_sigma2, _ptr(out), _lib.SRFX_ACCUMULATE, _stream())
         return Image(out, image.center, image.size)
 
-    def osem_subset(self, image, data, out=None):
+    def osem_subset(self, image, data, out=None, plans=None):
         """Projection + ratio backprojection (recon_step.py:33-41 over
         complete_lors.py:59-98) in one fused kernel (srfx_siddon_osem_subset):
         out += A^T (1 / A x), events with A x = 0 left out as the reference's
-        backprojection does (siddon.cu.cc:363-371)."""
+        backprojection does (siddon.cu.cc:363-371).  With plans (make_plans
+        of the same LORs), the projection kernel, then the tiled
+        backprojection of the plan."""
         x = _world(image)
         if out is None:
             out = torch.zeros_like(x)
+        if plans is not None:
+            if plans["siddon"] is not None:
+                plans["siddon"].osem_subset(x, out=out, accumulate=True)
+            return Image(out, image.center, image.size)
         lors, n, ld = _soa(data.soa, 7)
         if n:
             _lib.call("srfx_siddon_osem_subset", _ptr(lors), n, ld, _ptr(x), self._grid(image),
                       self.tof_bin, self.tof_sigma2, _ptr(out), _lib.SRFX_ACCUMULATE, _stream())
         return Image(out, image.center, image.size)
+
+    def make_plans(self, image, data, tile=None, chunk=None):
+        """Tiled Siddon plan of data's LORs for images laid out like image
+        (SiddonPlan), as {"siddon": plan} (None without LORs)."""
+        from .plan import DEFAULT_SIDDON_CHUNK, DEFAULT_SIDDON_TILE, SiddonPlan
+        x = _world(image)
+        if data.soa.shape[1] == 0:
+            return {"siddon": None}
+        return {"siddon": SiddonPlan(data.soa, list(x.shape), image.center, image.size,
+                                     self.tof_bin, self.tof_sigma2, strides=x.stride(),
+                                     tile=tile or DEFAULT_SIDDON_TILE,
+                                     chunk=chunk or DEFAULT_SIDDON_CHUNK)}
+
 
     def maplors(self, data, image, out=None):
         x = _world(image)

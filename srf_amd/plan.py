@@ -12,6 +12,10 @@ do not change between iterations (graph/reconstruction/worker.py:79-85).
 The plan keeps a reference to the LOR tensor: it must not be modified while
 the plan lives.  Images use the plan's grid layout: contiguous [gz][gy][gx] in
 the op frame, or the element strides given at construction.
+
+``SiddonPlan`` (srfx_siddon_plan_*) does the same for the Siddon model's
+backprojection: each LOR's walk is cut into per-tile runs once, and every
+backprojection accumulates the runs of a tile in LDS.
 """
 import ctypes
 
@@ -178,6 +182,120 @@ class TorPlan:
     def close(self):
         if getattr(self, "_h", None) and self._h.value:
             _lib.call("srfx_tor_plan_destroy", self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+DEFAULT_SIDDON_TILE = (16, 16, 16)
+DEFAULT_SIDDON_CHUNK = 2048
+
+
+class SiddonPlan:
+    """Tiled Siddon backprojection over a fixed LOR set (srfx_siddon_plan_*).
+
+    The plan cuts each LOR's walk (siddon.cu.cc:293-331) into runs, one per
+    image tile of ``tile`` voxels it crosses, and lists the runs by tile; its
+    backprojections accumulate each tile in LDS and flush it with coalesced
+    atomics.  Same voxels and segment values as the untiled ops
+    (siddon.cc:28-43 semantics), summed in another order.  OSEM reuses one
+    plan per subset for every iteration.
+
+    The plan keeps a reference to the LOR tensor (7 x N SoA: x1..z2, tof): it
+    must not be modified while the plan lives.  Images are (nx, ny, nz)
+    world-frame tensors with the plan's element strides.
+    """
+
+    def __init__(self, lors, grid, center, size, tof_bin=1.0e4, tof_sigma2=1.0e4,
+                 strides=None, tile=DEFAULT_SIDDON_TILE, chunk=DEFAULT_SIDDON_CHUNK, block=256):
+        self.lors, self.n, ld = _soa(lors, 7)
+        grid = [int(g) for g in grid]
+        if strides is None:
+            strides = (grid[1] * grid[2], grid[2], 1)
+        self.strides = tuple(int(s) for s in strides)
+        self.grid, self._cgrid = _geom(grid, center, size, self.strides)
+        # strides of a dense layout (some axis order of a contiguous array)
+        order = sorted(range(3), key=lambda k: self.strides[k])
+        dense, step = True, 1
+        for k in order:
+            dense &= self.strides[k] == step
+            step *= grid[k]
+        self._dense = dense
+        self.tile = tuple(int(t) for t in tile)
+        t = _lib.SrfxSiddonTile(*self.tile, int(chunk), int(block))
+        h = ctypes.c_void_p()
+        _lib.call("srfx_siddon_plan_create", _ptr(self.lors), self.n, ld, self._cgrid,
+                  float(tof_bin), float(tof_sigma2), t, ctypes.byref(h), _stream())
+        self._h = h
+
+    def info(self):
+        """(number of (LOR, tile) runs, number of workgroup chunks)."""
+        a, b = ctypes.c_int64(), ctypes.c_int32()
+        _lib.call("srfx_siddon_plan_info", self._h, ctypes.byref(a), ctypes.byref(b))
+        return int(a.value), int(b.value)
+
+    def device_bytes(self):
+        """Device memory held by the plan: 48 bytes per run, 52 per LOR (ray
+        record, projection buffer), 16 per chunk, 8 per tile."""
+        nr, nc = self.info()
+        nt = 1
+        for g, t in zip(self.grid, self.tile):
+            nt *= -(-g // min(t, g))
+        return 48 * nr + 52 * self.n + 16 * nc + 8 * nt
+
+    def _image(self, image, name):
+        """image: a (nx, ny, nz) tensor with the plan's strides, or, for a
+        plan whose strides describe a dense layout, any contiguous tensor of
+        that layout (e.g. [gz][gy][gx] for strides (1, gx, gx gy))."""
+        _check_dev(image, name)
+        g = self.grid
+        if list(image.shape) == g and tuple(image.stride()) == self.strides:
+            return image
+        if self._dense and image.is_contiguous() and image.numel() == g[0] * g[1] * g[2]:
+            return image
+        raise ValueError(f"{name} must be a {tuple(g)} tensor with strides {self.strides}")
+
+    def _out(self, out, like):
+        if out is None:
+            out = torch.empty_strided(self.grid, self.strides, dtype=torch.float32,
+                                      device=like.device)
+        return self._image(out, "out")
+
+    def backprojection(self, lors_value, out=None, accumulate=False):
+        v = _vec(lors_value, self.n, "lors_value")
+        out = self._out(out, v)
+        _lib.call("srfx_siddon_plan_backproject", self._h, _ptr(v), _ptr(out),
+                  _lib.SRFX_ACCUMULATE if accumulate else 0, _stream())
+        return out
+
+    def backprojection_adjoint(self, g, out=None, accumulate=False):
+        v = _vec(g, self.n, "g")
+        out = self._out(out, v)
+        _lib.call("srfx_siddon_plan_backproject_adjoint", self._h, _ptr(v), _ptr(out),
+                  _lib.SRFX_ACCUMULATE if accumulate else 0, _stream())
+        return out
+
+    def osem_subset(self, image, out=None, accumulate=False):
+        """Backprojection of 1/p (p > 0) with p the projection of image."""
+        image = self._image(image, "image")
+        out = self._out(out, image)
+        _lib.call("srfx_siddon_plan_osem_subset", self._h, _ptr(image), _ptr(out),
+                  _lib.SRFX_ACCUMULATE if accumulate else 0, _stream())
+        return out
+
+    def projection(self):
+        """The projection of the last osem_subset call (float32 [n])."""
+        out = torch.empty(self.n, dtype=torch.float32, device=self.lors.device)
+        _lib.call("srfx_siddon_plan_projection", self._h, _ptr(out), _stream())
+        return out
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            _lib.call("srfx_siddon_plan_destroy", self._h)
             self._h = ctypes.c_void_p()
 
     def __del__(self):
