@@ -329,6 +329,18 @@ int srfx_siddon_plan_create(const float *lors, int64_t n, int64_t ld, srfx_grid 
                             float tof_bin, float tof_sigma2, srfx_siddon_tile tile,
                             srfx_siddon_plan *plan, void *stream);
 
+/* The plans of consecutive LOR ranges of one LOR array, built together:
+ * plans[k] is the plan srfx_siddon_plan_create would build for LORs
+ * bounds[k] .. bounds[k + 1] - 1 (bounds: nplans + 1 non-decreasing host
+ * values, 0 <= bounds[0], bounds[nplans] <= n), e.g. the OSEM subsets
+ * (contiguous rows, graph/reconstruction/worker.py:79-85).  One count pass
+ * and one fill pass over all LORs, two host synchronisations and two shared
+ * device blocks; each plan is destroyed on its own. */
+int srfx_siddon_plan_create_subsets(const float *lors, int64_t n, int64_t ld, srfx_grid grid,
+                                    float tof_bin, float tof_sigma2, srfx_siddon_tile tile,
+                                    int32_t nplans, const int64_t *bounds,
+                                    srfx_siddon_plan *plans, void *stream);
+
 /* Runs (LOR, tile pairs) and workgroup chunks of a plan. */
 int srfx_siddon_plan_info(srfx_siddon_plan plan, int64_t *nruns, int32_t *nchunks);
 

@@ -125,6 +125,9 @@ _PROTOS = {
     "srfx_tor_plan_scales": [_P, _P, _P],
     "srfx_siddon_plan_create": [_P, _I64, _I64, SrfxGrid, _F, _F, SrfxSiddonTile,
                                 ctypes.POINTER(ctypes.c_void_p), _P],
+    "srfx_siddon_plan_create_subsets": [_P, _I64, _I64, SrfxGrid, _F, _F, SrfxSiddonTile, _I32,
+                                        ctypes.POINTER(_I64), ctypes.POINTER(ctypes.c_void_p),
+                                        _P],
     "srfx_siddon_plan_info": [_P, ctypes.POINTER(_I64), ctypes.POINTER(_I32)],
     "srfx_siddon_plan_destroy": [_P],
     "srfx_siddon_plan_backproject": [_P, _P, _P, _U32, _P],

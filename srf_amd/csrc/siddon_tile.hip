@@ -21,6 +21,7 @@
 // (FP64 pairs), [weight, steps, -, -]; listing (3 float4 per run):
 // [ray, x | y << 16, z | len << 16, -], [nextT0 nextT1], [nextT2 last_t].
 #include <algorithm>
+#include <atomic>
 #include <vector>
 
 #include "siddon_core.h"
@@ -69,20 +70,35 @@ __device__ __forceinline__ int tile_steps(int pos, int step, int t)
     return step > 0 ? t - pos % t : (step < 0 ? pos % t + 1 : 0x7fffffff);
 }
 
-// The plan's two passes over the rays, one lane per ray: the walk of
+// The plan set's two passes over the rays of all its plans (plan k holds
+// rays bounds[k] .. bounds[k + 1] - 1), one lane per ray: the walk of
 // siddon_core.h's walk() (same argmin, same FP64 updates, same end
 // conditions), cut into runs at tile boundaries.  A run with at least one
 // kept segment (all of them without TOF; t^2 / sigma^2 < 9 with TOF) is a
-// listing.  Count pass (FILL false): writes the ray records and counts each
-// tile's listings.  Fill pass: writes each listing at its tile's cursor.
+// listing.  Count pass (FILL false): writes the ray records and counts the
+// listings of each (plan, tile).  Fill pass: writes each listing at its
+// (plan, tile) cursor, with the ray's index inside its plan.
 template <bool FILL, bool TOF>
 __global__ __launch_bounds__(256) void sid_plan_pass(SidParams p, SidTileGeom g,
                                                      unsigned long long *tiles, float4 *rays,
-                                                     float4 *runs, unsigned long long nruns)
+                                                     float4 *runs, unsigned long long nruns,
+                                                     const int64_t *__restrict__ bounds,
+                                                     int nplans, int ntiles)
 {
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= p.n)
+    if (e >= p.n || e < bounds[0] || e >= bounds[nplans])
         return;
+    // the plan of ray e: the last k with bounds[k] <= e
+    int lo = 0, hi = nplans - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (bounds[mid] <= e)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    unsigned long long *ptiles = tiles + (int64_t)lo * ntiles;
+    const int el = (int)(e - bounds[lo]);
     Walk w;
     setup_walk<kBack>(p, e, w);
     if (!FILL) {
@@ -112,13 +128,13 @@ __global__ __launch_bounds__(256) void sid_plan_pass(SidParams p, SidTileGeom g,
     auto emit = [&]() {
         const int tile = tile_id(g, sx, sy, sz);
         if (!FILL) {
-            atomicAdd(tiles + tile, 1ull);
+            atomicAdd(ptiles + tile, 1ull);
         } else {
-            const unsigned long long slot = atomicAdd(tiles + tile, 1ull);
+            const unsigned long long slot = atomicAdd(ptiles + tile, 1ull);
             if (slot >= nruns)
                 return; // never: the fill pass finds the runs the count pass counted
             float4 *o = runs + kRunF4 * (int64_t)slot;
-            o[0] = make_float4(__int_as_float((int)e), __int_as_float(sx | (sy << 16)),
+            o[0] = make_float4(__int_as_float(el), __int_as_float(sx | (sy << 16)),
                                __int_as_float(sz | (len << 16)), 0.0f);
             o[1] = pack_d2(s0, s1);
             o[2] = pack_d2(s2, slast);
@@ -266,27 +282,32 @@ __global__ __launch_bounds__(256) void sid_tile_scatter(SidParams p, SidTileGeom
 using namespace srfx;
 
 struct srfx_siddon_plan_s {
+    // device blocks shared by the plans of one set, freed with the last plan
+    struct Arena {
+        void *mem[2];
+        std::atomic<int> refs;
+    };
     SidParams p{};
     srfx_grid grid{};
     SidTileGeom g{};
-    int ntiles = 0, nchunks = 0, block = 256;
+    int nchunks = 0, block = 256;
     int64_t nruns = 0;
     float4 *rays = nullptr;
     float4 *runs = nullptr;
     SidChunk *chunks = nullptr;
     float *proj = nullptr;
-    unsigned long long *tiles = nullptr;
+    Arena *arena = nullptr;
 };
 
 namespace {
 
 void plan_release(srfx_siddon_plan_s *pl)
 {
-    (void)hipFree(pl->rays);
-    (void)hipFree(pl->runs);
-    (void)hipFree(pl->chunks);
-    (void)hipFree(pl->proj);
-    (void)hipFree(pl->tiles);
+    if (pl->arena && pl->arena->refs.fetch_sub(1) == 1) {
+        (void)hipFree(pl->arena->mem[0]);
+        (void)hipFree(pl->arena->mem[1]);
+        delete pl->arena;
+    }
     delete pl;
 }
 
@@ -315,15 +336,17 @@ int sid_setup(SidParams &p, const float *lors, int64_t n, int64_t ld, const srfx
 }
 
 template <bool FILL>
-void launch_pass(const srfx_siddon_plan_s *pl, hipStream_t st)
+void launch_pass(const SidParams &p, const SidTileGeom &g, unsigned long long *tiles,
+                 float4 *rays, float4 *runs, int64_t nruns, const int64_t *bounds, int nplans,
+                 int ntiles, hipStream_t st)
 {
-    const dim3 grid(blocks_for(pl->p.n, 256)), block(256);
-    if (pl->p.use_tof)
-        hipLaunchKernelGGL((sid_plan_pass<FILL, true>), grid, block, 0, st, pl->p, pl->g,
-                           pl->tiles, pl->rays, pl->runs, (unsigned long long)pl->nruns);
+    const dim3 grid(blocks_for(p.n, 256)), block(256);
+    if (p.use_tof)
+        hipLaunchKernelGGL((sid_plan_pass<FILL, true>), grid, block, 0, st, p, g, tiles, rays,
+                           runs, (unsigned long long)nruns, bounds, nplans, ntiles);
     else
-        hipLaunchKernelGGL((sid_plan_pass<FILL, false>), grid, block, 0, st, pl->p, pl->g,
-                           pl->tiles, pl->rays, pl->runs, (unsigned long long)pl->nruns);
+        hipLaunchKernelGGL((sid_plan_pass<FILL, false>), grid, block, 0, st, p, g, tiles, rays,
+                           runs, (unsigned long long)nruns, bounds, nplans, ntiles);
 }
 
 int plan_scatter(const srfx_siddon_plan_s *pl, bool adj, const float *vals, float *image,
@@ -354,6 +377,150 @@ int plan_scatter(const srfx_siddon_plan_s *pl, bool adj, const float *vals, floa
     return launch_status();
 }
 
+size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+// The plans of rays bounds[k] .. bounds[k + 1] - 1 of one LOR array, built
+// together: one count pass and one fill pass over all the rays, two host
+// synchronisations and two device blocks for the whole set.
+int build_set(const float *lors, int64_t n, int64_t ld, const srfx_grid &grid, float tof_bin,
+              float tof_sigma2, const srfx_siddon_tile &tile, int K, const int64_t *bounds,
+              srfx_siddon_plan *plans, hipStream_t st)
+{
+    if (K < 1 || !plans || !bounds)
+        return SRFX_EINVAL;
+    for (int k = 0; k < K; k++)
+        plans[k] = nullptr;
+    if (bounds[0] < 0 || bounds[K] > n)
+        return SRFX_EINVAL;
+    for (int k = 0; k < K; k++)
+        if (bounds[k + 1] < bounds[k] || bounds[k + 1] - bounds[k] > (int64_t)INT32_MAX)
+            return SRFX_EINVAL;
+    const int t[3] = {tile.tx, tile.ty, tile.tz};
+    for (int c = 0; c < 3; c++)
+        if (t[c] <= 0 || grid.shape[c] >= 65536)
+            return SRFX_EINVAL;
+    if ((int64_t)t[0] * t[1] * t[2] > kMaxTileVox || tile.chunk < 0 ||
+        !(tile.block == 0 || tile.block == 64 || tile.block == 128 || tile.block == 256))
+        return SRFX_EINVAL;
+    SidParams p;
+    int rc = sid_setup(p, lors, n, ld, grid, tof_bin, tof_sigma2);
+    if (rc)
+        return rc;
+    const int chunk = tile.chunk ? tile.chunk : 2048;
+    SidTileGeom g{};
+    for (int c = 0; c < 3; c++) {
+        g.t[c] = std::min(t[c], grid.shape[c]);
+        g.nt[c] = (grid.shape[c] + g.t[c] - 1) / g.t[c];
+    }
+    // LDS order of a tile's voxels: the image's own axis order (fastest
+    // output stride first), so that a flush writes runs of consecutive voxels
+    int ord[3] = {0, 1, 2};
+    std::stable_sort(ord, ord + 3, [&](int a, int b) { return p.o[a] < p.o[b]; });
+    int ls = 1;
+    for (int k = 0; k < 3; k++) {
+        g.ord[k] = ord[k];
+        g.ls[ord[k]] = ls;
+        ls *= g.t[ord[k]];
+    }
+    g.tvox = ls;
+    const int ntiles = g.nt[0] * g.nt[1] * g.nt[2];
+    const int64_t ncnt = (int64_t)K * ntiles;
+    // block 0: ray records, projection buffer, (plan, tile) counters, bounds
+    const size_t nr = (size_t)std::max<int64_t>(n, 1);
+    const size_t o_proj = align256(sizeof(float4) * kRayF4 * nr);
+    const size_t o_cnt = o_proj + align256(sizeof(float) * nr);
+    const size_t o_bnd = o_cnt + align256(sizeof(unsigned long long) * ncnt);
+    const size_t b0 = o_bnd + align256(sizeof(int64_t) * (K + 1));
+    char *blk0 = nullptr, *blk1 = nullptr;
+    if (hipMalloc(&blk0, b0) != hipSuccess)
+        return SRFX_ENOMEM;
+    auto fail = [&](int code) {
+        (void)hipStreamSynchronize(st);
+        (void)hipFree(blk0);
+        (void)hipFree(blk1);
+        return code;
+    };
+    float4 *rays = (float4 *)blk0;
+    float *proj = (float *)(blk0 + o_proj);
+    auto *cnt = (unsigned long long *)(blk0 + o_cnt);
+    auto *dbounds = (int64_t *)(blk0 + o_bnd);
+    std::vector<unsigned long long> hc(ncnt, 0ull);
+    const bool any = bounds[K] > bounds[0];
+    if (any) {
+        if (hipMemsetAsync(cnt, 0, sizeof(unsigned long long) * ncnt, st) != hipSuccess ||
+            hipMemcpyAsync(dbounds, bounds, sizeof(int64_t) * (K + 1), hipMemcpyHostToDevice,
+                           st) != hipSuccess)
+            return fail(SRFX_ELAUNCH);
+        launch_pass<false>(p, g, cnt, rays, nullptr, 0, dbounds, K, ntiles, st);
+        if (launch_status() ||
+            hipMemcpyAsync(hc.data(), cnt, sizeof(unsigned long long) * ncnt,
+                           hipMemcpyDeviceToHost, st) != hipSuccess ||
+            hipStreamSynchronize(st) != hipSuccess)
+            return fail(SRFX_ELAUNCH);
+    }
+    // (plan, tile) offsets into one run array, then each plan's chunks of at
+    // most `chunk` runs of one tile (starts relative to the plan's runs)
+    std::vector<unsigned long long> off(ncnt);
+    std::vector<SidChunk> chunks;
+    std::vector<int64_t> run0(K + 1), chunk0(K + 1);
+    unsigned long long total = 0;
+    for (int k = 0; k < K; k++) {
+        run0[k] = (int64_t)total;
+        chunk0[k] = (int64_t)chunks.size();
+        for (int tt = 0; tt < ntiles; tt++) {
+            const unsigned long long c = hc[(size_t)k * ntiles + tt];
+            off[(size_t)k * ntiles + tt] = total;
+            for (unsigned long long s0 = 0; s0 < c; s0 += (unsigned long long)chunk)
+                chunks.push_back({(int64_t)(total + s0) - run0[k], tt,
+                                  (int)std::min<unsigned long long>(chunk, c - s0)});
+            total += c;
+        }
+    }
+    run0[K] = (int64_t)total;
+    chunk0[K] = (int64_t)chunks.size();
+    float4 *runs = nullptr;
+    SidChunk *dchunks = nullptr;
+    if (total > 0) {
+        const size_t o_ch = align256(sizeof(float4) * kRunF4 * total);
+        if (hipMalloc(&blk1, o_ch + sizeof(SidChunk) * chunks.size()) != hipSuccess)
+            return fail(SRFX_ENOMEM);
+        runs = (float4 *)blk1;
+        dchunks = (SidChunk *)(blk1 + o_ch);
+        if (hipMemcpyAsync(cnt, off.data(), sizeof(unsigned long long) * ncnt,
+                           hipMemcpyHostToDevice, st) != hipSuccess ||
+            hipMemcpyAsync(dchunks, chunks.data(), sizeof(SidChunk) * chunks.size(),
+                           hipMemcpyHostToDevice, st) != hipSuccess)
+            return fail(SRFX_ELAUNCH);
+        launch_pass<true>(p, g, cnt, rays, runs, (int64_t)total, dbounds, K, ntiles, st);
+        if (launch_status())
+            return fail(SRFX_ELAUNCH);
+    }
+    if (hipStreamSynchronize(st) != hipSuccess)
+        return fail(SRFX_ELAUNCH);
+    auto *arena = new srfx_siddon_plan_s::Arena;
+    arena->mem[0] = blk0;
+    arena->mem[1] = blk1;
+    arena->refs.store(K);
+    for (int k = 0; k < K; k++) {
+        auto *pl = new srfx_siddon_plan_s;
+        pl->p = p;
+        pl->p.lors = n > 0 ? lors + bounds[k] : lors;
+        pl->p.n = bounds[k + 1] - bounds[k];
+        pl->grid = grid;
+        pl->g = g;
+        pl->block = tile.block ? tile.block : 256;
+        pl->rays = rays + kRayF4 * bounds[k];
+        pl->proj = proj + bounds[k];
+        pl->runs = runs ? runs + kRunF4 * run0[k] : nullptr;
+        pl->nruns = run0[k + 1] - run0[k];
+        pl->chunks = dchunks ? dchunks + chunk0[k] : nullptr;
+        pl->nchunks = (int)(chunk0[k + 1] - chunk0[k]);
+        pl->arena = arena;
+        plans[k] = pl;
+    }
+    return SRFX_OK;
+}
+
 } // namespace
 
 extern "C" int srfx_siddon_plan_create(const float *lors, int64_t n, int64_t ld, srfx_grid grid,
@@ -362,103 +529,21 @@ extern "C" int srfx_siddon_plan_create(const float *lors, int64_t n, int64_t ld,
 {
     if (!plan)
         return SRFX_EINVAL;
-    *plan = nullptr;
-    const int t[3] = {tile.tx, tile.ty, tile.tz};
-    for (int c = 0; c < 3; c++)
-        if (t[c] <= 0 || grid.shape[c] >= 65536)
-            return SRFX_EINVAL;
-    if ((int64_t)t[0] * t[1] * t[2] > kMaxTileVox || tile.chunk < 0 ||
-        !(tile.block == 0 || tile.block == 64 || tile.block == 128 || tile.block == 256))
+    const int64_t bounds[2] = {0, n < 0 ? 0 : n};
+    if (n < 0)
         return SRFX_EINVAL;
-    auto *pl = new srfx_siddon_plan_s;
-    int rc = sid_setup(pl->p, lors, n, ld, grid, tof_bin, tof_sigma2);
-    if (rc) {
-        delete pl;
-        return rc;
-    }
-    pl->grid = grid;
-    pl->block = tile.block ? tile.block : 256;
-    const int chunk = tile.chunk ? tile.chunk : 2048;
-    SidTileGeom &g = pl->g;
-    for (int c = 0; c < 3; c++) {
-        g.t[c] = std::min(t[c], grid.shape[c]);
-        g.nt[c] = (grid.shape[c] + g.t[c] - 1) / g.t[c];
-    }
-    // LDS order of a tile's voxels: the image's own axis order (fastest
-    // output stride first), so that a flush writes runs of consecutive voxels
-    const int o[3] = {pl->p.o[0], pl->p.o[1], pl->p.o[2]};
-    int ord[3] = {0, 1, 2};
-    std::stable_sort(ord, ord + 3, [&](int a, int b) { return o[a] < o[b]; });
-    int ls = 1;
-    for (int k = 0; k < 3; k++) {
-        g.ord[k] = ord[k];
-        g.ls[ord[k]] = ls;
-        ls *= g.t[ord[k]];
-    }
-    g.tvox = ls;
-    pl->ntiles = g.nt[0] * g.nt[1] * g.nt[2];
-    hipStream_t st = (hipStream_t)stream;
-    const size_t nb = (size_t)std::max<int64_t>(n, 1);
-    if (hipMalloc(&pl->rays, sizeof(float4) * kRayF4 * nb) != hipSuccess ||
-        hipMalloc(&pl->proj, sizeof(float) * nb) != hipSuccess ||
-        hipMalloc(&pl->tiles, sizeof(unsigned long long) * pl->ntiles) != hipSuccess) {
-        plan_release(pl);
-        return SRFX_ENOMEM;
-    }
-    std::vector<unsigned long long> cnt(pl->ntiles, 0ull);
-    if (n > 0) {
-        if (hipMemsetAsync(pl->tiles, 0, sizeof(unsigned long long) * pl->ntiles, st) !=
-            hipSuccess) {
-            plan_release(pl);
-            return SRFX_ELAUNCH;
-        }
-        launch_pass<false>(pl, st);
-        if (launch_status() ||
-            hipMemcpyAsync(cnt.data(), pl->tiles, sizeof(unsigned long long) * pl->ntiles,
-                           hipMemcpyDeviceToHost, st) != hipSuccess ||
-            hipStreamSynchronize(st) != hipSuccess) {
-            plan_release(pl);
-            return SRFX_ELAUNCH;
-        }
-    }
-    // tile offsets, then chunks of at most `chunk` runs of one tile
-    std::vector<unsigned long long> off(pl->ntiles);
-    std::vector<SidChunk> chunks;
-    unsigned long long total = 0;
-    for (int k = 0; k < pl->ntiles; k++) {
-        off[k] = total;
-        for (unsigned long long s = 0; s < cnt[k]; s += (unsigned long long)chunk)
-            chunks.push_back({(int64_t)(total + s), k,
-                              (int)std::min<unsigned long long>(chunk, cnt[k] - s)});
-        total += cnt[k];
-    }
-    pl->nruns = (int64_t)total;
-    pl->nchunks = (int)chunks.size();
-    if (total > 0) {
-        if (hipMalloc(&pl->runs, sizeof(float4) * kRunF4 * total) != hipSuccess ||
-            hipMalloc(&pl->chunks, sizeof(SidChunk) * chunks.size()) != hipSuccess) {
-            plan_release(pl);
-            return SRFX_ENOMEM;
-        }
-        if (hipMemcpyAsync(pl->tiles, off.data(), sizeof(unsigned long long) * pl->ntiles,
-                           hipMemcpyHostToDevice, st) != hipSuccess ||
-            hipMemcpyAsync(pl->chunks, chunks.data(), sizeof(SidChunk) * chunks.size(),
-                           hipMemcpyHostToDevice, st) != hipSuccess) {
-            plan_release(pl);
-            return SRFX_ELAUNCH;
-        }
-        launch_pass<true>(pl, st);
-        if (launch_status()) {
-            plan_release(pl);
-            return SRFX_ELAUNCH;
-        }
-    }
-    if (hipStreamSynchronize(st) != hipSuccess) {
-        plan_release(pl);
-        return SRFX_ELAUNCH;
-    }
-    *plan = pl;
-    return SRFX_OK;
+    return build_set(lors, n, ld, grid, tof_bin, tof_sigma2, tile, 1, bounds, plan,
+                     (hipStream_t)stream);
+}
+
+extern "C" int srfx_siddon_plan_create_subsets(const float *lors, int64_t n, int64_t ld,
+                                               srfx_grid grid, float tof_bin, float tof_sigma2,
+                                               srfx_siddon_tile tile, int32_t nplans,
+                                               const int64_t *bounds, srfx_siddon_plan *plans,
+                                               void *stream)
+{
+    return build_set(lors, n, ld, grid, tof_bin, tof_sigma2, tile, nplans, bounds, plans,
+                     (hipStream_t)stream);
 }
 
 extern "C" int srfx_siddon_plan_info(srfx_siddon_plan plan, int64_t *nruns, int32_t *nchunks)

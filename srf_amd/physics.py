@@ -401,6 +401,22 @@ class CompleteLoRsModel:
                                      chunk=chunk or DEFAULT_SIDDON_CHUNK)}
 
 
+    def make_subset_plans(self, image, data, nb_subsets, tile=None, chunk=None):
+        """make_plans for every OSEM subset of ``data`` (contiguous rows,
+        recon.subset_bounds), built together (srfx_siddon_plan_create_subsets).
+        Returns a list over subsets of {"siddon": SiddonPlan}."""
+        from .plan import DEFAULT_SIDDON_CHUNK, DEFAULT_SIDDON_TILE, SiddonPlan
+        from .recon import subset_bounds
+        x = _world(image)
+        n = len(data)
+        K = int(nb_subsets)
+        bounds = [subset_bounds(n, k, K)[0] for k in range(K)] + [n]
+        plans = SiddonPlan.subsets(data.soa, bounds, list(x.shape), image.center, image.size,
+                                   self.tof_bin, self.tof_sigma2, strides=x.stride(),
+                                   tile=tile or DEFAULT_SIDDON_TILE,
+                                   chunk=chunk or DEFAULT_SIDDON_CHUNK)
+        return [{"siddon": pl} for pl in plans]
+
     def maplors(self, data, image, out=None):
         x = _world(image)
         if out is None:

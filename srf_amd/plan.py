@@ -213,6 +213,44 @@ class SiddonPlan:
     def __init__(self, lors, grid, center, size, tof_bin=1.0e4, tof_sigma2=1.0e4,
                  strides=None, tile=DEFAULT_SIDDON_TILE, chunk=DEFAULT_SIDDON_CHUNK, block=256):
         self.lors, self.n, ld = _soa(lors, 7)
+        self._layout(grid, center, size, strides, tile)
+        t = _lib.SrfxSiddonTile(*self.tile, int(chunk), int(block))
+        h = ctypes.c_void_p()
+        _lib.call("srfx_siddon_plan_create", _ptr(self.lors), self.n, ld, self._cgrid,
+                  float(tof_bin), float(tof_sigma2), t, ctypes.byref(h), _stream())
+        self._h = h
+
+    @classmethod
+    def subsets(cls, lors, bounds, grid, center, size, tof_bin=1.0e4, tof_sigma2=1.0e4,
+                strides=None, tile=DEFAULT_SIDDON_TILE, chunk=DEFAULT_SIDDON_CHUNK, block=256):
+        """Plans of the LOR ranges bounds[k]:bounds[k + 1] of one LOR tensor
+        (the OSEM subsets), built together by srfx_siddon_plan_create_subsets.
+        Returns one SiddonPlan per range."""
+        full, n, ld = _soa(lors, 7)
+        bounds = [int(b) for b in bounds]
+        K = len(bounds) - 1
+        if K < 1 or bounds[0] < 0 or bounds[-1] > n or any(
+                b1 < b0 for b0, b1 in zip(bounds, bounds[1:])):
+            raise ValueError("bounds must be non-decreasing LOR offsets within the LORs")
+        proto = cls.__new__(cls)
+        proto._layout(grid, center, size, strides, tile)
+        t = _lib.SrfxSiddonTile(*proto.tile, int(chunk), int(block))
+        hs = (ctypes.c_void_p * K)()
+        bs = (ctypes.c_int64 * (K + 1))(*bounds)
+        _lib.call("srfx_siddon_plan_create_subsets", _ptr(full), n, ld, proto._cgrid,
+                  float(tof_bin), float(tof_sigma2), t, K, bs, hs, _stream())
+        out = []
+        for k in range(K):
+            pl = cls.__new__(cls)
+            pl.__dict__.update({a: getattr(proto, a) for a in
+                                ("strides", "grid", "_cgrid", "_dense", "tile")})
+            pl.lors = full[:, bounds[k]:bounds[k + 1]]
+            pl.n = bounds[k + 1] - bounds[k]
+            pl._h = ctypes.c_void_p(hs[k])
+            out.append(pl)
+        return out
+
+    def _layout(self, grid, center, size, strides, tile):
         grid = [int(g) for g in grid]
         if strides is None:
             strides = (grid[1] * grid[2], grid[2], 1)
@@ -226,11 +264,6 @@ class SiddonPlan:
             step *= grid[k]
         self._dense = dense
         self.tile = tuple(int(t) for t in tile)
-        t = _lib.SrfxSiddonTile(*self.tile, int(chunk), int(block))
-        h = ctypes.c_void_p()
-        _lib.call("srfx_siddon_plan_create", _ptr(self.lors), self.n, ld, self._cgrid,
-                  float(tof_bin), float(tof_sigma2), t, ctypes.byref(h), _stream())
-        self._h = h
 
     def info(self):
         """(number of (LOR, tile) runs, number of workgroup chunks)."""

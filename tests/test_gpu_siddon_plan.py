@@ -172,3 +172,33 @@ def test_osem_driver_vs_oracle(cuda, fused):
     e = H.rel_l2(x.cpu().numpy(), ref)
     print(f"tiled Siddon OSEM 2 it x 4 subsets fused={fused}: rel_l2 {e:.2e}")
     assert e <= 1e-4
+
+
+def test_subset_plans_equal_single_plans(cuda, case):
+    """srfx_siddon_plan_create_subsets builds, for each LOR range, the plan
+    srfx_siddon_plan_create builds for it: same runs and chunks, and the same
+    backprojection; empty ranges included."""
+    from srf_amd.plan import SiddonPlan
+    lors, img = case
+    n = lors.shape[1]
+    bounds = [0, 700, 700, 2100, 3999, n]
+    L = _t(lors, cuda)
+    rng = np.random.default_rng(7)
+    v = _t(rng.uniform(0.5, 2.0, n).astype(np.float32), cuda)
+    set_plans = SiddonPlan.subsets(L, bounds, *GEO, 6.0, 400.0, strides=OPS_STRIDES,
+                                   tile=(8, 8, 8), chunk=300)
+    for k, pl in enumerate(set_plans):
+        lo, hi = bounds[k], bounds[k + 1]
+        one = SiddonPlan(L[:, lo:hi], *GEO, 6.0, 400.0, strides=OPS_STRIDES, tile=(8, 8, 8),
+                         chunk=300)
+        assert pl.info() == one.info()
+        a = pl.backprojection(v[lo:hi], out=_t(np.zeros_like(img), cuda)).cpu().numpy()
+        b = one.backprojection(v[lo:hi], out=_t(np.zeros_like(img), cuda)).cpu().numpy()
+        if hi > lo:
+            assert H.rel_l2(a, b) <= 1e-6
+        else:
+            assert not a.any() and not b.any()
+        one.close()
+    # plans of a set are freed one by one, in any order
+    for pl in reversed(set_plans):
+        pl.close()
