@@ -46,11 +46,13 @@ def parse():
     ap.add_argument("--unfused", action="store_true",
                     help="time separate projection + backprojection kernels")
     ap.add_argument("--workload", default="tof_tor_c2",
-                    choices=["tof_tor_c2", "siddon_c1", "autograd_c3", "batched_c4",
-                             "large_c5", "hbm_copy", "psf_c2", "io_c2"])
+                    choices=["tof_tor_c2", "tof_tor_c2_kw8", "siddon_c1", "autograd_c3",
+                             "batched_c4", "large_c5", "hbm_copy", "psf_c2", "io_c2"],
+                    help="tof_tor_c2_kw8: the C2 workload at kernel_width 8 mm (every "
+                         "event's candidate box exceeds 4 voxels: the chord-walk kernels)")
     ap.add_argument("--autotune", action="store_true",
                     help="tune the projection schedule and the tile shape on a sample of the "
-                         "events before timing (default: the fixed defaults, tile 8x32x20, "
+                         "events before timing (default: the fixed defaults, tile 12x32x12, "
                          "chunk 2048; the tuner's candidates lie within a few percent of "
                          "each other, inside its timing noise)")
     ap.add_argument("--no-sort", dest="sort", action="store_false",
@@ -167,12 +169,12 @@ def main():
                           "part of this build (use --gpus 1)"}))
         sys.exit(2)
     sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
-    if args.workload != "tof_tor_c2":
+    if args.workload not in ("tof_tor_c2", "tof_tor_c2_kw8"):
         import bench_workloads
         bench_workloads.run(args)
         return
     import torch
-    from srf_amd import effmap, physics, synth, preprocess
+    from srf_amd import _lib, effmap, physics, synth, preprocess
     from srf_amd.data import Image, split_from_groups
     from srf_amd.plan import DEFAULT_CHUNK, DEFAULT_TILE
     from srf_amd.recon import OSEM, subset_view
@@ -183,6 +185,8 @@ def main():
     scanner = CylindricalScanner()
     t0 = time.time()
     ph, ev, meta = synth.c2(n_events=args.events, device=dev)
+    if args.workload == "tof_tor_c2_kw8":
+        meta["kernel_width"] = 8.0
     torch.cuda.synchronize()
     log(f"generated {args.events} events in {time.time() - t0:.1f}s")
     t0 = time.time()
@@ -325,17 +329,22 @@ def main():
                     for i in range(nb)]
         byts = float(np.mean([n * (LOR_BYTES + 4) + 4 * z + 4 * nvox for n, z in launches]))
         kernels = {}
+        fr = physics.group_frame(grid, center, size, (1, 1, 1), a)
+        box = _lib.tor_patch_info(fr[0], fr[2], meta["kernel_width"])[1]
+        walk = (f"candidate-box walk, box {box}" if box else "chord walk (box wider than 4)")
         for name, sel, kname in (("projection", lambda t: t[0].elapsed_time(t[1]),
-                                  "record_proj_kernel (candidate-box walk from the walk records, values into their value slots)"),
+                                  f"record_proj_kernel ({walk} from the walk records, values "
+                                  "into their value slots)"),
                                  ("backprojection", lambda t: t[1].elapsed_time(t[2]),
-                                  "tile_scatter<kBack> (FP64 LDS tile accumulation, coalesced flush)")):
+                                  f"tile_scatter<kBack> ({walk}; FP64 LDS tile accumulation, "
+                                  "coalesced flush)")):
             ms = float(np.mean([sel(t) for t in timings[a]]))
             ach = byts / (ms / 1e3) / 1e9
             kernels[name] = {"kernel": kname, "group": a, "mean_launch_ms": round(ms, 4),
                              "algorithmic_bytes_per_launch": int(byts),
                              "achieved_GBps": round(ach, 1),
                              "frac": round(ach / HBM_PEAK_GBS, 4)}
-        traffic = _measured_traffic("tof_tor_c2")
+        traffic = _measured_traffic(args.workload)
         for name in kernels:
             t = traffic.get(name)
             if t:
@@ -385,7 +394,7 @@ def main():
         "value_excl_plan_build": round(value_kernels, 2),
         "plan_build_ms": round(plan_s * 1e3, 2) if plan_s is not None else None,
         "plan_build_amortised_over_iterations": C2_ITERATIONS if plan_s is not None else None,
-        "config": {"workload": "tof_tor_c2", "image": list(grid), "voxel_mm": list(meta["voxel"]),
+        "config": {"workload": args.workload, "image": list(grid), "voxel_mm": list(meta["voxel"]),
                    "events": args.events, "subsets": args.subsets,
                    "step": "one OSEM iteration over all events",
                    "kernel_width_mm": meta["kernel_width"], "tof_fwhm_ps": 530,
