@@ -314,9 +314,13 @@ int srfx_tor_plan_scales(srfx_tor_plan plan, float *out, void *stream);
  * FP64 in LDS), and lists the runs by tile; a scatter workgroup takes up to
  * `chunk` runs of one tile (0 = 2048), accumulates them in LDS and flushes
  * the tile with one atomic per non-zero voxel.  `block`: 64, 128 or 256
- * (0 = 256).  Grid dimensions below 65536. */
+ * (0 = 256).  `sort` nonzero: the plan keeps a copy of its LORs sorted by
+ * direction and position (a radix sort on the device), which its passes and
+ * its OSEM projection read, so that the lanes of a wavefront trace nearly
+ * parallel rays through neighbouring voxels; values and outputs stay in the
+ * caller's LOR order.  Grid dimensions below 65536. */
 typedef struct srfx_siddon_tile {
-    int32_t tx, ty, tz, chunk, block;
+    int32_t tx, ty, tz, chunk, block, sort;
 } srfx_siddon_tile;
 
 typedef struct srfx_siddon_plan_s *srfx_siddon_plan;
@@ -355,11 +359,16 @@ int srfx_siddon_plan_backproject_adjoint(srfx_siddon_plan plan, const float *g, 
                                          uint32_t flags, void *stream);
 
 /* OSEM subset step of the plan's LORs, as srfx_siddon_osem_subset: the
- * projection (srfx_siddon_project) into a plan buffer, then the tiled
- * backprojection of 1/p for p > 0 into bp.  Calls on one plan must not
- * overlap. */
-int srfx_siddon_plan_osem_subset(srfx_siddon_plan plan, const float *image, float *bp,
-                                 uint32_t flags, void *stream);
+ * projection (srfx_siddon_project, over the sorted LOR copy when the plan
+ * has one) into a plan buffer, then the tiled backprojection of 1/p for
+ * p > 0 into bp.  With counts (n floats, the caller's LOR order; NULL = all
+ * ones) the scale is c/p, the backprojection of lors_value = p / c that
+ * list-mode data with counts feeds the reference's backprojection
+ * (siddon.cu.cc:363-371).  SRFX_OSEM_PROJECT_ONLY / SRFX_OSEM_SCATTER_ONLY
+ * run one half (the projection stays in the plan in between).  Calls on one
+ * plan must not overlap. */
+int srfx_siddon_plan_osem_subset(srfx_siddon_plan plan, const float *image,
+                                 const float *counts, float *bp, uint32_t flags, void *stream);
 
 /* The projection of the last srfx_siddon_plan_osem_subset call (n floats, LOR
  * order), copied to out.  No reference counterpart. */

@@ -74,9 +74,10 @@ class SrfxTile(ctypes.Structure):
 
 class SrfxSiddonTile(ctypes.Structure):
     """srfx_siddon_tile: image tiles of `tx` x `ty` x `tz` voxels, at most
-    `chunk` runs per workgroup, `block` threads per workgroup."""
+    `chunk` runs per workgroup, `block` threads per workgroup; `sort` keeps a
+    direction/position-sorted copy of the LORs."""
     _fields_ = [("tx", ctypes.c_int32), ("ty", ctypes.c_int32), ("tz", ctypes.c_int32),
-                ("chunk", ctypes.c_int32), ("block", ctypes.c_int32)]
+                ("chunk", ctypes.c_int32), ("block", ctypes.c_int32), ("sort", ctypes.c_int32)]
 
 
 class SrfxError(RuntimeError):
@@ -132,7 +133,7 @@ _PROTOS = {
     "srfx_siddon_plan_destroy": [_P],
     "srfx_siddon_plan_backproject": [_P, _P, _P, _U32, _P],
     "srfx_siddon_plan_backproject_adjoint": [_P, _P, _P, _U32, _P],
-    "srfx_siddon_plan_osem_subset": [_P, _P, _P, _U32, _P],
+    "srfx_siddon_plan_osem_subset": [_P, _P, _P, _P, _U32, _P],
     "srfx_siddon_plan_projection": [_P, _P, _P],
 }
 

@@ -24,6 +24,8 @@
 #include <atomic>
 #include <vector>
 
+#include <hipcub/hipcub.hpp>
+
 #include "siddon_core.h"
 
 namespace srfx {
@@ -70,47 +72,15 @@ __device__ __forceinline__ int tile_steps(int pos, int step, int t)
     return step > 0 ? t - pos % t : (step < 0 ? pos % t + 1 : 0x7fffffff);
 }
 
-// The plan set's two passes over the rays of all its plans (plan k holds
-// rays bounds[k] .. bounds[k + 1] - 1), one lane per ray: the walk of
-// siddon_core.h's walk() (same argmin, same FP64 updates, same end
-// conditions), cut into runs at tile boundaries.  A run with at least one
-// kept segment (all of them without TOF; t^2 / sigma^2 < 9 with TOF) is a
-// listing.  Count pass (FILL false): writes the ray records and counts the
-// listings of each (plan, tile).  Fill pass: writes each listing at its
-// (plan, tile) cursor, with the ray's index inside its plan.
-template <bool FILL, bool TOF>
-__global__ __launch_bounds__(256) void sid_plan_pass(SidParams p, SidTileGeom g,
-                                                     unsigned long long *tiles, float4 *rays,
-                                                     float4 *runs, unsigned long long nruns,
-                                                     const int64_t *__restrict__ bounds,
-                                                     int nplans, int ntiles)
+// Walk a ray as siddon_core.h's walk() does (same argmin, same FP64
+// updates, same end conditions), cut into runs at tile boundaries, and call
+// f(tile, x, y, z, len, nextT0, nextT1, nextT2, last_t) with each run's first
+// voxel and walk state for every run with at least one kept segment (all of
+// them without TOF; t^2 / sigma^2 < 9 with TOF).
+template <bool TOF, typename F>
+__device__ __forceinline__ void for_each_run(const SidParams &p, const SidTileGeom &g,
+                                             const Walk &w, F &&f)
 {
-    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= p.n || e < bounds[0] || e >= bounds[nplans])
-        return;
-    // the plan of ray e: the last k with bounds[k] <= e
-    int lo = 0, hi = nplans - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (bounds[mid] <= e)
-            lo = mid;
-        else
-            hi = mid - 1;
-    }
-    unsigned long long *ptiles = tiles + (int64_t)lo * ntiles;
-    const int el = (int)(e - bounds[lo]);
-    Walk w;
-    setup_walk<kBack>(p, e, w);
-    if (!FILL) {
-        float4 *r = rays + kRayF4 * e;
-        r[0] = pack_d2(w.ok ? w.deltaT[0] : 0.0, w.ok ? w.deltaT[1] : 0.0);
-        r[1] = pack_d2(w.ok ? w.deltaT[2] : 0.0, w.ok ? w.t_tof : 0.0);
-        const int steps = w.ok ? (w.step[0] + 1) | ((w.step[1] + 1) << 2) | ((w.step[2] + 1) << 4)
-                               : 0;
-        r[2] = make_float4(w.weight, __int_as_float(steps), 0.0f, 0.0f);
-    }
-    if (!w.ok)
-        return;
     int x = w.pos[0], y = w.pos[1], z = w.pos[2];
     double n0 = w.nextT[0], n1 = w.nextT[1], n2 = w.nextT[2];
     const double t0 = w.deltaT[0], t1 = w.deltaT[1], t2 = w.deltaT[2];
@@ -125,21 +95,6 @@ __global__ __launch_bounds__(256) void sid_plan_pass(SidParams p, SidTileGeom g,
     int sx = x, sy = y, sz = z, len = 0;
     double s0 = n0, s1 = n1, s2 = n2, slast = 0.0;
     bool any = false;
-    auto emit = [&]() {
-        const int tile = tile_id(g, sx, sy, sz);
-        if (!FILL) {
-            atomicAdd(ptiles + tile, 1ull);
-        } else {
-            const unsigned long long slot = atomicAdd(ptiles + tile, 1ull);
-            if (slot >= nruns)
-                return; // never: the fill pass finds the runs the count pass counted
-            float4 *o = runs + kRunF4 * (int64_t)slot;
-            o[0] = make_float4(__int_as_float(el), __int_as_float(sx | (sy << 16)),
-                               __int_as_float(sz | (len << 16)), 0.0f);
-            o[1] = pack_d2(s0, s1);
-            o[2] = pack_d2(s2, slast);
-        }
-    };
     while (stillin && cap-- > 0) {
         int i = (n0 <= n1) ? ((n0 <= n2) ? 0 : 2) : ((n1 <= n2) ? 1 : 2);
         double nt = i == 0 ? n0 : (i == 1 ? n1 : n2);
@@ -169,7 +124,7 @@ __global__ __launch_bounds__(256) void sid_plan_pass(SidParams p, SidTileGeom g,
         }
         if (cross) {
             if (any)
-                emit();
+                f(tile_id(g, sx, sy, sz), sx, sy, sz, len, s0, s1, s2, slast);
             sx = x;
             sy = y;
             sz = z;
@@ -182,21 +137,132 @@ __global__ __launch_bounds__(256) void sid_plan_pass(SidParams p, SidTileGeom g,
         }
     }
     if (len > 0 && any)
-        emit();
+        f(tile_id(g, sx, sy, sz), sx, sy, sz, len, s0, s1, s2, slast);
+}
+
+__device__ __forceinline__ void put_run(float4 *o, int el, int sx, int sy, int sz, int len,
+                                        double s0, double s1, double s2, double slast)
+{
+    o[0] = make_float4(__int_as_float(el), __int_as_float(sx | (sy << 16)),
+                       __int_as_float(sz | (len << 16)), 0.0f);
+    o[1] = pack_d2(s0, s1);
+    o[2] = pack_d2(s2, slast);
+}
+
+// The plan set's two passes over the rays of all its plans (plan k holds
+// rays bounds[k] .. bounds[k + 1] - 1), one lane per ray.  Count pass (FILL
+// false): writes the ray records and counts the listed runs of each
+// (plan, tile) counter.  Fill pass: writes each run at its counter's cursor
+// (which starts at the counter's offset), with the ray's index inside its
+// plan.  LDS: the workgroup first counts its runs in LDS counters, so that
+// the global counters take one atomic per (workgroup, counter) instead of
+// one per run; the fill pass then reserves one range per (workgroup,
+// counter) and walks its rays a second time to write into those ranges.
+// Needs 12 bytes of LDS per counter.
+template <bool FILL, bool TOF, bool LDS>
+__global__ __launch_bounds__(256) void sid_plan_pass(SidParams p, SidTileGeom g,
+                                                     unsigned long long *tiles, float4 *rays,
+                                                     float4 *runs, unsigned long long nruns,
+                                                     const int64_t *__restrict__ bounds,
+                                                     int nplans, int ntiles)
+{
+    extern __shared__ unsigned lcnt[];
+    const int ncnt = nplans * ntiles;
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = e < p.n && e >= bounds[0] && e < bounds[nplans];
+    // the plan of ray e: the last k with bounds[k] <= e
+    int lo = 0;
+    if (valid) {
+        int hi = nplans - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (bounds[mid] <= e)
+                lo = mid;
+            else
+                hi = mid - 1;
+        }
+    }
+    const int c0 = lo * ntiles;
+    const int el = valid ? (int)(e - bounds[lo]) : 0;
+    Walk w;
+    w.ok = false;
+    if (valid) {
+        setup_walk<kBack>(p, e, w);
+        if (!FILL) {
+            float4 *r = rays + kRayF4 * e;
+            r[0] = pack_d2(w.ok ? w.deltaT[0] : 0.0, w.ok ? w.deltaT[1] : 0.0);
+            r[1] = pack_d2(w.ok ? w.deltaT[2] : 0.0, w.ok ? w.t_tof : 0.0);
+            const int steps =
+                w.ok ? (w.step[0] + 1) | ((w.step[1] + 1) << 2) | ((w.step[2] + 1) << 4) : 0;
+            r[2] = make_float4(w.weight, __int_as_float(steps), 0.0f, 0.0f);
+        }
+    }
+    if constexpr (!LDS) {
+        if (!w.ok)
+            return;
+        for_each_run<TOF>(p, g, w,
+                          [&](int tile, int sx, int sy, int sz, int len, double s0, double s1,
+                              double s2, double slast) {
+                              const unsigned long long slot = atomicAdd(tiles + c0 + tile, 1ull);
+                              if (!FILL || slot >= nruns)
+                                  return; // (never past nruns: the passes find the same runs)
+                              put_run(runs + kRunF4 * (int64_t)slot, el, sx, sy, sz, len, s0,
+                                      s1, s2, slast);
+                          });
+    } else {
+        for (int i = threadIdx.x; i < ncnt; i += blockDim.x)
+            lcnt[i] = 0u;
+        __syncthreads();
+        auto count = [&](int tile, int, int, int, int, double, double, double, double) {
+            atomicAdd(lcnt + c0 + tile, 1u);
+        };
+        if (w.ok)
+            for_each_run<TOF>(p, g, w, count);
+        __syncthreads();
+        if (!FILL) {
+            for (int i = threadIdx.x; i < ncnt; i += blockDim.x)
+                if (lcnt[i])
+                    atomicAdd(tiles + i, (unsigned long long)lcnt[i]);
+            return;
+        }
+        auto *lbase = reinterpret_cast<unsigned long long *>(lcnt + ((ncnt + 1) & ~1));
+        for (int i = threadIdx.x; i < ncnt; i += blockDim.x) {
+            const unsigned c = lcnt[i];
+            lbase[i] = c ? atomicAdd(tiles + i, (unsigned long long)c) : 0ull;
+            lcnt[i] = 0u;
+        }
+        __syncthreads();
+        if (w.ok)
+            for_each_run<TOF>(p, g, w,
+                              [&](int tile, int sx, int sy, int sz, int len, double s0,
+                                  double s1, double s2, double slast) {
+                                  const unsigned long long slot =
+                                      lbase[c0 + tile] + atomicAdd(lcnt + c0 + tile, 1u);
+                                  if (slot >= nruns)
+                                      return;
+                                  put_run(runs + kRunF4 * (int64_t)slot, el, sx, sy, sz, len,
+                                          s0, s1, s2, slast);
+                              });
+    }
 }
 
 // One workgroup per chunk of one tile's runs: zero the FP64 LDS tile, walk
 // every run of the chunk into it (scale from vals: 1 / v for v > 0 as
-// BackRayTracing's callers, siddon.cu.cc:363-371; ADJ: g itself, runs of
-// g = 0 skipped), then add the tile's non-zero voxels to the image with one
-// atomic each, consecutive lanes on consecutive voxels of the image's
-// fastest axis.
+// BackRayTracing's callers, siddon.cu.cc:363-371, or c / v with per-LOR
+// counts c; ADJ: g itself, runs of g = 0 skipped), then add the tile's
+// non-zero voxels to the image with one atomic each, consecutive lanes on
+// consecutive voxels of the image's fastest axis.  vals (vperm) and counts
+// (cperm) are read in the plan's LOR order or, through a permutation, in the
+// caller's.
 template <bool TOF, bool ADJ>
 __global__ __launch_bounds__(256) void sid_tile_scatter(SidParams p, SidTileGeom g,
                                                         const SidChunk *__restrict__ chunks,
                                                         const float4 *__restrict__ runs,
                                                         const float4 *__restrict__ rays,
-                                                        const float *__restrict__ vals)
+                                                        const float *__restrict__ vals,
+                                                        const int *__restrict__ vperm,
+                                                        const float *__restrict__ counts,
+                                                        const int *__restrict__ cperm)
 {
     extern __shared__ double acc[];
     const SidChunk ch = chunks[blockIdx.x];
@@ -211,7 +277,7 @@ __global__ __launch_bounds__(256) void sid_tile_scatter(SidParams p, SidTileGeom
         const float4 *L = runs + kRunF4 * (ch.start + j);
         const float4 a = L[0], b = L[1], c = L[2];
         const int e = __float_as_int(a.x);
-        const float v = vals[e];
+        const float v = vals[vperm ? vperm[e] : e];
         float scale;
         if (ADJ) {
             if (v == 0.0f)
@@ -220,7 +286,7 @@ __global__ __launch_bounds__(256) void sid_tile_scatter(SidParams p, SidTileGeom
         } else {
             if (!(v > 0.0f))
                 continue;
-            scale = 1.0f / v;
+            scale = counts ? counts[cperm ? cperm[e] : e] / v : 1.0f / v;
         }
         const float4 *R = rays + kRayF4 * (int64_t)e;
         double t0, t1, t2, t_tof;
@@ -276,6 +342,85 @@ __global__ __launch_bounds__(256) void sid_tile_scatter(SidParams p, SidTileGeom
     }
 }
 
+// Sort key of a LOR for the sorted LOR stream: transaxial direction (8
+// bits), tilt (6 bits), signed transaxial distance from the image axis (8
+// bits), axial midpoint (10 bits), so that consecutive LORs run nearly
+// parallel through neighbouring voxels.  The plan index rides above them, so
+// each plan's LORs stay in its own range.  Any order gives the same results
+// up to summation order; the key only buys locality.
+__global__ __launch_bounds__(256) void sid_sort_key(SidParams p, const int64_t *__restrict__ bounds,
+                                                    int nplans, unsigned long long *keys,
+                                                    int *idx)
+{
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t e = bounds[0] + j;
+    if (e >= bounds[nplans])
+        return;
+    int lo = 0, hi = nplans - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (bounds[mid] <= e)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    const float *L = p.lors;
+    const int64_t ld = p.ld;
+    const float x1 = L[e], y1 = L[ld + e], z1 = L[2 * ld + e];
+    float dx = L[3 * ld + e] - x1, dy = L[4 * ld + e] - y1, dz = L[5 * ld + e] - z1;
+    if (dx < 0.0f || (dx == 0.0f && dy < 0.0f)) {
+        dx = -dx;
+        dy = -dy;
+        dz = -dz;
+    }
+    const float lxy = sqrtf(dx * dx + dy * dy);
+    const float pi = 3.14159265f;
+    const float fphi = lxy > 0.0f ? atan2f(dy, dx) / pi + 0.5f : 0.5f;
+    const float fth = lxy > 0.0f ? atanf(dz / lxy) / pi + 0.5f : 1.0f;
+    const float rad = 0.5f * sqrtf(p.size[0] * p.size[0] + p.size[1] * p.size[1]);
+    const float s = lxy > 0.0f ? ((x1 - p.center[0]) * dy - (y1 - p.center[1]) * dx) / lxy : 0.0f;
+    const float fs = s / (2.0f * rad) + 0.5f;
+    const float fz = (0.5f * (z1 + L[5 * ld + e]) - p.center[2]) / p.size[2] + 0.5f;
+    auto q = [](float f, int bits) {
+        const float m = (float)(1 << bits);
+        return (unsigned)fminf(fmaxf(f * m, 0.0f), m - 1.0f);
+    };
+    const unsigned key = (q(fphi, 8) << 24) | (q(fth, 6) << 18) | (q(fs, 8) << 10) | q(fz, 10);
+    keys[j] = ((unsigned long long)lo << 32) | key;
+    idx[j] = (int)j;
+}
+
+// The sorted LOR copy (7 columns, leading dimension n, LOR bounds[0] + j in
+// sorted position j) and, for each sorted LOR, its index inside its plan in
+// the caller's order.
+__global__ __launch_bounds__(256) void sid_sort_gather(SidParams p, const int64_t *__restrict__ bounds,
+                                                       int nplans,
+                                                       const unsigned long long *__restrict__ keys,
+                                                       const int *__restrict__ idx, float *slors,
+                                                       int *perm)
+{
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t e = bounds[0] + j;
+    if (e >= bounds[nplans])
+        return;
+    const int64_t src = bounds[0] + idx[j];
+    const int k = (int)(keys[j] >> 32);
+#pragma unroll
+    for (int c = 0; c < 7; c++)
+        slors[c * p.n + e] = p.lors[c * p.ld + src];
+    perm[e] = (int)(src - bounds[k]);
+}
+
+// out[perm[e]] = in[e]: a plan's buffer back in the caller's LOR order.
+__global__ __launch_bounds__(256) void sid_unsort(const float *__restrict__ in,
+                                                  const int *__restrict__ perm, int64_t n,
+                                                  float *out)
+{
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < n)
+        out[perm[e]] = in[e];
+}
+
 } // namespace
 } // namespace srfx
 
@@ -284,7 +429,7 @@ using namespace srfx;
 struct srfx_siddon_plan_s {
     // device blocks shared by the plans of one set, freed with the last plan
     struct Arena {
-        void *mem[2];
+        void *mem[3];
         std::atomic<int> refs;
     };
     SidParams p{};
@@ -296,6 +441,7 @@ struct srfx_siddon_plan_s {
     float4 *runs = nullptr;
     SidChunk *chunks = nullptr;
     float *proj = nullptr;
+    int *perm = nullptr; // sorted plans: plan order -> caller order
     Arena *arena = nullptr;
 };
 
@@ -304,8 +450,8 @@ namespace {
 void plan_release(srfx_siddon_plan_s *pl)
 {
     if (pl->arena && pl->arena->refs.fetch_sub(1) == 1) {
-        (void)hipFree(pl->arena->mem[0]);
-        (void)hipFree(pl->arena->mem[1]);
+        for (void *m : pl->arena->mem)
+            (void)hipFree(m);
         delete pl->arena;
     }
     delete pl;
@@ -335,22 +481,42 @@ int sid_setup(SidParams &p, const float *lors, int64_t n, int64_t ld, const srfx
     return SRFX_OK;
 }
 
+// LDS bytes per (plan, tile) counter of the LDS plan passes, and the most
+// counters they take (64 KB of LDS).
+constexpr int kLdsCounterBytes = 12;
+constexpr int kMaxLdsCounters = 65536 / kLdsCounterBytes - 1;
+
 template <bool FILL>
 void launch_pass(const SidParams &p, const SidTileGeom &g, unsigned long long *tiles,
                  float4 *rays, float4 *runs, int64_t nruns, const int64_t *bounds, int nplans,
                  int ntiles, hipStream_t st)
 {
     const dim3 grid(blocks_for(p.n, 256)), block(256);
-    if (p.use_tof)
-        hipLaunchKernelGGL((sid_plan_pass<FILL, true>), grid, block, 0, st, p, g, tiles, rays,
-                           runs, (unsigned long long)nruns, bounds, nplans, ntiles);
-    else
-        hipLaunchKernelGGL((sid_plan_pass<FILL, false>), grid, block, 0, st, p, g, tiles, rays,
-                           runs, (unsigned long long)nruns, bounds, nplans, ntiles);
+    const int ncnt = nplans * ntiles;
+    const unsigned long long nr = (unsigned long long)nruns;
+    if (ncnt <= kMaxLdsCounters) {
+        const unsigned lds = (unsigned)(4 * ((ncnt + 1) & ~1) + 8 * ncnt);
+        if (p.use_tof)
+            hipLaunchKernelGGL((sid_plan_pass<FILL, true, true>), grid, block, lds, st, p, g,
+                               tiles, rays, runs, nr, bounds, nplans, ntiles);
+        else
+            hipLaunchKernelGGL((sid_plan_pass<FILL, false, true>), grid, block, lds, st, p, g,
+                               tiles, rays, runs, nr, bounds, nplans, ntiles);
+    } else {
+        if (p.use_tof)
+            hipLaunchKernelGGL((sid_plan_pass<FILL, true, false>), grid, block, 0, st, p, g,
+                               tiles, rays, runs, nr, bounds, nplans, ntiles);
+        else
+            hipLaunchKernelGGL((sid_plan_pass<FILL, false, false>), grid, block, 0, st, p, g,
+                               tiles, rays, runs, nr, bounds, nplans, ntiles);
+    }
 }
 
-int plan_scatter(const srfx_siddon_plan_s *pl, bool adj, const float *vals, float *image,
-                 uint32_t flags, hipStream_t st)
+// vals in the caller's LOR order (vperm: the plan's sorted order to it) or,
+// with vperm null, in the plan's order; counts (optional) likewise (cperm).
+int plan_scatter(const srfx_siddon_plan_s *pl, bool adj, const float *vals, const int *vperm,
+                 const float *counts, const int *cperm, float *image, uint32_t flags,
+                 hipStream_t st)
 {
     int rc = zero_grid(image, pl->grid, flags, st);
     if (rc || pl->nchunks == 0)
@@ -362,17 +528,17 @@ int plan_scatter(const srfx_siddon_plan_s *pl, bool adj, const float *vals, floa
     if (p.use_tof) {
         if (adj)
             hipLaunchKernelGGL((sid_tile_scatter<true, true>), grid, block, lds, st, p, pl->g,
-                               pl->chunks, pl->runs, pl->rays, vals);
+                               pl->chunks, pl->runs, pl->rays, vals, vperm, counts, cperm);
         else
             hipLaunchKernelGGL((sid_tile_scatter<true, false>), grid, block, lds, st, p, pl->g,
-                               pl->chunks, pl->runs, pl->rays, vals);
+                               pl->chunks, pl->runs, pl->rays, vals, vperm, counts, cperm);
     } else {
         if (adj)
             hipLaunchKernelGGL((sid_tile_scatter<false, true>), grid, block, lds, st, p, pl->g,
-                               pl->chunks, pl->runs, pl->rays, vals);
+                               pl->chunks, pl->runs, pl->rays, vals, vperm, counts, cperm);
         else
             hipLaunchKernelGGL((sid_tile_scatter<false, false>), grid, block, lds, st, p, pl->g,
-                               pl->chunks, pl->runs, pl->rays, vals);
+                               pl->chunks, pl->runs, pl->rays, vals, vperm, counts, cperm);
     }
     return launch_status();
 }
@@ -431,13 +597,14 @@ int build_set(const float *lors, int64_t n, int64_t ld, const srfx_grid &grid, f
     const size_t o_cnt = o_proj + align256(sizeof(float) * nr);
     const size_t o_bnd = o_cnt + align256(sizeof(unsigned long long) * ncnt);
     const size_t b0 = o_bnd + align256(sizeof(int64_t) * (K + 1));
-    char *blk0 = nullptr, *blk1 = nullptr;
+    char *blk0 = nullptr, *blk1 = nullptr, *blk2 = nullptr;
     if (hipMalloc(&blk0, b0) != hipSuccess)
         return SRFX_ENOMEM;
     auto fail = [&](int code) {
         (void)hipStreamSynchronize(st);
         (void)hipFree(blk0);
         (void)hipFree(blk1);
+        (void)hipFree(blk2);
         return code;
     };
     float4 *rays = (float4 *)blk0;
@@ -446,11 +613,49 @@ int build_set(const float *lors, int64_t n, int64_t ld, const srfx_grid &grid, f
     auto *dbounds = (int64_t *)(blk0 + o_bnd);
     std::vector<unsigned long long> hc(ncnt, 0ull);
     const bool any = bounds[K] > bounds[0];
-    if (any) {
-        if (hipMemsetAsync(cnt, 0, sizeof(unsigned long long) * ncnt, st) != hipSuccess ||
-            hipMemcpyAsync(dbounds, bounds, sizeof(int64_t) * (K + 1), hipMemcpyHostToDevice,
-                           st) != hipSuccess)
+    const int64_t m = bounds[K] - bounds[0];
+    int *perm = nullptr;
+    if (any && (hipMemsetAsync(cnt, 0, sizeof(unsigned long long) * ncnt, st) != hipSuccess ||
+                hipMemcpyAsync(dbounds, bounds, sizeof(int64_t) * (K + 1),
+                               hipMemcpyHostToDevice, st) != hipSuccess))
+        return fail(SRFX_ELAUNCH);
+    if (any && tile.sort) {
+        // the sorted LOR stream: keys, a radix sort of (key, index), then the
+        // sorted copy of the LORs that every pass and projection reads
+        if (m > (int64_t)INT32_MAX - 1)
+            return fail(SRFX_EINVAL);
+        size_t tmp_bytes = 0;
+        const int key_bits = 32 + std::max(1, 32 - __builtin_clz((unsigned)K));
+        if (hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, (unsigned long long *)nullptr,
+                                               (unsigned long long *)nullptr, (int *)nullptr,
+                                               (int *)nullptr, (int)m, 0, key_bits, st) !=
+            hipSuccess)
             return fail(SRFX_ELAUNCH);
+        const size_t o_perm = align256(sizeof(float) * 7 * nr);
+        const size_t o_k0 = o_perm + align256(sizeof(int) * nr);
+        const size_t o_k1 = o_k0 + align256(sizeof(unsigned long long) * m);
+        const size_t o_i0 = o_k1 + align256(sizeof(unsigned long long) * m);
+        const size_t o_i1 = o_i0 + align256(sizeof(int) * m);
+        const size_t o_tmp = o_i1 + align256(sizeof(int) * m);
+        if (hipMalloc(&blk2, o_tmp + tmp_bytes) != hipSuccess)
+            return fail(SRFX_ENOMEM);
+        auto *slors = (float *)blk2;
+        perm = (int *)(blk2 + o_perm);
+        auto *k0 = (unsigned long long *)(blk2 + o_k0), *k1 = (unsigned long long *)(blk2 + o_k1);
+        auto *i0 = (int *)(blk2 + o_i0), *i1 = (int *)(blk2 + o_i1);
+        const dim3 sg(blocks_for(m, 256)), sb(256);
+        hipLaunchKernelGGL(sid_sort_key, sg, sb, 0, st, p, dbounds, K, k0, i0);
+        if (launch_status() ||
+            hipcub::DeviceRadixSort::SortPairs(blk2 + o_tmp, tmp_bytes, k0, k1, i0, i1, (int)m, 0,
+                                               key_bits, st) != hipSuccess)
+            return fail(SRFX_ELAUNCH);
+        hipLaunchKernelGGL(sid_sort_gather, sg, sb, 0, st, p, dbounds, K, k1, i1, slors, perm);
+        if (launch_status())
+            return fail(SRFX_ELAUNCH);
+        p.lors = slors;
+        p.ld = n;
+    }
+    if (any) {
         launch_pass<false>(p, g, cnt, rays, nullptr, 0, dbounds, K, ntiles, st);
         if (launch_status() ||
             hipMemcpyAsync(hc.data(), cnt, sizeof(unsigned long long) * ncnt,
@@ -500,11 +705,13 @@ int build_set(const float *lors, int64_t n, int64_t ld, const srfx_grid &grid, f
     auto *arena = new srfx_siddon_plan_s::Arena;
     arena->mem[0] = blk0;
     arena->mem[1] = blk1;
+    arena->mem[2] = blk2;
     arena->refs.store(K);
     for (int k = 0; k < K; k++) {
         auto *pl = new srfx_siddon_plan_s;
         pl->p = p;
-        pl->p.lors = n > 0 ? lors + bounds[k] : lors;
+        pl->p.lors = n > 0 ? p.lors + bounds[k] : p.lors;
+        pl->perm = perm ? perm + bounds[k] : nullptr;
         pl->p.n = bounds[k + 1] - bounds[k];
         pl->grid = grid;
         pl->g = g;
@@ -571,7 +778,8 @@ extern "C" int srfx_siddon_plan_backproject(srfx_siddon_plan plan, const float *
 {
     if (!plan || !image || (plan->p.n > 0 && !lors_value))
         return SRFX_EINVAL;
-    return plan_scatter(plan, false, lors_value, image, flags, (hipStream_t)stream);
+    return plan_scatter(plan, false, lors_value, plan->perm, nullptr, nullptr, image, flags,
+                        (hipStream_t)stream);
 }
 
 extern "C" int srfx_siddon_plan_backproject_adjoint(srfx_siddon_plan plan, const float *g,
@@ -579,32 +787,47 @@ extern "C" int srfx_siddon_plan_backproject_adjoint(srfx_siddon_plan plan, const
 {
     if (!plan || !image || (plan->p.n > 0 && !g))
         return SRFX_EINVAL;
-    return plan_scatter(plan, true, g, image, flags, (hipStream_t)stream);
+    return plan_scatter(plan, true, g, plan->perm, nullptr, nullptr, image, flags,
+                        (hipStream_t)stream);
 }
 
 extern "C" int srfx_siddon_plan_osem_subset(srfx_siddon_plan plan, const float *image,
-                                            float *bp, uint32_t flags, void *stream)
+                                            const float *counts, float *bp, uint32_t flags,
+                                            void *stream)
 {
-    if (!plan || !bp || (plan->p.n > 0 && !image))
+    const bool project = !(flags & SRFX_OSEM_SCATTER_ONLY);
+    const bool scatter = !(flags & SRFX_OSEM_PROJECT_ONLY);
+    if (!plan || (scatter && !bp) || (project && plan->p.n > 0 && !image) ||
+        (!project && !scatter))
         return SRFX_EINVAL;
     hipStream_t st = (hipStream_t)stream;
     const SidParams &p = plan->p;
-    if (p.n > 0) {
+    if (project && p.n > 0) {
         const int rc = srfx_siddon_project(p.lors, p.n, p.ld, image, plan->grid, p.tof_bin,
                                            p.tof_sigma2, plan->proj, 0, stream);
         if (rc)
             return rc;
     }
-    return plan_scatter(plan, false, plan->proj, bp, flags, st);
+    if (!scatter)
+        return SRFX_OK;
+    return plan_scatter(plan, false, plan->proj, nullptr, counts, plan->perm, bp,
+                        flags & SRFX_ACCUMULATE, st);
 }
 
 extern "C" int srfx_siddon_plan_projection(srfx_siddon_plan plan, float *out, void *stream)
 {
     if (!plan || (plan->p.n > 0 && !out))
         return SRFX_EINVAL;
-    if (plan->p.n > 0 &&
-        hipMemcpyAsync(out, plan->proj, sizeof(float) * plan->p.n, hipMemcpyDeviceToDevice,
-                       (hipStream_t)stream) != hipSuccess)
+    if (plan->p.n == 0)
+        return SRFX_OK;
+    hipStream_t st = (hipStream_t)stream;
+    if (plan->perm) {
+        hipLaunchKernelGGL(sid_unsort, dim3(blocks_for(plan->p.n, 256)), dim3(256), 0, st,
+                           (const float *)plan->proj, (const int *)plan->perm, plan->p.n, out);
+        return launch_status();
+    }
+    if (hipMemcpyAsync(out, plan->proj, sizeof(float) * plan->p.n, hipMemcpyDeviceToDevice,
+                       st) != hipSuccess)
         return SRFX_ELAUNCH;
     return SRFX_OK;
 }

@@ -211,10 +211,12 @@ class SiddonPlan:
     """
 
     def __init__(self, lors, grid, center, size, tof_bin=1.0e4, tof_sigma2=1.0e4,
-                 strides=None, tile=DEFAULT_SIDDON_TILE, chunk=DEFAULT_SIDDON_CHUNK, block=256):
+                 strides=None, tile=DEFAULT_SIDDON_TILE, chunk=DEFAULT_SIDDON_CHUNK, block=256,
+                 sort=True):
         self.lors, self.n, ld = _soa(lors, 7)
         self._layout(grid, center, size, strides, tile)
-        t = _lib.SrfxSiddonTile(*self.tile, int(chunk), int(block))
+        self.sorted = bool(sort)
+        t = _lib.SrfxSiddonTile(*self.tile, int(chunk), int(block), int(self.sorted))
         h = ctypes.c_void_p()
         _lib.call("srfx_siddon_plan_create", _ptr(self.lors), self.n, ld, self._cgrid,
                   float(tof_bin), float(tof_sigma2), t, ctypes.byref(h), _stream())
@@ -222,7 +224,8 @@ class SiddonPlan:
 
     @classmethod
     def subsets(cls, lors, bounds, grid, center, size, tof_bin=1.0e4, tof_sigma2=1.0e4,
-                strides=None, tile=DEFAULT_SIDDON_TILE, chunk=DEFAULT_SIDDON_CHUNK, block=256):
+                strides=None, tile=DEFAULT_SIDDON_TILE, chunk=DEFAULT_SIDDON_CHUNK, block=256,
+                sort=True):
         """Plans of the LOR ranges bounds[k]:bounds[k + 1] of one LOR tensor
         (the OSEM subsets), built together by srfx_siddon_plan_create_subsets.
         Returns one SiddonPlan per range."""
@@ -234,7 +237,8 @@ class SiddonPlan:
             raise ValueError("bounds must be non-decreasing LOR offsets within the LORs")
         proto = cls.__new__(cls)
         proto._layout(grid, center, size, strides, tile)
-        t = _lib.SrfxSiddonTile(*proto.tile, int(chunk), int(block))
+        proto.sorted = bool(sort)
+        t = _lib.SrfxSiddonTile(*proto.tile, int(chunk), int(block), int(proto.sorted))
         hs = (ctypes.c_void_p * K)()
         bs = (ctypes.c_int64 * (K + 1))(*bounds)
         _lib.call("srfx_siddon_plan_create_subsets", _ptr(full), n, ld, proto._cgrid,
@@ -243,7 +247,7 @@ class SiddonPlan:
         for k in range(K):
             pl = cls.__new__(cls)
             pl.__dict__.update({a: getattr(proto, a) for a in
-                                ("strides", "grid", "_cgrid", "_dense", "tile")})
+                                ("strides", "grid", "_cgrid", "_dense", "tile", "sorted")})
             pl.lors = full[:, bounds[k]:bounds[k + 1]]
             pl.n = bounds[k + 1] - bounds[k]
             pl._h = ctypes.c_void_p(hs[k])
@@ -273,12 +277,15 @@ class SiddonPlan:
 
     def device_bytes(self):
         """Device memory held by the plan: 48 bytes per run, 52 per LOR (ray
-        record, projection buffer), 16 per chunk, 8 per tile."""
+        record, projection buffer), 16 per chunk, 8 per tile, and with the
+        sorted LOR stream 32 more per LOR (sorted copy, permutation) and the
+        build's sort buffers (24 per LOR)."""
         nr, nc = self.info()
         nt = 1
         for g, t in zip(self.grid, self.tile):
             nt *= -(-g // min(t, g))
-        return 48 * nr + 52 * self.n + 16 * nc + 8 * nt
+        return (48 * nr + 52 * self.n + 16 * nc + 8 * nt
+                + (56 * self.n if self.sorted else 0))
 
     def _image(self, image, name):
         """image: a (nx, ny, nz) tensor with the plan's strides, or, for a
@@ -312,12 +319,26 @@ class SiddonPlan:
                   _lib.SRFX_ACCUMULATE if accumulate else 0, _stream())
         return out
 
-    def osem_subset(self, image, out=None, accumulate=False):
-        """Backprojection of 1/p (p > 0) with p the projection of image."""
-        image = self._image(image, "image")
-        out = self._out(out, image)
-        _lib.call("srfx_siddon_plan_osem_subset", self._h, _ptr(image), _ptr(out),
-                  _lib.SRFX_ACCUMULATE if accumulate else 0, _stream())
+    def osem_subset(self, image, out=None, accumulate=False, counts=None, stage=None):
+        """Backprojection of 1/p (p > 0), or of c/p with per-LOR counts c
+        (float32 [n], LOR order), with p the projection of image.  stage
+        "project" / "scatter" runs only that half (the projection stays in the
+        plan in between)."""
+        flags = ((_lib.SRFX_ACCUMULATE if accumulate else 0)
+                 | {None: 0, "project": _lib.SRFX_OSEM_PROJECT_ONLY,
+                    "scatter": _lib.SRFX_OSEM_SCATTER_ONLY}[stage])
+        c = None if counts is None else _vec(counts, self.n, "counts")
+        if stage == "project":
+            image = self._image(image, "image")
+            _lib.call("srfx_siddon_plan_osem_subset", self._h, _ptr(image), None, None, flags,
+                      _stream())
+            return None
+        if stage != "scatter":
+            image = self._image(image, "image")
+        out = self._out(out, c if c is not None else self.lors)
+        _lib.call("srfx_siddon_plan_osem_subset", self._h,
+                  None if stage == "scatter" else _ptr(image),
+                  None if c is None else _ptr(c), _ptr(out), flags, _stream())
         return out
 
     def projection(self):
