@@ -209,6 +209,11 @@ def siddon_c1(args):
         kern[name] = {"kernel": kname, "mean_launch_ms": round(kms, 4),
                       "algorithmic_bytes_per_launch": byts, "achieved_GBps": round(ach, 1),
                       "frac": round(ach / 8000.0, 4)}
+    from bench import _measured_traffic
+    traffic = _measured_traffic("siddon_c1")
+    for name in kern:
+        if traffic.get(name):
+            kern[name]["traffic_bytes_per_launch"] = traffic[name]["bytes_per_launch"]
     dom = max(kern, key=lambda k: kern[k]["mean_launch_ms"])
     for pl in plans:
         pl.close()
@@ -255,7 +260,9 @@ def siddon_c1(args):
         plan_build_cold_ms=round(plan_times[0] * 1e3, 2),
         plan_build_amortised_over_iterations=C1_ITERATIONS,
         roofline={"bound": "hbm", "achieved": kern[dom]["achieved_GBps"], "peak": 8000.0,
-                  "unit": "GB/s", "frac": kern[dom]["frac"], "traffic": None,
+                  "unit": "GB/s", "frac": kern[dom]["frac"],
+                  "traffic": kern[dom].get("traffic_bytes_per_launch"),
+                  "traffic_source": (traffic[dom]["source"] if traffic.get(dom) else None),
                   "kernel": kern[dom]["kernel"], "mean_launch_ms": kern[dom]["mean_launch_ms"],
                   "algorithmic_bytes_per_launch": byts,
                   "bytes_formula": "sum_i (28 + 4 + 4*nnz_i) + 4*N_vox"},
