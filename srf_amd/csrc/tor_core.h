@@ -908,6 +908,78 @@ __device__ __forceinline__ float slice_gather_frame(const TorParams &p, const Ev
     return sum([&](int k) { return ((in >> k) & 1u) != 0; });
 }
 
+// The gather of a box whose patch lies inside the image (most slices of most
+// events): no image clipping, the patch's own edge clipped by an infinite
+// column or row term (dt = +inf fails the dt test), and the slice's
+// amplitude multiplying the slice's sum once instead of every candidate's
+// weight.  Returns false, having decided nothing, when the patch is not
+// inside, the box starts before the patch, or a candidate lies within
+// kThrMargin of the cut-off; the caller then runs the clipped gather, which
+// decides those candidates exactly.  The voxels are those of the clipped
+// gather: with the patch inside, it clips the candidates past the patch, as
+// here, and those past the ellipse's span (ib0 + nx and beyond, jb0 + ny and
+// beyond), which lie outside the padded ellipse (dt > kThrMargin) and fail
+// the dt test here.
+template <int BX, int BY>
+__device__ __forceinline__ bool slice_gather_inner(const TorParams &p, const Event &ev,
+                                                   const Track &t, const Cross &c, float amp,
+                                                   int s, float &out)
+{
+    constexpr int NC = BX * BY;
+    const float eps = 1.0e-4f;
+    const int ib0 = (int)ceilf((c.dx0 - t.hx) * p.rfinter_x - eps);
+    const int jb0 = (int)ceilf((c.dy0 - t.dymax) * p.rfinter_y - eps);
+    const int P = p.patch;
+    const bool inner = (ib0 >= 0) & (jb0 >= 0) & (c.ix0 >= 0) & (c.ix0 <= p.gx - P) &
+                       (c.iy0 >= 0) & (c.iy0 <= p.gy - P);
+    if (!inner)
+        return false;
+    float cx2[BX], cux[BX], rt2[BY], ruy[BY];
+#pragma unroll
+    for (int k = 0; k < BX; k++) {
+        col_terms(voxel_offset(c.dx0, ib0 + k, p.finter_x), ev.ux, cx2[k], cux[k]);
+        cx2[k] = ib0 + k < P ? cx2[k] : INFINITY;
+    }
+#pragma unroll
+    for (int k = 0; k < BY; k++) {
+        row_terms(voxel_offset(c.dy0, jb0 + k, p.finter_y), ev.uy, ev.thr, rt2[k], ruy[k]);
+        rt2[k] = jb0 + k < P ? rt2[k] : INFINITY;
+    }
+    const __amdgpu_buffer_rsrc_t rs =
+        __builtin_amdgcn_make_buffer_rsrc((void *)p.img, (short)0, p.img_bytes, 0x00020000);
+    const unsigned o0 = 4u * (unsigned)(s * p.sz + (c.iy0 + jb0) * p.sy + c.ix0 + ib0);
+    float v[NC];
+#pragma unroll
+    for (int jj = 0; jj < BY; jj++) {
+        const unsigned orow = o0 + 4u * (unsigned)(jj * p.sy);
+#pragma unroll
+        for (int ii = 0; ii < BX; ii++)
+            v[jj * BX + ii] =
+                __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs, orow + 4u * ii, 0, 0));
+    }
+    float dt[NC];
+    float gap = INFINITY;
+#pragma unroll
+    for (int jj = 0; jj < BY; jj++) {
+#pragma unroll
+        for (int ii = 0; ii < BX; ii++) {
+            const int k = jj * BX + ii;
+            dt[k] = tube_dt(cx2[ii], cux[ii], rt2[jj], ruy[jj]);
+            gap = fminf(gap, fabsf(dt[k]));
+        }
+    }
+    if (!(gap > kThrMargin))
+        return false;
+    const float c0 = ev.thr * ev.cw;
+    float a = 0.0f;
+#pragma unroll
+    for (int k = 0; k < NC; k++)
+        a = fmaf(__builtin_amdgcn_exp2f(fmaf(dt[k], ev.cw, c0)),
+                 dt[k] < -kThrMargin ? v[k] : 0.0f, a);
+    out = amp * a;
+    return true;
+}
+
 // Box shape of an event for a compiled box size NB: 1 = NB x (NB - 1)
 // (enough for every event whose ellipse spans at most NB - 1 rows, e.g. all
 // C2 events), 2 = NB x NB, 0 = larger (chord walk).
@@ -926,6 +998,12 @@ __device__ __forceinline__ float slice_box_image(const TorParams &p, const Event
                                                  const Track &t, const Cross &c, float amp, int s,
                                                  float scale, int shape)
 {
+    if constexpr (KIND == 0 && FRAME) {
+        float r;
+        if (shape == 1 ? slice_gather_inner<NB, NB - 1>(p, ev, t, c, amp, s, r)
+                       : slice_gather_inner<NB, NB>(p, ev, t, c, amp, s, r))
+            return r;
+    }
     const int P = p.patch;
     if (c.ix0 <= -P || c.ix0 >= p.gx || c.iy0 <= -P || c.iy0 >= p.gy)
         return 0.0f;
