@@ -262,7 +262,9 @@ struct srfx_tor_batch {
     srfx::VoxRange *d_vox = nullptr;
     long long total_vox = 0;
     unsigned *d_counters = nullptr;
-    std::vector<float *> frames; // op-frame copies, one per (reconstruction, non-empty group)
+    // the block holding the op-frame copies, one per (reconstruction,
+    // non-empty group)
+    std::vector<float *> frames;
     srfx::FrameJob *d_frames = nullptr;
     int nframes = 0;
     long long frame_blocks = 0;
@@ -376,8 +378,48 @@ extern "C" int srfx_tor_batch_create(const srfx_recon *recons, int32_t nrecon,
     std::vector<FrameJob> frames;
     long long off = 0;
     int rc_status = SRFX_OK;
+    // one block for the op-frame copies of every (reconstruction, non-empty
+    // group), carved in the loop below
+    size_t frame_floats = 0;
+    for (int r = 0; r < nrecon; r++)
+        for (int a = 0; a < 3; a++)
+            if (recons[r].n[a] > 0)
+                frame_floats += (size_t)recons[r].shape[0] * recons[r].shape[1] *
+                                recons[r].shape[2];
+    float *frame_next = nullptr;
+    if (frame_floats > 0) {
+        if (hipMalloc((void **)&frame_next, sizeof(float) * frame_floats) != hipSuccess) {
+            delete b;
+            return SRFX_ENOMEM;
+        }
+        b->frames.push_back(frame_next);
+    }
     // Per subset: contiguous rows [k n / K, (k+1) n / K) of every group
-    // (graph/reconstruction/worker.py:79-85), each with its own plan.
+    // (graph/reconstruction/worker.py:79-85), each with its own plan; the
+    // plans of every (reconstruction, group) are built in one call.
+    std::vector<std::vector<int64_t>> all_bounds((size_t)nrecon * 3,
+                                                 std::vector<int64_t>(nb_subsets + 1));
+    std::vector<SubsetRequest> requests;
+    for (int r = 0; r < nrecon; r++)
+        for (int a = 0; a < 3; a++) {
+            const srfx_recon &rc = recons[r];
+            std::vector<int64_t> &bd = all_bounds[(size_t)r * 3 + a];
+            for (int k = 0; k <= nb_subsets; k++)
+                bd[k] = k * rc.n[a] / nb_subsets;
+            requests.push_back(SubsetRequest{rc.lors[a], rc.n[a], rc.ld[a], group_grid(rc, a),
+                                             nb_subsets, bd.data()});
+        }
+    std::vector<srfx_tor_plan> all_plans(requests.size() * nb_subsets, nullptr);
+    rc_status = plan_create_subsets_many(requests.data(), (int)requests.size(), 10, m,
+                                         kBatchTile, all_plans.data(), nullptr);
+    if (rc_status) {
+        free_plan(b);
+        return rc_status;
+    }
+    // the batch owns every plan from here (free_plan), the empty ones too:
+    // they hold no device memory of their own, and destroying them here would
+    // synchronise the device once each
+    b->plans = all_plans;
     for (int r = 0; r < nrecon && !rc_status; r++) {
         const srfx_recon &rc = recons[r];
         for (int a = 0; a < 3 && !rc_status; a++) {
@@ -387,11 +429,8 @@ extern "C" int srfx_tor_batch_create(const srfx_recon *recons, int32_t nrecon,
             const srfx_grid gg = group_grid(rc, a);
             if (n > 0) {
                 const size_t nv = (size_t)rc.shape[0] * rc.shape[1] * rc.shape[2];
-                if (hipMalloc((void **)&frame, sizeof(float) * nv) != hipSuccess) {
-                    rc_status = SRFX_ENOMEM;
-                    break;
-                }
-                b->frames.push_back(frame);
+                frame = frame_next;
+                frame_next += nv;
                 FrameJob f;
                 f.src = rc.image;
                 f.dst = frame;
@@ -407,24 +446,9 @@ extern "C" int srfx_tor_batch_create(const srfx_recon *recons, int32_t nrecon,
                 b->frame_blocks += (long long)f.tx * f.ty * f.gz;
                 frames.push_back(f);
             }
-            // the group's subsets, planned together
-            std::vector<int64_t> bounds(nb_subsets + 1);
-            for (int k = 0; k <= nb_subsets; k++)
-                bounds[k] = k * n / nb_subsets;
-            std::vector<srfx_tor_plan> sub(nb_subsets, nullptr);
-            rc_status = srfx_tor_plan_create_subsets(rc.lors[a], n, rc.ld[a], 10,
-                                                     group_grid(rc, a), m, kBatchTile,
-                                                     nb_subsets, bounds.data(), sub.data(),
-                                                     nullptr);
-            if (rc_status)
-                break;
-            // the batch owns every non-empty plan from here (free_plan)
-            for (int k = 0; k < nb_subsets; k++) {
-                if (bounds[k + 1] > bounds[k])
-                    b->plans.push_back(sub[k]);
-                else
-                    srfx_tor_plan_destroy(sub[k]);
-            }
+            // the group's subset plans
+            const std::vector<int64_t> &bounds = all_bounds[(size_t)r * 3 + a];
+            const srfx_tor_plan *sub = all_plans.data() + ((size_t)r * 3 + a) * nb_subsets;
             for (int k = 0; k < nb_subsets; k++) {
                 const int64_t lo = bounds[k], hi = bounds[k + 1];
                 srfx_tor_plan plan = sub[k];
@@ -447,10 +471,9 @@ extern "C" int srfx_tor_batch_create(const srfx_recon *recons, int32_t nrecon,
                 for (int64_t s = 0; s < hi - lo; s += kChunk)
                     ev[k].push_back(
                         EvChunk{j, (int)s, (int)std::min<int64_t>(kChunk, hi - lo - s), 0});
-                std::vector<TileChunk> ch(plan->nchunks);
-                if (!ch.empty() &&
-                    hipMemcpy(ch.data(), plan->chunks, sizeof(TileChunk) * ch.size(),
-                              hipMemcpyDeviceToHost) != hipSuccess) {
+                // the plan's chunk table, kept on the host by its build
+                const std::vector<TileChunk> &ch = plan->hchunks;
+                if ((int)ch.size() != plan->nchunks) {
                     rc_status = SRFX_ELAUNCH;
                     break;
                 }

@@ -2,6 +2,7 @@
 // (include/srfx.h, srfx_tor_plan_*).  Device code in tile_core.h.
 #include <rocprim/device/device_scan.hpp>
 
+#include <cstring>
 #include <vector>
 
 #include "tile_core.h"
@@ -212,6 +213,7 @@ int plan_build(srfx_tor_plan_s *pl, hipStream_t st, int chunk, bool keep_slots)
     }
     pl->nlist = total;
     pl->nchunks = (int)ch.size();
+    pl->hchunks = ch;
     if ((rc = dev_alloc(&pl->list, total)) || (rc = dev_alloc(&pl->chunks, ch.size())))
         goto done;
     if (keep_slots && ((rc = dev_alloc(&pl->ev_slot, total)) ||
@@ -282,186 +284,261 @@ bool set_buildable(const TileGeom &g, int64_t n, bool keep_slots)
     return !keep_slots && g.S <= kSpanMaxSlab && n < ((int64_t)1 << kSpanEventBits);
 }
 
-// Build plans pls[0..K) together; plan k holds events first[k] ..
-// first[k + 1] - 1 of the array at lors (row stride ld), and pls[k]->p is
-// already set up for its own events.  One count pass and one fill pass over
-// all events, two host synchronisations instead of two per plan, and two
-// device blocks shared by the plans (records and projection buffers; lists
-// and chunks).  Every plan's entries carry their spans; chunks are sorted.
-int plan_build_set(srfx_tor_plan_s *const *pls, int K, const int64_t *first, const float *lors,
-                   int chunk, hipStream_t st)
+// One set of plans built together: plan k of pls[0..K) holds events
+// first[k] .. first[k + 1] - 1 of the array at lors (row stride ld), and
+// pls[k]->p is already set up for its own events.
+struct PlanSet {
+    srfx_tor_plan_s *const *pls;
+    int K;
+    const int64_t *first; // K + 1
+    const float *lors;
+    int chunk;
+};
+
+// Build every set of `sets` in shared phases: the count passes of all sets,
+// one host synchronisation, the fill passes of all sets, one host
+// synchronisation, then the chunk sorts.  One count pass and one fill pass
+// per set cover all its plans; all plans of all sets share two device blocks
+// (records and projection buffers; chunks and lists), freed with the last
+// plan.  Every plan's entries carry their spans; chunks are sorted.
+int plan_build_sets(const std::vector<PlanSet> &sets, hipStream_t st)
 {
-    TileGeom g = pls[0]->g;
-    g.spans = 1;
-    const int nt = g.nslab * g.nwx * g.nwy;
+    const int S = (int)sets.size();
+    int nplans = 0;
+    for (const PlanSet &ps : sets)
+        nplans += ps.K;
+    if (nplans == 0)
+        return SRFX_OK;
     auto *arena = new srfx_tor_plan_s::Arena;
     arena->mem[0] = arena->mem[1] = nullptr;
-    arena->refs.store(K);
-    for (int k = 0; k < K; k++)
-        pls[k]->arena = arena;
-    // block 0: records and projection buffers
-    std::vector<size_t> rec_off(K), proj_off(K);
-    size_t b0 = 0;
-    for (int k = 0; k < K; k++) {
-        const size_t n = (size_t)std::max<int64_t>(first[k + 1] - first[k], 1);
-        rec_off[k] = b0;
-        b0 += align256(sizeof(float4) * kWalkF4 * n);
-        proj_off[k] = b0;
-        b0 += align256(sizeof(float) * n);
+    arena->refs.store(nplans);
+    for (const PlanSet &ps : sets)
+        for (int k = 0; k < ps.K; k++)
+            ps.pls[k]->arena = arena;
+    // per set: tile geometry, tile count, and where its tables go
+    struct Layout {
+        TileGeom g;
+        int nt;
+        size_t head;  // offset of first [K + 1], record pointers [K], list pointers [K]
+        size_t ctr;   // offset (ints) of cnt [K nt], fill [K nt], off [K nt]
+        std::vector<int64_t> totals;
+        std::vector<int> nch;
+        size_t ch0;   // first chunk of the set in the chunk table
+    };
+    std::vector<Layout> L(S);
+    // block 0: records and projection buffers of every plan
+    size_t b0 = 0, head_b = 0, ctr_n = 0;
+    std::vector<size_t> rec_off(nplans), proj_off(nplans);
+    for (int s = 0, j = 0; s < S; s++) {
+        const PlanSet &ps = sets[s];
+        Layout &l = L[s];
+        l.g = ps.pls[0]->g;
+        l.g.spans = 1;
+        l.nt = l.g.nslab * l.g.nwx * l.g.nwy;
+        for (int k = 0; k < ps.K; k++, j++) {
+            const size_t n = (size_t)std::max<int64_t>(ps.first[k + 1] - ps.first[k], 1);
+            rec_off[j] = b0;
+            b0 += align256(sizeof(float4) * kWalkF4 * n);
+            proj_off[j] = b0;
+            b0 += align256(sizeof(float) * n);
+        }
+        l.head = head_b;
+        head_b += align256(sizeof(int64_t) * (ps.K + 1) + 2 * sizeof(void *) * ps.K);
+        l.ctr = ctr_n;
+        ctr_n += (size_t)3 * ps.K * l.nt;
     }
-    // temporaries: cnt [K nt], fill [K nt], off [K nt], err; first [K + 1];
-    // record and list pointers [K] each
-    const size_t cnt_b = align256(sizeof(int) * ((size_t)3 * K * nt + 1));
-    const size_t first_b = align256(sizeof(int64_t) * (K + 1));
-    const size_t ptr_b = align256(sizeof(void *) * K);
-    char *tmp = nullptr;
-    std::vector<int> cnt((size_t)K * nt), off((size_t)K * nt);
-    std::vector<int64_t> totals(K);
+    // temporaries: the sets' heads, their counters, one error count per set
+    const size_t ctr_b = align256(sizeof(int) * std::max<size_t>(ctr_n, 1));
+    const size_t tmp_b = head_b + ctr_b + sizeof(int) * S;
+    char *tmp = nullptr, *blk0 = nullptr, *blk1 = nullptr;
+    std::vector<char> head(head_b);
+    std::vector<int> ctr(ctr_n);
+    std::vector<int> errs(S, 0);
     std::vector<TileChunk> ch;
-    std::vector<int> nch(K);
-    std::vector<void *> ptrs(K);
-    int mismatch = 0, rc = SRFX_OK;
-    int *cnt_d, *fill_d, *off_d, *err_d;
-    int64_t *first_d;
-    void **rec_d, **list_d;
-    PlanSetArgs a{};
-    const unsigned lds_cnt = 2u * sizeof(int) * nt, lds_fill = 6u * sizeof(int) * nt;
-    const bool lds = lds_fill <= 65536u;
-    const dim3 grid(blocks_for(first[K], 256)), block(256);
-    TorParams p = pls[0]->p;
-    p.lors = lors;
-    char *blk0 = nullptr, *blk1 = nullptr;
-    if ((rc = dev_alloc(&blk0, b0)) || (rc = dev_alloc(&tmp, cnt_b + first_b + 2 * ptr_b)))
+    int rc = SRFX_OK;
+    int *ctr_d = nullptr, *err_d = nullptr;
+    auto args = [&](int s) {
+        const PlanSet &ps = sets[s];
+        const Layout &l = L[s];
+        PlanSetArgs a{};
+        a.cols7 = ps.pls[0]->cols7;
+        a.ntiles = l.nt;
+        a.nplans = ps.K;
+        a.lds = 6u * sizeof(int) * l.nt <= 65536u;
+        a.first = (const int64_t *)(tmp + l.head);
+        a.rec = (float4 *const *)(tmp + l.head + sizeof(int64_t) * (ps.K + 1));
+        a.list = (int *const *)(tmp + l.head + sizeof(int64_t) * (ps.K + 1) +
+                                sizeof(void *) * ps.K);
+        a.cnt = ctr_d + l.ctr;
+        a.fill = a.cnt + (size_t)ps.K * l.nt;
+        a.off = a.fill + (size_t)ps.K * l.nt;
+        a.err = err_d + s;
+        return a;
+    };
+    auto launch = [&](bool fill) {
+        for (int s = 0; s < S; s++) {
+            const PlanSet &ps = sets[s];
+            const int64_t n = ps.first[ps.K];
+            if (n <= 0)
+                continue;
+            const PlanSetArgs a = args(s);
+            TorParams p = ps.pls[0]->p;
+            p.lors = ps.lors;
+            const unsigned lds = a.lds ? (fill ? 6u : 2u) * sizeof(int) * L[s].nt : 0u;
+            if (fill)
+                hipLaunchKernelGGL(plan_set_pass<true>, dim3(blocks_for(n, 256)), dim3(256), lds,
+                                   st, p, L[s].g, a);
+            else
+                hipLaunchKernelGGL(plan_set_pass<false>, dim3(blocks_for(n, 256)), dim3(256), lds,
+                                   st, p, L[s].g, a);
+        }
+        return launch_status();
+    };
+    if ((rc = dev_alloc(&blk0, b0)) || (rc = dev_alloc(&tmp, tmp_b)))
         goto done;
     arena->mem[0] = blk0;
-    cnt_d = (int *)tmp;
-    fill_d = cnt_d + (size_t)K * nt;
-    off_d = fill_d + (size_t)K * nt;
-    err_d = off_d + (size_t)K * nt;
-    first_d = (int64_t *)(tmp + cnt_b);
-    rec_d = (void **)(tmp + cnt_b + first_b);
-    list_d = (void **)(tmp + cnt_b + first_b + ptr_b);
-    for (int k = 0; k < K; k++) {
-        pls[k]->rec = (float4 *)(blk0 + rec_off[k]);
-        pls[k]->proj = (float *)(blk0 + proj_off[k]);
-        ptrs[k] = pls[k]->rec;
+    ctr_d = (int *)(tmp + head_b);
+    err_d = (int *)(tmp + head_b + ctr_b);
+    for (int s = 0, j = 0; s < S; s++) {
+        const PlanSet &ps = sets[s];
+        char *h = head.data() + L[s].head;
+        std::memcpy(h, ps.first, sizeof(int64_t) * (ps.K + 1));
+        auto **rp = reinterpret_cast<void **>(h + sizeof(int64_t) * (ps.K + 1));
+        for (int k = 0; k < ps.K; k++, j++) {
+            ps.pls[k]->rec = (float4 *)(blk0 + rec_off[j]);
+            ps.pls[k]->proj = (float *)(blk0 + proj_off[j]);
+            rp[k] = ps.pls[k]->rec;
+        }
     }
-    hipLaunchKernelGGL(fill_int, dim3(blocks_for(3 * (int64_t)K * nt + 1, 256)), dim3(256), 0, st,
-                       cnt_d, 3 * (int64_t)K * nt + 1, 0);
-    if (hipMemcpyAsync(first_d, first, sizeof(int64_t) * (K + 1), hipMemcpyHostToDevice, st) !=
-            hipSuccess ||
-        hipMemcpyAsync(rec_d, ptrs.data(), sizeof(void *) * K, hipMemcpyHostToDevice, st) !=
-            hipSuccess) {
+    {
+        // the counters and the error counts behind them
+        const int64_t nz = (int64_t)(ctr_b / sizeof(int) + S);
+        hipLaunchKernelGGL(fill_int, dim3(blocks_for(nz, 256)), dim3(256), 0, st, ctr_d, nz, 0);
+    }
+    if (hipMemcpyAsync(tmp, head.data(), head_b, hipMemcpyHostToDevice, st) != hipSuccess) {
         rc = SRFX_ELAUNCH;
         goto done;
     }
-    a.cols7 = pls[0]->cols7;
-    a.ntiles = nt;
-    a.nplans = K;
-    a.lds = lds;
-    a.first = first_d;
-    a.rec = (float4 *const *)rec_d;
-    a.cnt = cnt_d;
-    a.off = off_d;
-    a.fill = fill_d;
-    a.list = (int *const *)list_d;
-    a.err = err_d;
-    if (first[K] > 0)
-        hipLaunchKernelGGL(plan_set_pass<false>, grid, block, lds ? lds_cnt : 0u, st, p, g, a);
-    if ((rc = launch_status()))
+    if ((rc = launch(false)))
         goto done;
-    if (hipMemcpyAsync(cnt.data(), cnt_d, sizeof(int) * cnt.size(), hipMemcpyDeviceToHost, st) !=
-            hipSuccess ||
+    if ((ctr_n > 0 && hipMemcpyAsync(ctr.data(), ctr_d, sizeof(int) * ctr_n,
+                                     hipMemcpyDeviceToHost, st) != hipSuccess) ||
         hipStreamSynchronize(st) != hipSuccess) {
         rc = SRFX_ELAUNCH;
         goto done;
     }
-    // per plan: slot offsets per tile, then chunks window by window, each
-    // window's slabs in order (an event's listings in consecutive slabs mostly
-    // share a window, so its record is read again while the L2 holds it)
-    for (int k = 0; k < K; k++) {
-        const int *c = cnt.data() + (size_t)k * nt;
-        int *o = off.data() + (size_t)k * nt;
-        int64_t tot = 0;
-        for (int t = 0; t < nt; t++) {
-            o[t] = (int)tot;
-            tot += c[t];
-            if (tot > (int64_t)INT32_MAX - 1) {
-                rc = SRFX_EUNSUPPORTED;
-                goto done;
+    // per plan: slot offsets per tile (into the counters' off part), then
+    // chunks window by window, each window's slabs in order (an event's
+    // listings in consecutive slabs mostly share a window, so its record is
+    // read again while the L2 holds it)
+    for (int s = 0; s < S; s++) {
+        const PlanSet &ps = sets[s];
+        Layout &l = L[s];
+        const int nt = l.nt, nw = l.g.nwx * l.g.nwy;
+        l.totals.assign(ps.K, 0);
+        l.nch.assign(ps.K, 0);
+        l.ch0 = ch.size();
+        for (int k = 0; k < ps.K; k++) {
+            const int *c = ctr.data() + l.ctr + (size_t)k * nt;
+            int *o = ctr.data() + l.ctr + (size_t)(2 * ps.K + k) * nt;
+            int64_t tot = 0;
+            for (int t = 0; t < nt; t++) {
+                o[t] = (int)tot;
+                tot += c[t];
+                if (tot > (int64_t)INT32_MAX - 1) {
+                    rc = SRFX_EUNSUPPORTED;
+                    goto done;
+                }
             }
+            l.totals[k] = tot;
+            const size_t before = ch.size();
+            for (int w = 0; w < nw; w++)
+                for (int t = w; t < nt; t += nw)
+                    for (int q = 0; q < c[t]; q += ps.chunk)
+                        ch.push_back(TileChunk{t, o[t] + q,
+                                               c[t] - q < ps.chunk ? c[t] - q : ps.chunk, 0});
+            l.nch[k] = (int)(ch.size() - before);
         }
-        totals[k] = tot;
-        const size_t before = ch.size();
-        const int nw = g.nwx * g.nwy;
-        for (int w = 0; w < nw; w++)
-            for (int t = w; t < nt; t += nw)
-                for (int s = 0; s < c[t]; s += chunk)
-                    ch.push_back(TileChunk{t, o[t] + s, c[t] - s < chunk ? c[t] - s : chunk, 0});
-        nch[k] = (int)(ch.size() - before);
     }
     {
         // block 1: every plan's chunks, contiguous, then the lists
         const size_t chunk_b = align256(sizeof(TileChunk) * std::max<size_t>(ch.size(), 1));
         size_t b1 = chunk_b;
-        std::vector<size_t> list_off(K);
-        for (int k = 0; k < K; k++) {
-            list_off[k] = b1;
-            b1 += align256(sizeof(int) * (size_t)std::max<int64_t>(totals[k], 1));
-        }
+        std::vector<size_t> list_off(nplans);
+        for (int s = 0, j = 0; s < S; s++)
+            for (int k = 0; k < sets[s].K; k++, j++) {
+                list_off[j] = b1;
+                b1 += align256(sizeof(int) * (size_t)std::max<int64_t>(L[s].totals[k], 1));
+            }
         if ((rc = dev_alloc(&blk1, b1)))
             goto done;
         arena->mem[1] = blk1;
-        size_t c0 = 0;
-        for (int k = 0; k < K; k++) {
-            pls[k]->chunks = (TileChunk *)blk1 + c0;
-            pls[k]->nchunks = nch[k];
-            c0 += nch[k];
-            pls[k]->list = (int *)(blk1 + list_off[k]);
-            pls[k]->nlist = totals[k];
-            pls[k]->g = g;
-            ptrs[k] = pls[k]->list;
+        for (int s = 0, j = 0; s < S; s++) {
+            const PlanSet &ps = sets[s];
+            const Layout &l = L[s];
+            char *h = head.data() + l.head;
+            auto **lp = reinterpret_cast<void **>(h + sizeof(int64_t) * (ps.K + 1) +
+                                                  sizeof(void *) * ps.K);
+            size_t c0 = l.ch0;
+            for (int k = 0; k < ps.K; k++, j++) {
+                srfx_tor_plan_s *pl = ps.pls[k];
+                pl->chunks = (TileChunk *)blk1 + c0;
+                pl->nchunks = l.nch[k];
+                pl->hchunks.assign(ch.begin() + c0, ch.begin() + c0 + l.nch[k]);
+                c0 += l.nch[k];
+                pl->list = (int *)(blk1 + list_off[j]);
+                pl->nlist = l.totals[k];
+                pl->g = l.g;
+                lp[k] = pl->list;
+            }
         }
     }
-    if (hipMemcpyAsync(off_d, off.data(), sizeof(int) * off.size(), hipMemcpyHostToDevice, st) !=
-            hipSuccess ||
-        hipMemcpyAsync(list_d, ptrs.data(), sizeof(void *) * K, hipMemcpyHostToDevice, st) !=
-            hipSuccess ||
+    if ((ctr_n > 0 && hipMemcpyAsync(ctr_d, ctr.data(), sizeof(int) * ctr_n,
+                                     hipMemcpyHostToDevice, st) != hipSuccess) ||
+        hipMemcpyAsync(tmp, head.data(), head_b, hipMemcpyHostToDevice, st) != hipSuccess ||
         (!ch.empty() && hipMemcpyAsync(blk1, ch.data(), sizeof(TileChunk) * ch.size(),
                                        hipMemcpyHostToDevice, st) != hipSuccess)) {
         rc = SRFX_ELAUNCH;
         goto done;
     }
-    if (first[K] > 0)
-        hipLaunchKernelGGL(plan_set_pass<true>, grid, block, lds ? lds_fill : 0u, st, p, g, a);
-    if ((rc = launch_status()))
+    if ((rc = launch(true)))
         goto done;
-    if (hipMemcpyAsync(&mismatch, err_d, sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess ||
+    if (hipMemcpyAsync(errs.data(), err_d, sizeof(int) * S, hipMemcpyDeviceToHost, st) !=
+            hipSuccess ||
         hipStreamSynchronize(st) != hipSuccess) {
         rc = SRFX_ELAUNCH;
         goto done;
     }
-    if (mismatch) {
-        rc = SRFX_ELAUNCH; // count and fill passes disagreed: no valid plan
-        goto done;
-    }
+    for (int s = 0; s < S; s++)
+        if (errs[s]) {
+            rc = SRFX_ELAUNCH; // count and fill passes disagreed: no valid plan
+            goto done;
+        }
     // listings of each chunk ordered by slice count, longest first
-    if (chunk <= 16 * 256) {
-        for (int k = 0; k < K; k++) {
-            srfx_tor_plan_s *pl = pls[k];
+    for (const PlanSet &ps : sets) {
+        if (ps.chunk > 16 * 256)
+            continue;
+        for (int k = 0; k < ps.K; k++) {
+            srfx_tor_plan_s *pl = ps.pls[k];
             if (pl->nchunks > 0)
                 hipLaunchKernelGGL(sort_chunks, dim3(pl->nchunks), dim3(256),
-                                   sizeof(int) * (33 + chunk), st, pl->p, pl->g, pl->chunks,
+                                   sizeof(int) * (33 + ps.chunk), st, pl->p, pl->g, pl->chunks,
                                    pl->list, pl->rec);
             pl->g.sorted = 1;
         }
-        rc = launch_status();
     }
+    rc = launch_status();
 done:
     // host buffers above are read by the copies: finish them before returning
     (void)hipStreamSynchronize(st);
     (void)hipFree(tmp);
     return rc;
+}
+
+int plan_build_set(srfx_tor_plan_s *const *pls, int K, const int64_t *first, const float *lors,
+                   int chunk, hipStream_t st)
+{
+    return plan_build_sets({PlanSet{pls, K, first, lors, chunk}}, st);
 }
 
 template <int MODE, bool VREC, bool YF>
@@ -634,40 +711,59 @@ extern "C" int srfx_tor_plan_create(const float *lors, int64_t n, int64_t ld, in
     return SRFX_OK;
 }
 
-extern "C" int srfx_tor_plan_create_subsets(const float *lors, int64_t n, int64_t ld,
-                                            int32_t ncols, srfx_grid grid, srfx_tor_model m,
-                                            srfx_tile tile, int32_t nplans, const int64_t *bounds,
-                                            srfx_tor_plan *plans, void *stream)
+namespace srfx {
+
+int plan_create_subsets_many(const SubsetRequest *req, int nreq, int32_t ncols,
+                             const srfx_tor_model &m, const srfx_tile &tile, srfx_tor_plan *plans,
+                             hipStream_t st)
 {
-    if (!plans || !bounds || nplans < 1 || n < 0)
+    if (!plans || (nreq > 0 && !req) || nreq < 0)
         return SRFX_EINVAL;
-    for (int k = 0; k < nplans; k++)
-        plans[k] = nullptr;
-    if (bounds[0] < 0 || bounds[nplans] > n)
-        return SRFX_EINVAL;
-    for (int k = 0; k < nplans; k++)
-        if (bounds[k + 1] < bounds[k])
+    int total = 0;
+    for (int r = 0; r < nreq; r++) {
+        const SubsetRequest &q = req[r];
+        if (!q.bounds || q.nplans < 1 || q.n < 0)
             return SRFX_EINVAL;
-    std::vector<srfx_tor_plan_s *> pls(nplans, nullptr);
-    std::vector<int64_t> first(nplans + 1);
+        total += q.nplans;
+    }
+    for (int i = 0; i < total; i++)
+        plans[i] = nullptr;
+    for (int r = 0; r < nreq; r++) {
+        const SubsetRequest &q = req[r];
+        if (q.bounds[0] < 0 || q.bounds[q.nplans] > q.n)
+            return SRFX_EINVAL;
+        for (int k = 0; k < q.nplans; k++)
+            if (q.bounds[k + 1] < q.bounds[k])
+                return SRFX_EINVAL;
+    }
+    std::vector<srfx_tor_plan_s *> pls(total, nullptr);
+    std::vector<std::vector<int64_t>> first(nreq);
     int rc = SRFX_OK;
     bool together = tile.projection == 0;
-    for (int k = 0; k < nplans && !rc; k++) {
-        const int64_t nk = bounds[k + 1] - bounds[k];
-        rc = plan_new(lors ? lors + bounds[k] : nullptr, nk, ld, ncols, grid, m, tile, &pls[k]);
-        if (!rc)
-            together = together && set_buildable(pls[k]->g, nk, false);
-        first[k] = bounds[k] - bounds[0];
+    for (int r = 0, i = 0; r < nreq && !rc; r++) {
+        const SubsetRequest &q = req[r];
+        first[r].resize(q.nplans + 1);
+        for (int k = 0; k < q.nplans && !rc; k++, i++) {
+            const int64_t nk = q.bounds[k + 1] - q.bounds[k];
+            rc = plan_new(q.lors ? q.lors + q.bounds[k] : nullptr, nk, q.ld, ncols, q.grid, m,
+                          tile, &pls[i]);
+            if (!rc)
+                together = together && set_buildable(pls[i]->g, nk, false);
+            first[r][k] = q.bounds[k] - q.bounds[0];
+        }
+        first[r][q.nplans] = q.bounds[q.nplans] - q.bounds[0];
     }
-    first[nplans] = bounds[nplans] - bounds[0];
-    hipStream_t st = (hipStream_t)stream;
     if (!rc) {
         if (together) {
-            rc = plan_build_set(pls.data(), nplans, first.data(), lors + bounds[0], tile.chunk,
-                                st);
+            std::vector<PlanSet> sets;
+            for (int r = 0, i = 0; r < nreq; i += req[r].nplans, r++)
+                sets.push_back(PlanSet{pls.data() + i, req[r].nplans, first[r].data(),
+                                       req[r].lors ? req[r].lors + req[r].bounds[0] : nullptr,
+                                       tile.chunk});
+            rc = plan_build_sets(sets, st);
         } else {
-            for (int k = 0; k < nplans && !rc; k++)
-                rc = plan_build(pls[k], st, tile.chunk, tile.projection != 0);
+            for (int i = 0; i < total && !rc; i++)
+                rc = plan_build(pls[i], st, tile.chunk, tile.projection != 0);
         }
     }
     if (rc) {
@@ -675,9 +771,22 @@ extern "C" int srfx_tor_plan_create_subsets(const float *lors, int64_t n, int64_
             plan_free(pl);
         return rc;
     }
-    for (int k = 0; k < nplans; k++)
-        plans[k] = pls[k];
+    for (int i = 0; i < total; i++)
+        plans[i] = pls[i];
     return SRFX_OK;
+}
+
+} // namespace srfx
+
+extern "C" int srfx_tor_plan_create_subsets(const float *lors, int64_t n, int64_t ld,
+                                            int32_t ncols, srfx_grid grid, srfx_tor_model m,
+                                            srfx_tile tile, int32_t nplans, const int64_t *bounds,
+                                            srfx_tor_plan *plans, void *stream)
+{
+    if (!plans || !bounds || nplans < 1 || n < 0)
+        return SRFX_EINVAL;
+    const SubsetRequest q{lors, n, ld, grid, nplans, bounds};
+    return plan_create_subsets_many(&q, 1, ncols, m, tile, plans, (hipStream_t)stream);
 }
 
 extern "C" int srfx_tor_plan_info(srfx_tor_plan plan, int64_t *nlistings, int32_t *nchunks)

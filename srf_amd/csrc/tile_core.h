@@ -27,6 +27,7 @@
 #pragma once
 
 #include <atomic>
+#include <vector>
 
 #include "tor_core.h"
 
@@ -996,6 +997,23 @@ __global__ __launch_bounds__(256) void tile_reduce(const float *__restrict__ par
 }
 
 } // namespace
+// The OSEM subset plans of several LOR arrays at once: for request r, the
+// plans srfx_tor_plan_create_subsets builds for (lors, n, ld, grid, nplans,
+// bounds), written in request order to plans (sum of nplans entries).  All
+// requests share the build's phases (one count pass per request, one host
+// synchronisation, one fill pass per request, one host synchronisation) and
+// two device blocks.
+struct SubsetRequest {
+    const float *lors;
+    int64_t n, ld;
+    srfx_grid grid;
+    int32_t nplans;
+    const int64_t *bounds;
+};
+int plan_create_subsets_many(const SubsetRequest *req, int nreq, int32_t ncols,
+                             const srfx_tor_model &m, const srfx_tile &tile, srfx_tor_plan *plans,
+                             hipStream_t st);
+
 } // namespace srfx
 
 // Plan object of srfx_tor_plan_create.
@@ -1020,4 +1038,5 @@ struct srfx_tor_plan_s {
         void *mem[2];
         std::atomic<int> refs;
     } *arena;
+    std::vector<srfx::TileChunk> hchunks; // host copy of the chunk table
 };
