@@ -42,6 +42,16 @@ struct EvChunk {
     int job, start, count, pad;
 };
 
+// Box classes of the batch's jobs: 0 = candidate box 3, 1 = box 4, 2 = chord
+// walk.  Each class has its own work lists and persistent kernels, so that a
+// kernel holds only its class's walk (and its registers).
+constexpr int kBoxClasses = 3;
+
+inline int box_class(int box)
+{
+    return box == 3 ? 0 : (box == 4 ? 1 : 2);
+}
+
 struct TileWork {
     int job, tile, start, count;
 };
@@ -74,7 +84,7 @@ __device__ __forceinline__ int find_range(const VoxRange *r, int nrec, long long
 __global__ __launch_bounds__(256) void batch_zero(const VoxRange *__restrict__ r, int nrec,
                                                   long long total, unsigned *counters)
 {
-    if (blockIdx.x == 0 && threadIdx.x < 2)
+    if (blockIdx.x == 0 && threadIdx.x < 2 * kBoxClasses)
         counters[threadIdx.x] = 0u;
     long long stride = (long long)gridDim.x * blockDim.x;
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += stride) {
@@ -118,14 +128,13 @@ __device__ __forceinline__ int next_wave_item(unsigned *counter)
 }
 
 // Waves pull work items of T events from the event chunks (kChunk / T items
-// per chunk), walked in rounds of 64 / LPE events at LPE lanes per event,
-// without synchronising with the other waves of their block, so a wave with
-// long walks holds up no other.  The default schedule, LPE = 4 and T = 64
-// (quarter chunks), is the projection schedule of the plan OSEM step.  The
-// gathers are the strided ones: the frame gathers (record_project<.., true>),
-// faster for one plan, measured slower in this kernel (C4: 657 against 751
-// Mevents/s).
-template <int LPE, int T>
+// per chunk) of one box class (NB), walked in rounds of 64 / LPE events at
+// LPE lanes per event, without synchronising with the other waves of their
+// block, so a wave with long walks holds up no other.  The default schedule,
+// LPE = 4 and T = 64 (quarter chunks), is the projection schedule of the plan
+// OSEM step.  The gathers read the jobs' contiguous op frames
+// (record_project<.., true>).
+template <int NB, int LPE, int T>
 __global__ __launch_bounds__(256) void batch_project(const BatchJob *__restrict__ jobs,
                                                      const EvChunk *__restrict__ chunks,
                                                      int nchunks, unsigned *counter)
@@ -144,40 +153,17 @@ __global__ __launch_bounds__(256) void batch_project(const BatchJob *__restrict_
             const int i = (q % kParts) * T + r + lane / LPE;
             const bool valid = i < ch.count;
             const int64_t e = (int64_t)ch.start + i;
-            switch (job.p.box) {
-            case 3:
-                record_project<3, LPE>(job.p, rec, e, sub, valid);
-                break;
-            case 4:
-                record_project<4, LPE>(job.p, rec, e, sub, valid);
-                break;
-            default:
-                record_project<0, LPE>(job.p, rec, e, sub, valid);
-                break;
-            }
+            record_project<NB, LPE, true>(job.p, rec, e, sub, valid);
         }
     }
 }
 
-template <bool YF>
-__device__ __forceinline__ void scatter_job(const BatchJob &job, const TileChunk &ch, double *ldsd)
-{
-    switch (job.p.box) {
-    case 3:
-        scatter_chunk<kBack, 3, true, YF>(job.p, job.g, ch, job.list, job.rec, ldsd);
-        break;
-    case 4:
-        scatter_chunk<kBack, 4, true, YF>(job.p, job.g, ch, job.list, job.rec, ldsd);
-        break;
-    default:
-        scatter_chunk<kBack, 0, true, YF>(job.p, job.g, ch, job.list, job.rec, ldsd);
-        break;
-    }
-}
-
-__global__ __launch_bounds__(256) void batch_scatter(const BatchJob *__restrict__ jobs,
-                                                     const TileWork *__restrict__ work,
-                                                     int nwork, unsigned *counter)
+// Blocks pull tile chunks of one box class (NB); four waves per SIMD where
+// the walk fits (as tile_scatter).
+template <int NB>
+__global__ __launch_bounds__(256, NB == 4 ? 3 : 4) void batch_scatter(
+    const BatchJob *__restrict__ jobs, const TileWork *__restrict__ work, int nwork,
+    unsigned *counter)
 {
     extern __shared__ double ldsd[];
     for (;;) {
@@ -188,9 +174,9 @@ __global__ __launch_bounds__(256) void batch_scatter(const BatchJob *__restrict_
         const BatchJob &job = jobs[__builtin_amdgcn_readfirstlane(w.job)];
         const TileChunk ch{w.tile, w.start, w.count, 0};
         if (flush_y_fastest(job.p))
-            scatter_job<true>(job, ch, ldsd);
+            scatter_chunk<kBack, NB, true, true>(job.p, job.g, ch, job.list, job.rec, ldsd);
         else
-            scatter_job<false>(job, ch, ldsd);
+            scatter_chunk<kBack, NB, true, false>(job.p, job.g, ch, job.list, job.rec, ldsd);
     }
 }
 
@@ -255,6 +241,7 @@ struct srfx_tor_batch {
     int nrec = 0, nsub = 0, njobs = 0;
     std::vector<srfx_tor_plan> plans;
     srfx::BatchJob *d_jobs = nullptr;
+    // work lists per (subset, box class): index subset * kBoxClasses + class
     std::vector<srfx::EvChunk *> d_ev;
     std::vector<int> nev;
     std::vector<srfx::TileWork *> d_work;
@@ -269,7 +256,8 @@ struct srfx_tor_batch {
     int nframes = 0;
     long long frame_blocks = 0;
     unsigned scatter_lds = 0;
-    int proj_blocks = 0, scatter_blocks = 0, elem_blocks = 0;
+    int proj_blocks[srfx::kBoxClasses] = {}, scatter_blocks[srfx::kBoxClasses] = {};
+    int elem_blocks = 0;
     int cus = 0;
     int proj_lanes = 4, proj_events = 64; // projection schedule (batch_project<LPE, T>)
 };
@@ -278,20 +266,33 @@ using namespace srfx;
 
 // The persistent projection of schedule (LPE, T): with subset < 0 sets the
 // plan's grid from the kernel's occupancy, else enqueues it for that subset.
-template <int LPE, int T>
-static int project_launch(srfx_tor_batch *b, int subset, hipStream_t st)
+template <int NB, int LPE, int T>
+static int project_class(srfx_tor_batch *b, int c, int subset, hipStream_t st)
 {
     if (subset < 0) {
         int per_cu = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_project<LPE, T>, 256, 0) !=
-            hipSuccess)
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_project<NB, LPE, T>, 256,
+                                                         0) != hipSuccess)
             return SRFX_ELAUNCH;
-        b->proj_blocks = std::max(1, b->cus * std::max(1, per_cu));
+        b->proj_blocks[c] = std::max(1, b->cus * std::max(1, per_cu));
         return SRFX_OK;
     }
-    hipLaunchKernelGGL((batch_project<LPE, T>), dim3(b->proj_blocks), dim3(256), 0, st, b->d_jobs,
-                       b->d_ev[subset], b->nev[subset], b->d_counters);
+    const int w = subset * kBoxClasses + c;
+    if (b->nev[w] > 0)
+        hipLaunchKernelGGL((batch_project<NB, LPE, T>), dim3(b->proj_blocks[c]), dim3(256), 0, st,
+                           b->d_jobs, b->d_ev[w], b->nev[w], b->d_counters + 2 * c);
     return SRFX_OK;
+}
+
+template <int LPE, int T>
+static int project_launch(srfx_tor_batch *b, int subset, hipStream_t st)
+{
+    int rc = project_class<3, LPE, T>(b, 0, subset, st);
+    if (!rc)
+        rc = project_class<4, LPE, T>(b, 1, subset, st);
+    if (!rc)
+        rc = project_class<0, LPE, T>(b, 2, subset, st);
+    return rc;
 }
 
 // Lanes per event {1, 4, 16} x events per work item {64, 128, 256}.
@@ -364,6 +365,9 @@ extern "C" int srfx_tor_batch_create(const srfx_recon *recons, int32_t nrecon,
                            m.geo_sigma2_flag, m.tof_bin, m.tof_sigma2);
             if (st)
                 return st;
+            // the frame gathers address an op frame with 32-bit byte offsets
+            if (4 * (int64_t)rc.shape[0] * rc.shape[1] * rc.shape[2] >= ((int64_t)1 << 31))
+                return SRFX_EUNSUPPORTED;
         }
     }
     srfx_tor_batch *b = new (std::nothrow) srfx_tor_batch;
@@ -372,8 +376,8 @@ extern "C" int srfx_tor_batch_create(const srfx_recon *recons, int32_t nrecon,
     b->nrec = nrecon;
     b->nsub = nb_subsets;
     std::vector<BatchJob> jobs;
-    std::vector<std::vector<EvChunk>> ev(nb_subsets);
-    std::vector<std::vector<TileWork>> work(nb_subsets);
+    std::vector<std::vector<EvChunk>> ev((size_t)nb_subsets * kBoxClasses);
+    std::vector<std::vector<TileWork>> work((size_t)nb_subsets * kBoxClasses);
     std::vector<VoxRange> vox;
     std::vector<FrameJob> frames;
     long long off = 0;
@@ -456,10 +460,12 @@ extern "C" int srfx_tor_batch_create(const srfx_recon *recons, int32_t nrecon,
                     continue;
                 BatchJob job;
                 job.p = plan->p;
-                job.p.img = frame; // contiguous op frame
+                job.p.img = frame; // contiguous op frame, read by the frame gathers
                 job.p.sx = 1;
                 job.p.sy = gg.shape[0];
                 job.p.sz = gg.shape[0] * gg.shape[1];
+                job.p.img_bytes = (int)(4 * (int64_t)job.p.sz * gg.shape[2]);
+                const int bc = box_class(plan->p.box);
                 job.p.out = rc.bp;
                 job.p.val = plan->proj;
                 job.g = plan->g;
@@ -469,7 +475,7 @@ extern "C" int srfx_tor_batch_create(const srfx_recon *recons, int32_t nrecon,
                 const int j = (int)jobs.size();
                 jobs.push_back(job);
                 for (int64_t s = 0; s < hi - lo; s += kChunk)
-                    ev[k].push_back(
+                    ev[(size_t)k * kBoxClasses + bc].push_back(
                         EvChunk{j, (int)s, (int)std::min<int64_t>(kChunk, hi - lo - s), 0});
                 // the plan's chunk table, kept on the host by its build
                 const std::vector<TileChunk> &ch = plan->hchunks;
@@ -478,7 +484,8 @@ extern "C" int srfx_tor_batch_create(const srfx_recon *recons, int32_t nrecon,
                     break;
                 }
                 for (const TileChunk &c : ch)
-                    work[k].push_back(TileWork{j, c.tile, c.start, c.count});
+                    work[(size_t)k * kBoxClasses + bc].push_back(
+                        TileWork{j, c.tile, c.start, c.count});
             }
         }
         long long nv = (long long)rc.shape[0] * rc.shape[1] * rc.shape[2];
@@ -493,21 +500,25 @@ extern "C" int srfx_tor_batch_create(const srfx_recon *recons, int32_t nrecon,
     b->total_vox = off;
     b->nframes = (int)frames.size();
     bool ok = upload(&b->d_jobs, jobs) && upload(&b->d_vox, vox) && upload(&b->d_frames, frames) &&
-              hipMalloc(&b->d_counters, 2 * sizeof(unsigned)) == hipSuccess;
-    for (int k = 0; ok && k < nb_subsets; k++) {
+              hipMalloc(&b->d_counters, 2 * kBoxClasses * sizeof(unsigned)) == hipSuccess;
+    for (size_t w = 0; ok && w < ev.size(); w++) {
         EvChunk *e = nullptr;
-        TileWork *w = nullptr;
-        ok = upload(&e, ev[k]) && upload(&w, work[k]);
+        TileWork *t = nullptr;
+        ok = upload(&e, ev[w]) && upload(&t, work[w]);
         b->d_ev.push_back(e);
-        b->nev.push_back((int)ev[k].size());
-        b->d_work.push_back(w);
-        b->nwork.push_back((int)work[k].size());
+        b->nev.push_back((int)ev[w].size());
+        b->d_work.push_back(t);
+        b->nwork.push_back((int)work[w].size());
     }
     b->scatter_lds = (unsigned)(sizeof(double) * kBatchTile.slices * kBatchTile.wx * kBatchTile.wy);
-    int dev = 0, cus = 0, per_cu_s = 0;
+    int dev = 0, cus = 0, per_cu_s[kBoxClasses] = {};
     ok = ok && hipGetDevice(&dev) == hipSuccess &&
          hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
-         hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu_s, batch_scatter, 256,
+         hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu_s[0], batch_scatter<3>, 256,
+                                                      b->scatter_lds) == hipSuccess &&
+         hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu_s[1], batch_scatter<4>, 256,
+                                                      b->scatter_lds) == hipSuccess &&
+         hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu_s[2], batch_scatter<0>, 256,
                                                       b->scatter_lds) == hipSuccess;
     b->cus = cus;
     ok = ok && project_schedule(b, b->proj_lanes, b->proj_events, -1, nullptr) == SRFX_OK;
@@ -515,7 +526,8 @@ extern "C" int srfx_tor_batch_create(const srfx_recon *recons, int32_t nrecon,
         free_plan(b);
         return SRFX_ELAUNCH;
     }
-    b->scatter_blocks = std::max(1, cus * std::max(1, per_cu_s));
+    for (int c = 0; c < kBoxClasses; c++)
+        b->scatter_blocks[c] = std::max(1, cus * std::max(1, per_cu_s[c]));
     b->elem_blocks = std::max(1, cus * 4);
     *out = b;
     return SRFX_OK;
@@ -531,11 +543,24 @@ extern "C" int srfx_tor_batch_subset(srfx_tor_batch *b, int32_t subset, void *st
     if (b->nframes > 0)
         hipLaunchKernelGGL(batch_frames, dim3((unsigned)b->frame_blocks), dim3(256), 0, st,
                            b->d_frames, b->nframes);
-    if (b->nev[subset] > 0)
-        project_schedule(b, b->proj_lanes, b->proj_events, subset, st);
-    if (b->nwork[subset] > 0)
-        hipLaunchKernelGGL(batch_scatter, dim3(b->scatter_blocks), dim3(256), b->scatter_lds, st,
-                           b->d_jobs, b->d_work[subset], b->nwork[subset], b->d_counters + 1);
+    project_schedule(b, b->proj_lanes, b->proj_events, subset, st);
+    for (int c = 0; c < kBoxClasses; c++) {
+        const int w = subset * kBoxClasses + c;
+        if (b->nwork[w] == 0)
+            continue;
+        if (c == 0)
+            hipLaunchKernelGGL(batch_scatter<3>, dim3(b->scatter_blocks[c]), dim3(256),
+                               b->scatter_lds, st, b->d_jobs, b->d_work[w], b->nwork[w],
+                               b->d_counters + 2 * c + 1);
+        else if (c == 1)
+            hipLaunchKernelGGL(batch_scatter<4>, dim3(b->scatter_blocks[c]), dim3(256),
+                               b->scatter_lds, st, b->d_jobs, b->d_work[w], b->nwork[w],
+                               b->d_counters + 2 * c + 1);
+        else
+            hipLaunchKernelGGL(batch_scatter<0>, dim3(b->scatter_blocks[c]), dim3(256),
+                               b->scatter_lds, st, b->d_jobs, b->d_work[w], b->nwork[w],
+                               b->d_counters + 2 * c + 1);
+    }
     hipLaunchKernelGGL(batch_update, dim3(b->elem_blocks), dim3(256), 0, st, b->d_vox, b->nrec,
                        b->total_vox);
     return launch_status();
