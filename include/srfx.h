@@ -275,8 +275,11 @@ int srfx_tor_plan_info(srfx_tor_plan plan, int64_t *nlistings, int32_t *nchunks)
 /* Synchronises the device, then frees the plan. */
 int srfx_tor_plan_destroy(srfx_tor_plan plan);
 
-/* Projection (tor.cc:16-28) of the plan's events; deterministic.  Returns
- * SRFX_EUNSUPPORTED for plans built without `projection`. */
+/* Projection (tor.cc:16-28) of the plan's events; deterministic.  Plans
+ * built with `projection` use the tiled gather; the others (and every call
+ * with SRFX_IMAGE_OP_FRAME, whose image is the op-frame copy) walk each
+ * event from its walk record, as the first kernel of the plan's OSEM step
+ * does, with the flags' schedule. */
 int srfx_tor_plan_project(srfx_tor_plan plan, const float *image, float *proj, uint32_t flags,
                           void *stream);
 

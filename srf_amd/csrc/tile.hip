@@ -84,34 +84,36 @@ bool frame_image(const TorParams &p, int &bytes)
 }
 
 template <int NB, int LPE>
-int launch_record_pg(const TorParams &p, float4 *rec, int block, hipStream_t st)
+int launch_record_pg(const TorParams &p, float4 *rec, float *pout, int block, hipStream_t st)
 {
     TorParams q = p;
     if (frame_image(q, q.img_bytes))
         hipLaunchKernelGGL((record_proj_kernel<NB, LPE, true>), dim3(blocks_for(q.n * LPE, block)),
-                           dim3(block), 0, st, q, rec);
+                           dim3(block), 0, st, q, rec, pout);
     else
         hipLaunchKernelGGL((record_proj_kernel<NB, LPE, false>),
-                           dim3(blocks_for(q.n * LPE, block)), dim3(block), 0, st, q, rec);
+                           dim3(blocks_for(q.n * LPE, block)), dim3(block), 0, st, q, rec, pout);
     return launch_status();
 }
 
 template <int LPE>
-int launch_record_lpe(const TorParams &p, float4 *rec, int block, hipStream_t st)
+int launch_record_lpe(const TorParams &p, float4 *rec, float *pout, int block, hipStream_t st)
 {
     switch (p.full_patch ? 0 : p.box) {
     case 3:
-        return launch_record_pg<3, LPE>(p, rec, block, st);
+        return launch_record_pg<3, LPE>(p, rec, pout, block, st);
     case 4:
-        return launch_record_pg<4, LPE>(p, rec, block, st);
+        return launch_record_pg<4, LPE>(p, rec, pout, block, st);
     default:
-        return launch_record_pg<0, LPE>(p, rec, block, st);
+        return launch_record_pg<0, LPE>(p, rec, pout, block, st);
     }
 }
 
 // The record projection with the schedule of the flags (lanes per event,
-// block size), as launch<kProj> takes it.
-int launch_record_project(const TorParams &p, float4 *rec, uint32_t flags, hipStream_t st)
+// block size), as launch<kProj> takes it; into the records' value slots
+// (1/p), or with pout the projection itself into pout.
+int launch_record_project(const TorParams &p, float4 *rec, uint32_t flags, hipStream_t st,
+                          float *pout = nullptr)
 {
     if (p.n == 0)
         return SRFX_OK;
@@ -120,11 +122,11 @@ int launch_record_project(const TorParams &p, float4 *rec, uint32_t flags, hipSt
     // no lanes-per-event flag: four lanes per event, the fastest measured
     switch ((flags >> SRFX_LPE_SHIFT) & 3u) {
     case 1:
-        return launch_record_lpe<1>(p, rec, block, st);
+        return launch_record_lpe<1>(p, rec, pout, block, st);
     case 3:
-        return launch_record_lpe<16>(p, rec, block, st);
+        return launch_record_lpe<16>(p, rec, pout, block, st);
     default:
-        return launch_record_lpe<4>(p, rec, block, st);
+        return launch_record_lpe<4>(p, rec, pout, block, st);
     }
 }
 
@@ -813,8 +815,20 @@ extern "C" int srfx_tor_plan_project(srfx_tor_plan plan, const float *image, flo
 {
     if (!plan || plan->cols7 || (plan->p.n > 0 && (!image || !proj)))
         return SRFX_EINVAL;
-    (void)flags;
-    return plan_project(plan, image, proj, (hipStream_t)stream);
+    hipStream_t st = (hipStream_t)stream;
+    if (plan->ev_slot && !(flags & SRFX_IMAGE_OP_FRAME))
+        return plan_project(plan, image, proj, st);
+    // a plan without the tiled gather's tables: the projection from the walk
+    // records (the first kernel of the plan's OSEM step), with the flags'
+    // schedule; SRFX_IMAGE_OP_FRAME: image is the op-frame copy
+    TorParams q = plan->p;
+    q.img = image;
+    if (flags & SRFX_IMAGE_OP_FRAME) {
+        q.sx = 1;
+        q.sy = q.gx;
+        q.sz = q.gx * q.gy;
+    }
+    return launch_record_project(q, plan->rec, flags, st, proj);
 }
 
 static int plan_scatter_entry(srfx_tor_plan plan, int mode, const float *vals, float *image,

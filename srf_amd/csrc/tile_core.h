@@ -711,10 +711,11 @@ __device__ __forceinline__ float tile_event(const TorParams &p, const TileGeom &
 // each of the event's lanes.  The head's value slot receives the scale of
 // the event's backprojection: 1/p for a projection p > 1e-7, else 0 (the
 // event adds nothing, as the reference's ratio skips it), so that each of the
-// event's listings reads it instead of dividing again.
+// event's listings reads it instead of dividing again.  With pout the
+// projection p itself goes to pout[e] instead (the plan's projection op).
 template <int NB, int LPE, bool FRAME = false>
 __device__ __forceinline__ void record_project(const TorParams &p, float4 *rec, int64_t e,
-                                               int sub, bool valid)
+                                               int sub, bool valid, float *pout = nullptr)
 {
     float part = 0.0f;
     Event ev;
@@ -736,9 +737,13 @@ __device__ __forceinline__ void record_project(const TorParams &p, float4 *rec, 
         part = walk_event<0, NB, FRAME>(p, ev, t, p.use_tof, 0.0f, base + sub, ev.s_hi, LPE,
                                         ev.s_lo);
     const float pv = group_sum<LPE>(part);
-    if (valid && sub == 0)
-        reinterpret_cast<float *>(rec)[4 * kWalkHead * e + kWalkValue] =
-            pv > 1e-7 ? 1.0f / pv : 0.0f;
+    if (valid && sub == 0) {
+        if (pout)
+            pout[e] = pv;
+        else
+            reinterpret_cast<float *>(rec)[4 * kWalkHead * e + kWalkValue] =
+                pv > 1e-7 ? 1.0f / pv : 0.0f;
+    }
 }
 
 // FRAME: p.img is a contiguous op frame of p.img_bytes (frame gathers).
@@ -754,11 +759,11 @@ __device__ __forceinline__ unsigned xcd_block()
 }
 
 template <int NB, int LPE, bool FRAME>
-__global__ __launch_bounds__(256) void record_proj_kernel(TorParams p, float4 *rec)
+__global__ __launch_bounds__(256) void record_proj_kernel(TorParams p, float4 *rec, float *pout)
 {
     const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t e = gid / LPE;
-    record_project<NB, LPE, FRAME>(p, rec, e, (int)(gid % LPE), e < p.n);
+    record_project<NB, LPE, FRAME>(p, rec, e, (int)(gid % LPE), e < p.n, pout);
 }
 
 

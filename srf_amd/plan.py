@@ -120,10 +120,21 @@ class TorPlan:
             return torch.empty((g[2], g[1], g[0]), dtype=torch.float32, device=like.device)
         return self._image(out, "out")
 
-    def projection(self, image):
-        image = self._image(image)
+    def projection(self, image, op_frame=False, schedule=None):
+        """Projection of the plan's events: the tiled gather for plans built
+        with tiled_projection, else the walk from the event records (the OSEM
+        step's projection kernel).  op_frame: image is the contiguous op-frame
+        copy [gz][gy][gx] (frame gathers)."""
+        if op_frame:
+            _check_dev(image, "image")
+            g = self.grid
+            if tuple(image.shape) != (g[2], g[1], g[0]) or not image.is_contiguous():
+                raise ValueError(f"op-frame image must be a contiguous {g[::-1]} tensor")
+        else:
+            image = self._image(image)
         out = torch.empty(self.n, dtype=torch.float32, device=image.device)
-        _lib.call("srfx_tor_plan_project", self._h, _ptr(image), _ptr(out), 0, _stream())
+        flags = _lib.schedule_flags(schedule) | (_lib.SRFX_IMAGE_OP_FRAME if op_frame else 0)
+        _lib.call("srfx_tor_plan_project", self._h, _ptr(image), _ptr(out), flags, _stream())
         return out
 
     def _scatter(self, name, values, out, accumulate, device):

@@ -73,6 +73,28 @@ def test_plan_project_backproject(cuda, case, tile, chunk, geo, tof):
     plan.close()
 
 
+# The projection from the event records (the OSEM step's walk), on the
+# image in place and on the contiguous op-frame copy (frame gathers).
+@pytest.mark.parametrize("geo,tof", [(True, True), (False, True), (True, False)])
+def test_plan_record_projection(cuda, case, geo, tof):
+    from oracle import lib as O
+    lors, img = case
+    ts2 = H.TOF_SIGMA2 if tof else 1.0e6
+    args = (H.TOR_GRID, H.TOR_CENTER, H.TOR_SIZE, H.KW, geo, H.TOF_BIN, ts2)
+    L, X = _t(lors, cuda), _t(img, cuda)
+    assert X.is_contiguous() and tuple(X.shape) == tuple(H.TOR_GRID[::-1])
+    ref = O.tor_project(lors, img, *args, prec="f64")
+    for tiled in (False, True):
+        plan = _plan(L, geo, tof, (8, 32, 32), 2048, tiled_projection=tiled)
+        a = plan.projection(X, op_frame=True).cpu().numpy()
+        assert H.rel_l2(a, ref) <= TOL64
+        if not tiled:
+            b = plan.projection(X).cpu().numpy()
+            assert H.rel_l2(b, ref) <= TOL64
+            assert np.abs(a - b).max() <= 1e-6 * np.abs(ref).max()
+        plan.close()
+
+
 # Plans without the tiled projection's slot tables sort their chunks and
 # carry each listing's slices in its list entry when the slab has at most 16
 # slices; a 20-slice slab sorts without them.
@@ -128,7 +150,8 @@ def test_plan_empty(cuda):
 
 
 def test_plan_without_projection_tables(cuda, case):
-    """A plan built for scatters only refuses the tiled projection and still
+    """A plan built without the tiled projection's tables refuses the OSEM
+    step's tiled projection, projects from its event records instead, and
     backprojects like the untiled op."""
     from srf_amd import ops
     from srf_amd._lib import SRFX_EUNSUPPORTED, SrfxError
@@ -136,12 +159,11 @@ def test_plan_without_projection_tables(cuda, case):
     L, X = _t(lors, cuda), _t(img, cuda)
     plan = _plan(L, False, True, (8, 16, 16), 1024, tiled_projection=False)
     with pytest.raises(SrfxError) as ei:
-        plan.projection(X)
-    assert ei.value.status == SRFX_EUNSUPPORTED
-    with pytest.raises(SrfxError):
         plan.osem_subset(X, tiled_projection=True)
+    assert ei.value.status == SRFX_EUNSUPPORTED
     p = ops.projection(L, X, H.TOR_GRID, H.TOR_CENTER, H.TOR_SIZE, H.KW, False, H.TOF_BIN,
                        H.TOF_SIGMA2)
+    assert H.rel_l2(plan.projection(X).cpu().numpy(), p.cpu().numpy()) <= 1e-6
     b0 = ops.backprojection(X, L, p, H.TOR_GRID, H.TOR_CENTER, H.TOR_SIZE, H.TOF_BIN,
                             H.TOF_SIGMA2, H.KW, False)
     b1 = plan.backprojection(p)
