@@ -402,7 +402,9 @@ typedef struct srfx_tor_batch srfx_tor_batch;
 /* Plan a batch of independent OSEM reconstructions that share one TOR model
  * and one subset count.  Allocates the plan's device tables; no reference
  * counterpart (the reference reconstructs one image per process,
- * graph/reconstruction/local_reconstruction.py:44-51). */
+ * graph/reconstruction/local_reconstruction.py:44-51).  SRFX_EUNSUPPORTED for
+ * an image of 2^31 bytes or more, or with an op-frame slice of 2^24 voxels or
+ * more (the frame gathers' 32-bit offsets). */
 int srfx_tor_batch_create(const srfx_recon *recons, int32_t nrecon, int32_t nb_subsets,
                           srfx_tor_model model, srfx_tor_batch **plan);
 

@@ -366,7 +366,10 @@ extern "C" int srfx_tor_batch_create(const srfx_recon *recons, int32_t nrecon,
             if (st)
                 return st;
             // the frame gathers address an op frame with 32-bit byte offsets
-            if (4 * (int64_t)rc.shape[0] * rc.shape[1] * rc.shape[2] >= ((int64_t)1 << 31))
+            // and 24-bit slice products
+            const srfx_grid gg = group_grid(rc, a);
+            if (4 * (int64_t)rc.shape[0] * rc.shape[1] * rc.shape[2] >= ((int64_t)1 << 31) ||
+                (int64_t)gg.shape[0] * gg.shape[1] >= ((int64_t)1 << 24))
                 return SRFX_EUNSUPPORTED;
         }
     }

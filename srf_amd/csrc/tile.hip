@@ -74,13 +74,15 @@ void launch_pass(const TorParams &p, const TileGeom &g, const PlanArgs &a, hipSt
         hipLaunchKernelGGL((plan_pass<FILL, false>), grid, block, 0, st, p, g, a);
 }
 
-// A read image in contiguous op-frame layout of fewer than 2^31 bytes goes
-// through the frame gathers (buffer loads with 32-bit offsets).
+// A read image in contiguous op-frame layout of fewer than 2^31 bytes, with
+// fewer than 2^24 voxels per slice, goes through the frame gathers (buffer
+// loads with 32-bit offsets formed by 24-bit products).
 bool frame_image(const TorParams &p, int &bytes)
 {
     const int64_t b = (int64_t)4 * p.gx * p.gy * p.gz;
     bytes = (int)b;
-    return p.sx == 1 && p.sy == p.gx && p.sz == p.gx * p.gy && b > 0 && b < ((int64_t)1 << 31);
+    return p.sx == 1 && p.sy == p.gx && p.sz == p.gx * p.gy && b > 0 &&
+           b < ((int64_t)1 << 31) && (int64_t)p.gx * p.gy < ((int64_t)1 << 24);
 }
 
 template <int NB, int LPE>

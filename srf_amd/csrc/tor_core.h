@@ -871,7 +871,11 @@ __device__ __forceinline__ float slice_gather_frame(const TorParams &p, const Ev
     }
     const __amdgpu_buffer_rsrc_t rs =
         __builtin_amdgcn_make_buffer_rsrc((void *)p.img, (short)0, p.img_bytes, 0x00020000);
-    const unsigned o0 = 4u * (unsigned)(s * p.sz + (c.iy0 + jb) * p.sy + c.ix0 + ib);
+    // 24-bit products (full rate): slice and row indices are non-negative and
+    // a frame slice holds fewer than 2^24 voxels
+    const unsigned o0 =
+        4u * (__umul24((unsigned)s, (unsigned)p.sz) +
+              __umul24((unsigned)(c.iy0 + jb), (unsigned)p.sy) + (unsigned)(c.ix0 + ib));
     float v[NC];
 #pragma unroll
     for (int jj = 0; jj < BY; jj++) {
@@ -909,17 +913,16 @@ __device__ __forceinline__ float slice_gather_frame(const TorParams &p, const Ev
 }
 
 // The gather of a box whose patch lies inside the image (most slices of most
-// events): no image clipping, the patch's own edge clipped by an infinite
-// column or row term (dt = +inf fails the dt test), and the slice's
+// events): no image clipping, the box kept inside the patch, and the slice's
 // amplitude multiplying the slice's sum once instead of every candidate's
 // weight.  Returns false, having decided nothing, when the patch is not
 // inside, the box starts before the patch, or a candidate lies within
 // kThrMargin of the cut-off; the caller then runs the clipped gather, which
 // decides those candidates exactly.  The voxels are those of the clipped
-// gather: with the patch inside, it clips the candidates past the patch, as
-// here, and those past the ellipse's span (ib0 + nx and beyond, jb0 + ny and
-// beyond), which lie outside the padded ellipse (dt > kThrMargin) and fail
-// the dt test here.
+// gather: with the patch inside, it drops the candidates past the patch
+// (here the box ends on the patch's edge) and those past the ellipse's span
+// (ib0 + nx and beyond, jb0 + ny and beyond), which lie outside the padded
+// ellipse (dt > kThrMargin) and fail the dt test here.
 template <int BX, int BY>
 __device__ __forceinline__ bool slice_gather_inner(const TorParams &p, const Event &ev,
                                                    const Track &t, const Cross &c, float amp,
@@ -931,23 +934,38 @@ __device__ __forceinline__ bool slice_gather_inner(const TorParams &p, const Eve
     const int jb0 = (int)ceilf((c.dy0 - t.dymax) * p.rfinter_y - eps);
     const int P = p.patch;
     const bool inner = (ib0 >= 0) & (jb0 >= 0) & (c.ix0 >= 0) & (c.ix0 <= p.gx - P) &
-                       (c.iy0 >= 0) & (c.iy0 <= p.gy - P);
+                       (c.iy0 >= 0) & (c.iy0 <= p.gy - P) & (P >= BX) & (P >= BY);
     if (!inner)
         return false;
+    // A box running past the patch's far edge moves back to end on it: a
+    // superset of the patch columns (rows) it covered, all inside the patch,
+    // so no candidate needs clipping.  The added ones lie before the
+    // ellipse's span and fail the dt test (or, within kThrMargin of the
+    // cut-off, send the slice to the clipped gather); the kept voxels and
+    // their order in the sum are those of the unshifted box.
+    const int ib = min(ib0, P - BX), jb = min(jb0, P - BY);
     float cx2[BX], cux[BX], rt2[BY], ruy[BY];
+    {
+        float dx = fmaf(-(float)ib, p.finter_x, c.dx0);
 #pragma unroll
-    for (int k = 0; k < BX; k++) {
-        col_terms(voxel_offset(c.dx0, ib0 + k, p.finter_x), ev.ux, cx2[k], cux[k]);
-        cx2[k] = ib0 + k < P ? cx2[k] : INFINITY;
-    }
+        for (int k = 0; k < BX; k++) {
+            col_terms(dx, ev.ux, cx2[k], cux[k]);
+            dx -= p.finter_x;
+        }
+        float dy = fmaf(-(float)jb, p.finter_y, c.dy0);
 #pragma unroll
-    for (int k = 0; k < BY; k++) {
-        row_terms(voxel_offset(c.dy0, jb0 + k, p.finter_y), ev.uy, ev.thr, rt2[k], ruy[k]);
-        rt2[k] = jb0 + k < P ? rt2[k] : INFINITY;
+        for (int k = 0; k < BY; k++) {
+            row_terms(dy, ev.uy, ev.thr, rt2[k], ruy[k]);
+            dy -= p.finter_y;
+        }
     }
     const __amdgpu_buffer_rsrc_t rs =
         __builtin_amdgcn_make_buffer_rsrc((void *)p.img, (short)0, p.img_bytes, 0x00020000);
-    const unsigned o0 = 4u * (unsigned)(s * p.sz + (c.iy0 + jb0) * p.sy + c.ix0 + ib0);
+    // 24-bit products (full rate): slice and row indices are non-negative and
+    // a frame slice holds fewer than 2^24 voxels
+    const unsigned o0 =
+        4u * (__umul24((unsigned)s, (unsigned)p.sz) +
+              __umul24((unsigned)(c.iy0 + jb), (unsigned)p.sy) + (unsigned)(c.ix0 + ib));
     float v[NC];
 #pragma unroll
     for (int jj = 0; jj < BY; jj++) {
@@ -1051,9 +1069,10 @@ __device__ __forceinline__ float walk_event(const TorParams &p, const Event &ev,
         Cross c;
         cross_at(p, ev, t, s, c);
         // scatters: the scale rides in amp (one product per slice, not per
-        // voxel), with the TOF constant folded into it once
+        // voxel), with the TOF constant folded into it once; gathers: the TOF
+        // constant multiplies the walk's sum once, after the loop
         float amp = KIND == 1 ? (tof ? tof_exp(p, ev, t, s) * ts : scale)
-                              : (tof ? tof_factor(p, ev, t, s) : 1.0f);
+                              : (tof ? tof_exp(p, ev, t, s) : 1.0f);
         if (NB > 0 && shape)
             acc += slice_box_image<KIND, NBB, FRAME>(p, ev, t, c, amp, s, scale, shape);
         else if (p.full_patch)
@@ -1061,7 +1080,7 @@ __device__ __forceinline__ float walk_event(const TorParams &p, const Event &ev,
         else
             acc += patch_walk<KIND, false>(p, ev, t, c, amp, s, scale);
     }
-    return acc;
+    return KIND == 0 && tof ? acc * p.tofk : acc;
 }
 
 // Sum over the LPE consecutive lanes that share an event.  Every lane of the
