@@ -52,7 +52,7 @@ def parse():
                          "event's candidate box exceeds 4 voxels: the chord-walk kernels)")
     ap.add_argument("--autotune", action="store_true",
                     help="tune the projection schedule and the tile shape on a sample of the "
-                         "events before timing (default: the fixed defaults, tile 12x32x12, "
+                         "events before timing (default: the fixed defaults, tile 12x32x10, "
                          "chunk 2048; the tuner's candidates lie within a few percent of "
                          "each other, inside its timing noise)")
     ap.add_argument("--no-sort", dest="sort", action="store_false",

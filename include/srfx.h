@@ -272,6 +272,12 @@ int srfx_tor_plan_create_subsets(const float *lors, int64_t n, int64_t ld, int32
 /* Listings (event, tile pairs) and workgroup chunks of a plan. */
 int srfx_tor_plan_info(srfx_tor_plan plan, int64_t *nlistings, int32_t *nchunks);
 
+/* The candidate box of the plan's walks (3 or 4; 0: chord walks) and
+ * whether every event walks the NB x (NB - 1) box.  With box 3, such a plan
+ * scatters through the kernel that holds only that walk when its tile takes
+ * at most a fifth of a CU's LDS.  No reference counterpart. */
+int srfx_tor_plan_walks(srfx_tor_plan plan, int32_t *box, int32_t *single_walk);
+
 /* Synchronises the device, then frees the plan. */
 int srfx_tor_plan_destroy(srfx_tor_plan plan);
 

@@ -117,6 +117,7 @@ _PROTOS = {
                                      _I32, ctypes.POINTER(_I64), ctypes.POINTER(ctypes.c_void_p),
                                      _P],
     "srfx_tor_plan_info": [_P, ctypes.POINTER(_I64), ctypes.POINTER(_I32)],
+    "srfx_tor_plan_walks": [_P, ctypes.POINTER(_I32), ctypes.POINTER(_I32)],
     "srfx_tor_plan_destroy": [_P],
     "srfx_tor_plan_project": [_P, _P, _P, _U32, _P],
     "srfx_tor_plan_backproject": [_P, _P, _P, _U32, _P],

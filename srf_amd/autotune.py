@@ -24,8 +24,8 @@ LANES = (1, 4, 16)
 BLOCKS = (64, 128, 256)
 SCHEDULES = tuple((l, b) for l in LANES for b in BLOCKS)
 # Tile plans: (slices, wx, wy) and events per workgroup chunk.
-TILES = ((12, 32, 12), (8, 32, 20), (16, 32, 10), (8, 32, 16), (8, 16, 32), (8, 20, 32),
-         (10, 32, 16), (4, 32, 32), (16, 16, 16), (8, 32, 32))
+TILES = ((12, 32, 10), (12, 32, 12), (8, 32, 20), (16, 32, 10), (8, 32, 16), (8, 16, 32),
+         (8, 20, 32), (10, 32, 16), (4, 32, 32), (16, 16, 16), (8, 32, 32))
 CHUNKS = (1024, 2048, 4096)
 
 _cache = {}

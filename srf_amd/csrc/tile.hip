@@ -62,6 +62,7 @@ unsigned lds_bytes_scatter(const TileGeom &g)
 
 // Largest tile count whose per-workgroup counters (3 per tile) fit in 64 KB.
 constexpr int kLdsTiles = 16384 / 3;
+constexpr unsigned kLdsPerCU = 160u * 1024u; // LDS of one CU
 
 template <bool FILL>
 void launch_pass(const TorParams &p, const TileGeom &g, const PlanArgs &a, hipStream_t st)
@@ -147,22 +148,24 @@ int plan_build(srfx_tor_plan_s *pl, hipStream_t st, int chunk, bool keep_slots)
     std::vector<TileChunk> ch;
     int64_t total = 0;
     size_t tmp_bytes = 0;
-    int last_cnt = 0, last_off = 0, mismatch = 0, total32 = 0;
+    int last_cnt = 0, last_off = 0, mismatch = 0, total32 = 0, other = 0;
     PlanArgs a{};
-    // cnt_d: tile counts [ntiles], tile fill [ntiles], error count [1]
-    if ((rc = dev_alloc(&cnt_d, 2 * (size_t)ntiles + 1)) ||
+    // cnt_d: tile counts [ntiles], tile fill [ntiles], error count [1], events
+    // that need another walk than the NB x (NB - 1) box [1]
+    if ((rc = dev_alloc(&cnt_d, 2 * (size_t)ntiles + 2)) ||
         (rc = dev_alloc(&pl->rec, (size_t)kWalkF4 * n)) ||
         (rc = dev_alloc(&tile_off_d, ntiles)) || (rc = dev_alloc(&pl->proj, n)))
         goto done;
     if (keep_slots && ((rc = dev_alloc(&ev_cnt, n)) || (rc = dev_alloc(&pl->ev_off, n + 1))))
         goto done;
-    hipLaunchKernelGGL(fill_int, dim3(blocks_for(2 * (int64_t)ntiles + 1, 256)), dim3(256), 0,
-                       st, cnt_d, 2 * (int64_t)ntiles + 1, 0);
+    hipLaunchKernelGGL(fill_int, dim3(blocks_for(2 * (int64_t)ntiles + 2, 256)), dim3(256), 0,
+                       st, cnt_d, 2 * (int64_t)ntiles + 2, 0);
     a.cols7 = pl->cols7;
     a.ntiles = ntiles;
     a.tile_cnt = cnt_d;
     a.ev_cnt = ev_cnt;
     a.rec = pl->rec;
+    a.other = cnt_d + 2 * ntiles + 1;
     if (n > 0)
         launch_pass<false>(p, g, a, st);
     if ((rc = launch_status()))
@@ -188,10 +191,13 @@ int plan_build(srfx_tor_plan_s *pl, hipStream_t st, int chunk, bool keep_slots)
     }
     if (hipMemcpyAsync(cnt.data(), cnt_d, sizeof(int) * ntiles, hipMemcpyDeviceToHost, st) !=
             hipSuccess ||
+        hipMemcpyAsync(&other, cnt_d + 2 * ntiles + 1, sizeof(int), hipMemcpyDeviceToHost, st) !=
+            hipSuccess ||
         hipStreamSynchronize(st) != hipSuccess) {
         rc = SRFX_ELAUNCH;
         goto done;
     }
+    pl->one = p.box > 0 && !p.full_patch && other == 0;
     for (int t = 0; t < ntiles; t++) {
         off[t] = (int)total;
         total += cnt[t];
@@ -351,16 +357,20 @@ int plan_build_sets(const std::vector<PlanSet> &sets, hipStream_t st)
         l.ctr = ctr_n;
         ctr_n += (size_t)3 * ps.K * l.nt;
     }
-    // temporaries: the sets' heads, their counters, one error count per set
+    // temporaries: the sets' heads, their counters, one error count per set,
+    // per plan the events that need another walk than the NB x (NB - 1) box
     const size_t ctr_b = align256(sizeof(int) * std::max<size_t>(ctr_n, 1));
-    const size_t tmp_b = head_b + ctr_b + sizeof(int) * S;
+    const size_t tmp_b = head_b + ctr_b + sizeof(int) * (S + nplans);
     char *tmp = nullptr, *blk0 = nullptr, *blk1 = nullptr;
     std::vector<char> head(head_b);
     std::vector<int> ctr(ctr_n);
-    std::vector<int> errs(S, 0);
+    std::vector<int> errs(S, 0), others(nplans, 0);
     std::vector<TileChunk> ch;
     int rc = SRFX_OK;
-    int *ctr_d = nullptr, *err_d = nullptr;
+    int *ctr_d = nullptr, *err_d = nullptr, *other_d = nullptr;
+    std::vector<int> plan0(S + 1, 0); // first plan of each set
+    for (int s = 0; s < S; s++)
+        plan0[s + 1] = plan0[s] + sets[s].K;
     auto args = [&](int s) {
         const PlanSet &ps = sets[s];
         const Layout &l = L[s];
@@ -377,6 +387,7 @@ int plan_build_sets(const std::vector<PlanSet> &sets, hipStream_t st)
         a.fill = a.cnt + (size_t)ps.K * l.nt;
         a.off = a.fill + (size_t)ps.K * l.nt;
         a.err = err_d + s;
+        a.other = other_d + plan0[s];
         return a;
     };
     auto launch = [&](bool fill) {
@@ -403,6 +414,7 @@ int plan_build_sets(const std::vector<PlanSet> &sets, hipStream_t st)
     arena->mem[0] = blk0;
     ctr_d = (int *)(tmp + head_b);
     err_d = (int *)(tmp + head_b + ctr_b);
+    other_d = err_d + S;
     for (int s = 0, j = 0; s < S; s++) {
         const PlanSet &ps = sets[s];
         char *h = head.data() + L[s].head;
@@ -415,8 +427,8 @@ int plan_build_sets(const std::vector<PlanSet> &sets, hipStream_t st)
         }
     }
     {
-        // the counters and the error counts behind them
-        const int64_t nz = (int64_t)(ctr_b / sizeof(int) + S);
+        // the counters, the error counts and the other-walk counts behind them
+        const int64_t nz = (int64_t)(ctr_b / sizeof(int) + S + nplans);
         hipLaunchKernelGGL(fill_int, dim3(blocks_for(nz, 256)), dim3(256), 0, st, ctr_d, nz, 0);
     }
     if (hipMemcpyAsync(tmp, head.data(), head_b, hipMemcpyHostToDevice, st) != hipSuccess) {
@@ -427,10 +439,17 @@ int plan_build_sets(const std::vector<PlanSet> &sets, hipStream_t st)
         goto done;
     if ((ctr_n > 0 && hipMemcpyAsync(ctr.data(), ctr_d, sizeof(int) * ctr_n,
                                      hipMemcpyDeviceToHost, st) != hipSuccess) ||
+        hipMemcpyAsync(others.data(), other_d, sizeof(int) * nplans, hipMemcpyDeviceToHost,
+                       st) != hipSuccess ||
         hipStreamSynchronize(st) != hipSuccess) {
         rc = SRFX_ELAUNCH;
         goto done;
     }
+    for (int s = 0; s < S; s++)
+        for (int k = 0; k < sets[s].K; k++) {
+            srfx_tor_plan_s *pl = sets[s].pls[k];
+            pl->one = pl->p.box > 0 && !pl->p.full_patch && others[plan0[s] + k] == 0;
+        }
     // per plan: slot offsets per tile (into the counters' off part), then
     // chunks window by window, each window's slabs in order (an event's
     // listings in consecutive slabs mostly share a window, so its record is
@@ -551,6 +570,15 @@ void launch_scatter_l(const TorParams &p, const srfx_tor_plan_s *pl, dim3 grid, 
 {
     switch (p.box) {
     case 3:
+        // plans whose events all walk the 3 x 2 box, with tiles that let five
+        // workgroups share a CU's LDS: the single-walk kernel
+        if constexpr (YF) {
+            if (pl->one && 5u * sh <= kLdsPerCU) {
+                hipLaunchKernelGGL((tile_scatter<MODE, 3, VREC, true, true>), grid, block, sh, st,
+                                   p, pl->g, pl->chunks, pl->list, pl->rec);
+                break;
+            }
+        }
         hipLaunchKernelGGL((tile_scatter<MODE, 3, VREC, YF>), grid, block, sh, st, p, pl->g,
                            pl->chunks, pl->list, pl->rec);
         break;
@@ -801,6 +829,17 @@ extern "C" int srfx_tor_plan_info(srfx_tor_plan plan, int64_t *nlistings, int32_
         *nlistings = plan->nlist;
     if (nchunks)
         *nchunks = plan->nchunks;
+    return SRFX_OK;
+}
+
+extern "C" int srfx_tor_plan_walks(srfx_tor_plan plan, int32_t *box, int32_t *single_walk)
+{
+    if (!plan)
+        return SRFX_EINVAL;
+    if (box)
+        *box = plan->p.full_patch ? 0 : plan->p.box;
+    if (single_walk)
+        *single_walk = plan->one;
     return SRFX_OK;
 }
 

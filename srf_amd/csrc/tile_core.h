@@ -242,7 +242,17 @@ struct PlanArgs {
     int *list;            // fill pass: event of each slot
     int *ev_slot;         // fill pass: slot of each of an event's listings (nullable)
     int *err;             // fill pass: count of listings that did not fit
+    int *other;           // count pass: events that need another walk than the
+                          // NB x (NB - 1) box (nullable)
 };
+
+// Whether the walks of an event take the NB x (NB - 1) candidate box (box
+// shape 1) in every slice: then a plan's scatter can run the kernel that
+// holds only that walk.
+__device__ __forceinline__ bool shape_one(const TorParams &p, const Track &t)
+{
+    return !p.full_patch && p.box > 0 && t.nx <= p.box && t.ny <= p.box - 1;
+}
 
 __device__ __forceinline__ bool plan_init(const TorParams &p, int cols7, const RawEvent &r,
                                           const ExactGeom &eg, Event &ev)
@@ -295,6 +305,8 @@ __global__ __launch_bounds__(256) void plan_pass(TorParams p, TileGeom g, PlanAr
     }
     if (!FILL) {
         int k = 0;
+        if (ok && a.other && p.box > 0 && !shape_one(p, t))
+            atomicAdd(a.other, 1);
         if (ok)
             for_each_tile(p, g, ev, t, [&](int tile, int, int, int) {
                 atomicAdd(LDS ? &sh[tile] : &a.tile_cnt[tile], 1);
@@ -377,6 +389,8 @@ struct PlanSetArgs {
     int *fill;              // fill pass: slots taken per (plan, tile)
     int *const *list;       // fill pass: [nplans] list entries
     int *err;               // fill pass: listings that did not fit
+    int *other;             // count pass: per plan, events that need another walk
+                            // than the NB x (NB - 1) box
 };
 
 // The plan holding event e of the set's array.
@@ -440,6 +454,8 @@ __global__ __launch_bounds__(256) void plan_set_pass(TorParams p, TileGeom g, Pl
             if (ok)
                 make_track(q, ev, eg, r, t);
             pack_walk(q, r, ev, t, ok, a.rec[k], le);
+            if (ok && q.box > 0 && !shape_one(q, t))
+                atomicAdd(&a.other[k], 1);
         }
     }
     if (!FILL || lds) {
@@ -776,7 +792,9 @@ __global__ __launch_bounds__(256, NB == 3 && FRAME && LPE <= 4 ? 5 : 4) void rec
 // calls it; it synchronises the workgroup before and after the flush.
 // VREC (kBack only): the backprojection scales 1/p that the record projection
 // stored in the event records (the OSEM step of a plan), not p.val.
-template <int MODE, int NB, bool VREC = false, bool YF = false>
+// ONE: every listing's event walks the NB x (NB - 1) box (a plan whose
+// events all have box shape 1); the other walks are left out of the kernel.
+template <int MODE, int NB, bool VREC = false, bool YF = false, bool ONE = false>
 __device__ __forceinline__ void scatter_chunk(const TorParams &p, const TileGeom &g,
                                               const TileChunk &ch,
                                               const int *__restrict__ list,
@@ -829,7 +847,8 @@ __device__ __forceinline__ void scatter_chunk(const TorParams &p, const TileGeom
             constexpr int NBB = NB > 1 ? NB : 2;
             constexpr bool POS = MODE == kBack; // scale 1/v with v > 1e-7
             const int shape = box_shape<NB>(t);
-            if (YF && NB > 0 && shape == 1)
+            static_assert(!ONE || (YF && NB > 0), "the single walk is the y-fastest box walk");
+            if (ONE || (YF && NB > 0 && shape == 1))
                 scatter_box_yf<NBB, NBB - 1, POS>(p, g, b, ev, t, tof, scale, s0, s1, ldsd);
             else if (YF && NB > 0 && shape == 2)
                 scatter_box_yf<NBB, NBB, POS>(p, g, b, ev, t, tof, scale, s0, s1, ldsd);
@@ -943,15 +962,17 @@ __global__ __launch_bounds__(256) void sort_chunks(TorParams p, TileGeom g,
 // at several lanes per cycle where FP32 ones are serialised lane by lane,
 // and the tile sums gain precision.  Four waves per SIMD where the walk fits
 // in 128 VGPRs without spilling (the LDS atomics need the latency cover);
-// the 4 x 4 box walk needs more registers and runs three.
-template <int MODE, int NB, bool VREC, bool YF>
-__global__ __launch_bounds__(256, NB == 4 ? 3 : 4) void tile_scatter(TorParams p, TileGeom g,
-                                                    const TileChunk *__restrict__ chunks,
-                                                    const int *__restrict__ list,
-                                                    const float4 *__restrict__ rec)
+// the 4 x 4 box walk needs more registers and runs three.  ONE (only the
+// NB x (NB - 1) box walk, NB = 3): five waves per SIMD in 96 VGPRs, for
+// tiles of at most a fifth of a CU's LDS (0.39 against 0.42 ms per 1M
+// events).
+template <int MODE, int NB, bool VREC, bool YF, bool ONE = false>
+__global__ __launch_bounds__(256, ONE ? 5 : (NB == 4 ? 3 : 4)) void tile_scatter(
+    TorParams p, TileGeom g, const TileChunk *__restrict__ chunks, const int *__restrict__ list,
+    const float4 *__restrict__ rec)
 {
     extern __shared__ double ldsd[];
-    scatter_chunk<MODE, NB, VREC, YF>(p, g, chunks[blockIdx.x], list, rec, ldsd);
+    scatter_chunk<MODE, NB, VREC, YF, ONE>(p, g, chunks[blockIdx.x], list, rec, ldsd);
 }
 
 // Gather of one chunk: partial[slot] = the listing's projection over the tile.
@@ -1048,4 +1069,5 @@ struct srfx_tor_plan_s {
         std::atomic<int> refs;
     } *arena;
     std::vector<srfx::TileChunk> hchunks; // host copy of the chunk table
+    int one;               // every event walks the NB x (NB - 1) box (box shape 1)
 };

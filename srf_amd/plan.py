@@ -24,7 +24,7 @@ import torch
 from . import _lib
 from .ops import _check_dev, _geom, _ptr, _soa, _stream, _vec
 
-DEFAULT_TILE = (12, 32, 12)
+DEFAULT_TILE = (12, 32, 10)
 DEFAULT_CHUNK = 2048
 
 
@@ -94,6 +94,14 @@ class TorPlan:
         a, b = ctypes.c_int64(), ctypes.c_int32()
         _lib.call("srfx_tor_plan_info", self._h, ctypes.byref(a), ctypes.byref(b))
         return int(a.value), int(b.value)
+
+    def walks(self):
+        """(candidate box 3 or 4, or 0 for chord walks; whether every event
+        walks the NB x (NB - 1) box, so that the scatter can run its
+        single-walk kernel)."""
+        a, b = ctypes.c_int32(), ctypes.c_int32()
+        _lib.call("srfx_tor_plan_walks", self._h, ctypes.byref(a), ctypes.byref(b))
+        return int(a.value), bool(b.value)
 
     def device_bytes(self):
         """Device memory held by the plan: 4 bytes per listing (its event),

@@ -89,6 +89,8 @@ def test_invalid_arguments_are_rejected():
     # frame copy: in place, null output
     assert lib.srfx_frame_copy(fake, _grid(), fake, None) == _lib.SRFX_EINVAL
     assert lib.srfx_frame_copy(fake, _grid(), None, None) == _lib.SRFX_EINVAL
+    # plan queries: null plan
+    assert lib.srfx_tor_plan_walks(None, None, None) == _lib.SRFX_EINVAL
     assert lib.srfx_strerror(_lib.SRFX_EINVAL) == b"invalid argument"
 
 

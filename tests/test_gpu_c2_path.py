@@ -5,7 +5,8 @@ kernel_width 4 mm, and a seeded 150k-event sample of the 20M-event workload
 benchmark does.  The path is the benchmark's exactly: per axis group the
 op-frame copy of the image, the record projection (4 lanes per event,
 frame gathers, record_proj_kernel) and the tiled scatter of 1/p (default
-tile 8 x 32 x 20, chunk 2048, FP64 LDS tile, world-layout flush).
+tile 12 x 32 x 10, chunk 2048, FP64 LDS tile, world-layout flush; every
+event walks the 3 x 2 box, so the scatter runs its single-walk kernel).
 
 * projection (read back from the scales the projection stores, 1/p) and
   backprojection of one OSEM step: rel L2 <= 1e-5 against the FP64 oracle;
@@ -30,7 +31,7 @@ def workload(cuda):
     from srf_amd.data import split_from_groups
     from srf_amd.plan import DEFAULT_CHUNK, DEFAULT_TILE
     from srf_amd.scanner import CylindricalScanner
-    assert DEFAULT_TILE == (12, 32, 12) and DEFAULT_CHUNK == 2048
+    assert DEFAULT_TILE == (12, 32, 10) and DEFAULT_CHUNK == 2048
     sc = CylindricalScanner()
     _, ev, meta = synth.c2(n_events=N_EVENTS, seed=2, scanner=sc, device=cuda)
     groups = preprocess.recon_process(ev, sc.tof_res_ps)
@@ -56,6 +57,7 @@ def test_osem_step_vs_oracle(cuda, workload):
     x = torch.from_numpy(x_np).to(cuda)
     image = Image(x, center, size)
     plans = model.make_plans(image, data)
+    assert all(plans[a].walks() == (3, True) for a in "xy")
     out = torch.zeros_like(x)
     model.osem_subset(image, data, out=out, plans=plans)
     bp = out.cpu().numpy()
@@ -86,6 +88,34 @@ def test_osem_step_vs_oracle(cuda, workload):
           f"OSEM-step backprojection rel_l2 {e_bp:.2e} (FP32 port {e_bp32:.2e})")
     assert e_proj <= 1e-5
     assert e_bp <= 1e-5
+
+
+def test_single_walk_scatter_matches_general(cuda, workload):
+    """The single-walk scatter kernel (tile 12 x 32 x 10: five workgroups per
+    CU) and the general one (tile 12 x 32 x 12, whose FP64 LDS tile admits
+    four) give the same OSEM-step backprojection."""
+    from srf_amd.data import Image
+    model, groups, gnp, meta, data = workload
+    grid, center, size = meta["grid"], meta["center"], meta["size"]
+    x = torch.from_numpy(H.image(tuple(grid)[::-1], seed=23).reshape(grid)).to(cuda)
+    image = Image(x, center, size)
+    outs = []
+    try:
+        for tile in ((12, 32, 10), (12, 32, 12)):
+            model.tile = tile
+            plans = model.make_plans(image, data)
+            assert all(plans[a].walks() == (3, True) for a in "xy")
+            out = torch.zeros_like(x)
+            model.osem_subset(image, data, out=out, plans=plans)
+            outs.append(out.cpu().numpy())
+            for p in plans.values():
+                if p is not None:
+                    p.close()
+    finally:
+        model.tile = None
+    e = H.rel_l2(outs[0], outs[1])
+    print(f"single-walk vs general scatter kernel: rel_l2 {e:.2e}")
+    assert e <= 1e-6
 
 
 def test_osem_2x2_vs_oracle(cuda, workload):
