@@ -4,7 +4,7 @@
 // At creation every (reconstruction, axis group, subset) gets a tile plan of
 // its events (tile_core.h).  A subset step is four kernels on one stream:
 //   batch_zero     zero every reconstruction's backprojection buffer and the
-//                  two work counters;
+//                  work counters (two per box class);
 //   batch_frames   copy every reconstruction's image into the op frame of
 //                  each of its axis groups (frame_core.h), which the
 //                  projection reads (a box row of voxels shares a line);
@@ -42,14 +42,18 @@ struct EvChunk {
     int job, start, count, pad;
 };
 
-// Box classes of the batch's jobs: 0 = candidate box 3, 1 = box 4, 2 = chord
-// walk.  Each class has its own work lists and persistent kernels, so that a
-// kernel holds only its class's walk (and its registers).
-constexpr int kBoxClasses = 3;
+// Box classes of the batch's jobs: 0 = candidate box 3 with every event on
+// the 3 x 2 box and a y-fastest flush (the single-walk scatter), 1 = other
+// box-3 jobs, 2 = box 4, 3 = chord walk.  Each class has its own work lists
+// and persistent kernels, so that a kernel holds only its class's walk (and
+// its registers).
+constexpr int kBoxClasses = 4;
 
-inline int box_class(int box)
+inline int box_class(const TorParams &p, bool single_walk)
 {
-    return box == 3 ? 0 : (box == 4 ? 1 : 2);
+    if (p.box == 3)
+        return single_walk && flush_y_fastest(p) ? 0 : 1;
+    return p.box == 4 ? 2 : 3;
 }
 
 struct TileWork {
@@ -159,9 +163,9 @@ __global__ __launch_bounds__(256) void batch_project(const BatchJob *__restrict_
 }
 
 // Blocks pull tile chunks of one box class (NB); four waves per SIMD where
-// the walk fits (as tile_scatter).
-template <int NB>
-__global__ __launch_bounds__(256, NB == 4 ? 3 : 4) void batch_scatter(
+// the walk fits, five for the single-walk class (ONE), as tile_scatter.
+template <int NB, bool ONE = false>
+__global__ __launch_bounds__(256, ONE ? 5 : (NB == 4 ? 3 : 4)) void batch_scatter(
     const BatchJob *__restrict__ jobs, const TileWork *__restrict__ work, int nwork,
     unsigned *counter)
 {
@@ -173,7 +177,9 @@ __global__ __launch_bounds__(256, NB == 4 ? 3 : 4) void batch_scatter(
         const TileWork w = work[c];
         const BatchJob &job = jobs[__builtin_amdgcn_readfirstlane(w.job)];
         const TileChunk ch{w.tile, w.start, w.count, 0};
-        if (flush_y_fastest(job.p))
+        if constexpr (ONE)
+            scatter_chunk<kBack, NB, true, true, true>(job.p, job.g, ch, job.list, job.rec, ldsd);
+        else if (flush_y_fastest(job.p))
             scatter_chunk<kBack, NB, true, true>(job.p, job.g, ch, job.list, job.rec, ldsd);
         else
             scatter_chunk<kBack, NB, true, false>(job.p, job.g, ch, job.list, job.rec, ldsd);
@@ -230,9 +236,9 @@ srfx_grid group_grid(const srfx_recon &r, int a)
     return g;
 }
 
-// Tile of the batch's plans: 12 slices x 32 x 12 voxels, <= 2048 events per
-// workgroup (the FP64 LDS tile is 36 KB: four workgroups per CU).
-constexpr srfx_tile kBatchTile = {12, 32, 12, 2048, 256, 0};
+// Tile of the batch's plans: 12 slices x 32 x 10 voxels, <= 2048 events per
+// workgroup (the FP64 LDS tile is 30 KB: five workgroups per CU).
+constexpr srfx_tile kBatchTile = {12, 32, 10, 2048, 256, 0};
 
 } // namespace
 } // namespace srfx
@@ -289,9 +295,11 @@ static int project_launch(srfx_tor_batch *b, int subset, hipStream_t st)
 {
     int rc = project_class<3, LPE, T>(b, 0, subset, st);
     if (!rc)
-        rc = project_class<4, LPE, T>(b, 1, subset, st);
+        rc = project_class<3, LPE, T>(b, 1, subset, st);
     if (!rc)
-        rc = project_class<0, LPE, T>(b, 2, subset, st);
+        rc = project_class<4, LPE, T>(b, 2, subset, st);
+    if (!rc)
+        rc = project_class<0, LPE, T>(b, 3, subset, st);
     return rc;
 }
 
@@ -468,7 +476,7 @@ extern "C" int srfx_tor_batch_create(const srfx_recon *recons, int32_t nrecon,
                 job.p.sy = gg.shape[0];
                 job.p.sz = gg.shape[0] * gg.shape[1];
                 job.p.img_bytes = (int)(4 * (int64_t)job.p.sz * gg.shape[2]);
-                const int bc = box_class(plan->p.box);
+                const int bc = box_class(job.p, plan->one != 0);
                 job.p.out = rc.bp;
                 job.p.val = plan->proj;
                 job.g = plan->g;
@@ -517,11 +525,13 @@ extern "C" int srfx_tor_batch_create(const srfx_recon *recons, int32_t nrecon,
     int dev = 0, cus = 0, per_cu_s[kBoxClasses] = {};
     ok = ok && hipGetDevice(&dev) == hipSuccess &&
          hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
-         hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu_s[0], batch_scatter<3>, 256,
+         hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu_s[0], batch_scatter<3, true>, 256,
                                                       b->scatter_lds) == hipSuccess &&
-         hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu_s[1], batch_scatter<4>, 256,
+         hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu_s[1], batch_scatter<3>, 256,
                                                       b->scatter_lds) == hipSuccess &&
-         hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu_s[2], batch_scatter<0>, 256,
+         hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu_s[2], batch_scatter<4>, 256,
+                                                      b->scatter_lds) == hipSuccess &&
+         hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu_s[3], batch_scatter<0>, 256,
                                                       b->scatter_lds) == hipSuccess;
     b->cus = cus;
     ok = ok && project_schedule(b, b->proj_lanes, b->proj_events, -1, nullptr) == SRFX_OK;
@@ -552,10 +562,14 @@ extern "C" int srfx_tor_batch_subset(srfx_tor_batch *b, int32_t subset, void *st
         if (b->nwork[w] == 0)
             continue;
         if (c == 0)
-            hipLaunchKernelGGL(batch_scatter<3>, dim3(b->scatter_blocks[c]), dim3(256),
+            hipLaunchKernelGGL((batch_scatter<3, true>), dim3(b->scatter_blocks[c]), dim3(256),
                                b->scatter_lds, st, b->d_jobs, b->d_work[w], b->nwork[w],
                                b->d_counters + 2 * c + 1);
         else if (c == 1)
+            hipLaunchKernelGGL(batch_scatter<3>, dim3(b->scatter_blocks[c]), dim3(256),
+                               b->scatter_lds, st, b->d_jobs, b->d_work[w], b->nwork[w],
+                               b->d_counters + 2 * c + 1);
+        else if (c == 2)
             hipLaunchKernelGGL(batch_scatter<4>, dim3(b->scatter_blocks[c]), dim3(256),
                                b->scatter_lds, st, b->d_jobs, b->d_work[w], b->nwork[w],
                                b->d_counters + 2 * c + 1);
