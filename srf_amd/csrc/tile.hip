@@ -150,8 +150,8 @@ int plan_build(srfx_tor_plan_s *pl, hipStream_t st, int chunk, bool keep_slots)
     size_t tmp_bytes = 0;
     int last_cnt = 0, last_off = 0, mismatch = 0, total32 = 0, other = 0;
     PlanArgs a{};
-    // cnt_d: tile counts [ntiles], tile fill [ntiles], error count [1], events
-    // that need another walk than the NB x (NB - 1) box [1]
+    // cnt_d: tile counts [ntiles], tile fill [ntiles], error count [1], the
+    // flag of an event that needs another walk than the NB x (NB - 1) box [1]
     if ((rc = dev_alloc(&cnt_d, 2 * (size_t)ntiles + 2)) ||
         (rc = dev_alloc(&pl->rec, (size_t)kWalkF4 * n)) ||
         (rc = dev_alloc(&tile_off_d, ntiles)) || (rc = dev_alloc(&pl->proj, n)))
@@ -358,7 +358,8 @@ int plan_build_sets(const std::vector<PlanSet> &sets, hipStream_t st)
         ctr_n += (size_t)3 * ps.K * l.nt;
     }
     // temporaries: the sets' heads, their counters, one error count per set,
-    // per plan the events that need another walk than the NB x (NB - 1) box
+    // per plan the flag of an event that needs another walk than the
+    // NB x (NB - 1) box
     const size_t ctr_b = align256(sizeof(int) * std::max<size_t>(ctr_n, 1));
     const size_t tmp_b = head_b + ctr_b + sizeof(int) * (S + nplans);
     char *tmp = nullptr, *blk0 = nullptr, *blk1 = nullptr;
@@ -378,7 +379,7 @@ int plan_build_sets(const std::vector<PlanSet> &sets, hipStream_t st)
         a.cols7 = ps.pls[0]->cols7;
         a.ntiles = l.nt;
         a.nplans = ps.K;
-        a.lds = 6u * sizeof(int) * l.nt <= 65536u;
+        a.lds = 3u * sizeof(int) * l.nt <= 65536u;
         a.first = (const int64_t *)(tmp + l.head);
         a.rec = (float4 *const *)(tmp + l.head + sizeof(int64_t) * (ps.K + 1));
         a.list = (int *const *)(tmp + l.head + sizeof(int64_t) * (ps.K + 1) +
@@ -399,7 +400,7 @@ int plan_build_sets(const std::vector<PlanSet> &sets, hipStream_t st)
             const PlanSetArgs a = args(s);
             TorParams p = ps.pls[0]->p;
             p.lors = ps.lors;
-            const unsigned lds = a.lds ? (fill ? 6u : 2u) * sizeof(int) * L[s].nt : 0u;
+            const unsigned lds = a.lds ? (fill ? 3u : 1u) * sizeof(int) * L[s].nt : 0u;
             if (fill)
                 hipLaunchKernelGGL(plan_set_pass<true>, dim3(blocks_for(n, 256)), dim3(256), lds,
                                    st, p, L[s].g, a);
@@ -427,7 +428,7 @@ int plan_build_sets(const std::vector<PlanSet> &sets, hipStream_t st)
         }
     }
     {
-        // the counters, the error counts and the other-walk counts behind them
+        // the counters, the error counts and the other-walk flags behind them
         const int64_t nz = (int64_t)(ctr_b / sizeof(int) + S + nplans);
         hipLaunchKernelGGL(fill_int, dim3(blocks_for(nz, 256)), dim3(256), 0, st, ctr_d, nz, 0);
     }
@@ -583,6 +584,14 @@ void launch_scatter_l(const TorParams &p, const srfx_tor_plan_s *pl, dim3 grid, 
                            pl->chunks, pl->list, pl->rec);
         break;
     case 4:
+        // the 4 x 3 box walk alone: four waves per SIMD instead of three
+        if constexpr (YF) {
+            if (pl->one && 4u * sh <= kLdsPerCU) {
+                hipLaunchKernelGGL((tile_scatter<MODE, 4, VREC, true, true>), grid, block, sh, st,
+                                   p, pl->g, pl->chunks, pl->list, pl->rec);
+                break;
+            }
+        }
         hipLaunchKernelGGL((tile_scatter<MODE, 4, VREC, YF>), grid, block, sh, st, p, pl->g,
                            pl->chunks, pl->list, pl->rec);
         break;

@@ -242,8 +242,8 @@ struct PlanArgs {
     int *list;            // fill pass: event of each slot
     int *ev_slot;         // fill pass: slot of each of an event's listings (nullable)
     int *err;             // fill pass: count of listings that did not fit
-    int *other;           // count pass: events that need another walk than the
-                          // NB x (NB - 1) box (nullable)
+    int *other;           // count pass: set to 1 if an event needs another walk than
+                          // the NB x (NB - 1) box (nullable)
 };
 
 // Whether the walks of an event take the NB x (NB - 1) candidate box (box
@@ -252,6 +252,15 @@ struct PlanArgs {
 __device__ __forceinline__ bool shape_one(const TorParams &p, const Track &t)
 {
     return !p.full_patch && p.box > 0 && t.nx <= p.box && t.ny <= p.box - 1;
+}
+
+// Set *flag once an event needs another walk: a relaxed load first, so that
+// once the flag is visible the lanes only read it (one cached load per wave)
+// instead of all storing to one address.
+__device__ __forceinline__ void flag_other(int *flag)
+{
+    if (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0)
+        __hip_atomic_store(flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 __device__ __forceinline__ bool plan_init(const TorParams &p, int cols7, const RawEvent &r,
@@ -306,7 +315,7 @@ __global__ __launch_bounds__(256) void plan_pass(TorParams p, TileGeom g, PlanAr
     if (!FILL) {
         int k = 0;
         if (ok && a.other && p.box > 0 && !shape_one(p, t))
-            atomicAdd(a.other, 1);
+            flag_other(a.other);
         if (ok)
             for_each_tile(p, g, ev, t, [&](int tile, int, int, int) {
                 atomicAdd(LDS ? &sh[tile] : &a.tile_cnt[tile], 1);
@@ -381,7 +390,7 @@ __global__ __launch_bounds__(256) void plan_pass(TorParams p, TileGeom g, PlanAr
 // plan-major.
 struct PlanSetArgs {
     int cols7, ntiles, nplans;
-    int lds;                // the per-workgroup counters fit in LDS (2 plans)
+    int lds;                // the per-workgroup counters fit in LDS
     const int64_t *first;   // [nplans + 1]
     float4 *const *rec;     // [nplans] walk records
     int *cnt;               // count pass: listings per (plan, tile); fill pass: read
@@ -389,8 +398,8 @@ struct PlanSetArgs {
     int *fill;              // fill pass: slots taken per (plan, tile)
     int *const *list;       // fill pass: [nplans] list entries
     int *err;               // fill pass: listings that did not fit
-    int *other;             // count pass: per plan, events that need another walk
-                            // than the NB x (NB - 1) box
+    int *other;             // count pass: per plan, set to 1 if an event needs
+                            // another walk than the NB x (NB - 1) box
 };
 
 // The plan holding event e of the set's array.
@@ -408,14 +417,14 @@ __device__ __forceinline__ int plan_of(const PlanSetArgs &a, int64_t e)
 }
 
 // plan_pass for a plan set, over all events of the set in one launch.  A
-// workgroup whose events belong to at most two plans counts in LDS (two
-// per-plan slices of counters), as plan_pass does for one plan; otherwise
-// (tiny plans) it uses the global counters directly.  List entries carry the
-// listing's slices (the set's plans always have spans).
+// workgroup whose events all belong to one plan counts in LDS, as plan_pass
+// does for one plan; a workgroup across a plan boundary (one per boundary)
+// uses the global counters directly.  List entries carry the listing's
+// slices (the set's plans always have spans).
 template <bool FILL>
 __global__ __launch_bounds__(256) void plan_set_pass(TorParams p, TileGeom g, PlanSetArgs a)
 {
-    extern __shared__ int sh[]; // [2 plans][counts (FILL: + bases, ranks)][ntiles]
+    extern __shared__ int sh[]; // [counts (FILL: + bases, ranks)][ntiles]
     const int nt = a.ntiles;
     const int per = FILL ? 3 * nt : nt;
     const int64_t total = a.first[a.nplans];
@@ -423,9 +432,9 @@ __global__ __launch_bounds__(256) void plan_set_pass(TorParams p, TileGeom g, Pl
     const int64_t e = e0 + threadIdx.x;
     const bool valid = e < total;
     const int kf = plan_of(a, e0), kl = plan_of(a, min(e0 + (int64_t)blockDim.x, total) - 1);
-    const bool lds = a.lds && kl - kf <= 1;
+    const bool lds = a.lds && kl == kf;
     if (lds) {
-        for (int i = threadIdx.x; i < 2 * per; i += blockDim.x)
+        for (int i = threadIdx.x; i < per; i += blockDim.x)
             sh[i] = 0;
         __syncthreads();
     }
@@ -434,7 +443,7 @@ __global__ __launch_bounds__(256) void plan_set_pass(TorParams p, TileGeom g, Pl
     TorParams q = p; // plan k's events
     q.lors = p.lors + a.first[k];
     q.n = a.first[k + 1] - a.first[k];
-    int *lcnt = sh + (k - kf) * per;
+    int *lcnt = sh;
     int *gcnt = a.cnt + (int64_t)k * nt;
     Event ev{};
     Track t{};
@@ -455,7 +464,7 @@ __global__ __launch_bounds__(256) void plan_set_pass(TorParams p, TileGeom g, Pl
                 make_track(q, ev, eg, r, t);
             pack_walk(q, r, ev, t, ok, a.rec[k], le);
             if (ok && q.box > 0 && !shape_one(q, t))
-                atomicAdd(&a.other[k], 1);
+                flag_other(&a.other[k]);
         }
     }
     if (!FILL || lds) {
@@ -465,15 +474,13 @@ __global__ __launch_bounds__(256) void plan_set_pass(TorParams p, TileGeom g, Pl
             });
         if (lds) {
             __syncthreads();
-            for (int i = threadIdx.x; i < 2 * nt; i += blockDim.x) {
-                const int slot = i >= nt, tile = i - slot * nt;
-                const int c = sh[slot * per + tile];
-                if (c && kf + slot <= kl) {
+            for (int tile = threadIdx.x; tile < nt; tile += blockDim.x) {
+                const int c = sh[tile];
+                if (c) {
                     if (FILL)
-                        sh[slot * per + nt + tile] =
-                            atomicAdd(&a.fill[(int64_t)(kf + slot) * nt + tile], c);
+                        sh[nt + tile] = atomicAdd(&a.fill[(int64_t)kf * nt + tile], c);
                     else
-                        atomicAdd(&a.cnt[(int64_t)(kf + slot) * nt + tile], c);
+                        atomicAdd(&a.cnt[(int64_t)kf * nt + tile], c);
                 }
             }
             __syncthreads();
@@ -967,7 +974,7 @@ __global__ __launch_bounds__(256) void sort_chunks(TorParams p, TileGeom g,
 // tiles of at most a fifth of a CU's LDS (0.39 against 0.42 ms per 1M
 // events).
 template <int MODE, int NB, bool VREC, bool YF, bool ONE = false>
-__global__ __launch_bounds__(256, ONE ? 5 : (NB == 4 ? 3 : 4)) void tile_scatter(
+__global__ __launch_bounds__(256, NB == 4 ? (ONE ? 4 : 3) : (ONE ? 5 : 4)) void tile_scatter(
     TorParams p, TileGeom g, const TileChunk *__restrict__ chunks, const int *__restrict__ list,
     const float4 *__restrict__ rec)
 {

@@ -135,6 +135,44 @@ def test_plans(cuda, case, tile, chunk):
     plan.close()
 
 
+def test_single_walk_plans(cuda, case):
+    """Events whose ellipse spans at most 3 rows (nearly parallel to the slice
+    axis) all walk the 4 x 3 box: the plan's scatter then runs the kernel
+    holding only that walk (tile 12 x 32 x 10: four workgroups per CU fit its
+    LDS), and the general kernel where the tile is too large for that
+    (12 x 32 x 16).  Both against the FP64 oracle."""
+    from oracle import lib as O
+    from srf_amd.plan import TorPlan
+    lors, img = case
+    lors = lors.copy()
+    rng = np.random.default_rng(53)
+    n = lors.shape[1]
+    lors[3:5] = lors[0:2] + rng.uniform(-4.0, 4.0, (2, n)).astype(np.float32)
+    r = rng.uniform(0.2, 0.8, n).astype(np.float32)
+    lors[6:9] = lors[0:3] + r * (lors[3:6] - lors[0:3])
+    # the geometric sigma factor scales the tube's radius: at most 0.6 keeps
+    # every ellipse within 3 rows of 1.6 mm
+    lors[9] = rng.uniform(0.3, 0.6, n).astype(np.float32)
+    args = _args(True, True)
+    L, X = _t(lors, cuda), _t(img, cuda)
+    ref = O.tor_project(lors, img, *args, prec="f64")
+    keep = ref > 1e-7
+    inv = np.where(keep, 1.0 / np.where(keep, ref, 1.0), 0.0)
+    rf = O.tor_backproject_adjoint(lors, inv.astype(np.float32), *args)
+    v = np.random.default_rng(54).uniform(0.5, 2.0, n).astype(np.float32)
+    rb = O.tor_backproject(lors, v, *args, prec="f64")
+    for tile in ((12, 32, 10), (12, 32, 16)):
+        plan = TorPlan(L, *GEO, H.KW, True, TOF_BIN, TOF_SIGMA2, tile=tile, chunk=1024)
+        assert plan.walks() == (4, True)
+        f = plan.osem_subset(X).cpu().numpy()
+        b = plan.backprojection(_t(v, cuda)).cpu().numpy()
+        plan.close()
+        print(f"box 4 single-walk plan {tile}: OSEM step rel_l2 {H.rel_l2(f, rf):.2e}, "
+              f"backprojection {H.rel_l2(b, rb):.2e}")
+        assert H.rel_l2(f, rf) <= TOL64
+        assert H.rel_l2(b, rb) <= TOL64
+
+
 def test_plan_maplors(cuda, case):
     from oracle import lib as O
     from srf_amd.plan import TorPlan
