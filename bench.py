@@ -338,11 +338,15 @@ def main():
         fr = physics.group_frame(grid, center, size, (1, 1, 1), a)
         box = _lib.tor_patch_info(fr[0], fr[2], meta["kernel_width"])[1]
         walk = (f"candidate-box walk, box {box}" if box else "chord walk (box wider than 4)")
+        # every event of the group's plans on the box-by-(box - 1) walk: the
+        # scatter runs its single-walk kernel
+        single = box and all(ps[a].walks()[1] for ps in osem.plans if ps.get(a) is not None)
+        swalk = f"{walk}, single-walk kernel" if single else walk
         for name, sel, kname in (("projection", lambda t: t[0].elapsed_time(t[1]),
                                   f"record_proj_kernel ({walk} from the walk records, values "
                                   "into their value slots)"),
                                  ("backprojection", lambda t: t[1].elapsed_time(t[2]),
-                                  f"tile_scatter<kBack> ({walk}; FP64 LDS tile accumulation, "
+                                  f"tile_scatter<kBack> ({swalk}; FP64 LDS tile accumulation, "
                                   "coalesced flush)")):
             ms = float(np.mean([sel(t) for t in timings[a]]))
             ach = byts / (ms / 1e3) / 1e9
