@@ -176,7 +176,7 @@ def main():
     import torch
     from srf_amd import _lib, effmap, physics, synth, preprocess
     from srf_amd.data import Image, split_from_groups
-    from srf_amd.plan import DEFAULT_CHUNK, DEFAULT_TILE
+    from srf_amd.plan import DEFAULT_CHUNK
     from srf_amd.recon import OSEM, subset_view
     from srf_amd.scanner import CylindricalScanner
 
@@ -413,7 +413,8 @@ def main():
                    "nnz_per_event": round(total_nnz / args.events, 2),
                    "path": _path_label(args, osem.tiled), "sorted": bool(args.sort),
                    "tiled": osem.tiled,
-                   "tile": list(model.tile or DEFAULT_TILE) if osem.tiled else None,
+                   "tile": (list(next(p.tile for ps in osem.plans for p in ps.values()
+                                      if p is not None)) if osem.tiled else None),
                    "chunk_default": None if model.chunk else DEFAULT_CHUNK,
                    "plan_build_s": round(plan_s, 3) if plan_s is not None else None,
                    "listings_per_event": (round(listings / args.events, 2)

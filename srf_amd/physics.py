@@ -131,7 +131,7 @@ class SplitLoRsModel:
         laid out like ``image`` (world (nx, ny, nz), any strides): the events
         binned into image tiles once, for the tiled backprojection /
         Maplors / OSEM step.  Returns {axis: TorPlan or None}."""
-        from .plan import DEFAULT_CHUNK, DEFAULT_TILE, TorPlan
+        from .plan import DEFAULT_CHUNK, TorPlan, default_tile
         x = _world(image)
         plans = {}
         for a in self.AXIS:
@@ -142,8 +142,8 @@ class SplitLoRsModel:
             g, c, sz, st = group_frame(x.shape, image.center, image.size, x.stride(), a)
             plans[a] = TorPlan(lm.soa, g, c, sz, self.kernel_width, self.geo_sigma2_flag,
                                self.tof_bin, self.tof_sigma2, strides=st, ncols=ncols,
-                               tile=self.tile or DEFAULT_TILE, chunk=self.chunk or DEFAULT_CHUNK,
-                               axis=a)
+                               tile=self.tile or default_tile(g, sz, self.kernel_width),
+                               chunk=self.chunk or DEFAULT_CHUNK, axis=a)
         return plans
 
     def make_subset_plans(self, image, data, nb_subsets, ncols=10):
@@ -151,7 +151,7 @@ class SplitLoRsModel:
         each group, recon.subset_bounds), each group's subsets built together
         (srfx_tor_plan_create_subsets).  Returns a list over subsets of
         {axis: TorPlan or None}."""
-        from .plan import DEFAULT_CHUNK, DEFAULT_TILE, TorPlan
+        from .plan import DEFAULT_CHUNK, TorPlan, default_tile
         from .recon import subset_bounds
         x = _world(image)
         K = int(nb_subsets)
@@ -167,7 +167,8 @@ class SplitLoRsModel:
             g, c, sz, st = group_frame(x.shape, image.center, image.size, x.stride(), a)
             plans = TorPlan.subsets(lm.soa, bounds, g, c, sz, self.kernel_width,
                                     self.geo_sigma2_flag, self.tof_bin, self.tof_sigma2,
-                                    strides=st, ncols=ncols, tile=self.tile or DEFAULT_TILE,
+                                    strides=st, ncols=ncols,
+                                    tile=self.tile or default_tile(g, sz, self.kernel_width),
                                     chunk=self.chunk or DEFAULT_CHUNK, axis=a)
             for k, pl in enumerate(plans):
                 if pl.n == 0:

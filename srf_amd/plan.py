@@ -25,6 +25,17 @@ from . import _lib
 from .ops import _check_dev, _geom, _ptr, _soa, _stream, _vec
 
 DEFAULT_TILE = (12, 32, 10)
+# Chord walks (events whose candidate box would be wider than 4 voxels) list
+# fewer (event, tile) pairs in deeper windows, and have no single-walk kernel
+# to fit five workgroups per CU.
+CHORD_TILE = (12, 32, 12)
+
+
+def default_tile(grid, size, kernel_width):
+    """The default tile of a plan over a frame of ``grid`` voxels spanning
+    ``size`` mm at ``kernel_width``: DEFAULT_TILE for the candidate-box
+    walks, CHORD_TILE for the chord walk."""
+    return DEFAULT_TILE if _lib.tor_patch_info(grid, size, kernel_width)[1] else CHORD_TILE
 DEFAULT_CHUNK = 2048
 
 

@@ -94,3 +94,15 @@ def test_patch_info_unsupported():
     with pytest.raises(_lib.SrfxError) as ei:
         _lib.tor_patch_info((40, 36, 24), (4.0, 3.6, 2.4), 40.0)  # P far above 16
     assert ei.value.status == _lib.SRFX_EUNSUPPORTED
+
+
+def test_default_tile_by_walk():
+    """Candidate-box walks take DEFAULT_TILE (the single-walk scatter fits
+    five workgroups per CU), chord walks the deeper CHORD_TILE."""
+    from srf_amd import _lib
+    from srf_amd.plan import CHORD_TILE, DEFAULT_TILE, default_tile
+    grid, size = (200, 100, 200), (400.0, 220.0, 400.0)  # the C2 y-group frame
+    assert _lib.tor_patch_info(grid, size, 4.0)[1] == 3
+    assert default_tile(grid, size, 4.0) == DEFAULT_TILE == (12, 32, 10)
+    assert _lib.tor_patch_info(grid, size, 8.0)[1] == 0
+    assert default_tile(grid, size, 8.0) == CHORD_TILE == (12, 32, 12)
