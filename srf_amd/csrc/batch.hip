@@ -137,11 +137,14 @@ __device__ __forceinline__ int next_wave_item(unsigned *counter)
 // block, so a wave with long walks holds up no other.  The default schedule,
 // LPE = 4 and T = 64 (quarter chunks), is the projection schedule of the plan
 // OSEM step.  The gathers read the jobs' contiguous op frames
-// (record_project<.., true>).
+// (record_project<.., true>).  Five waves per SIMD for the 3 x 3 box at one
+// lane per event and items of up to 128 events (a 12-byte spill outside the
+// walk; C4 iteration 88 against 93 ms); the other schedules measured slower
+// at five and keep four.
 template <int NB, int LPE, int T>
-__global__ __launch_bounds__(256) void batch_project(const BatchJob *__restrict__ jobs,
-                                                     const EvChunk *__restrict__ chunks,
-                                                     int nchunks, unsigned *counter)
+__global__ __launch_bounds__(256, NB == 3 && LPE == 1 && T <= 128 ? 5 : 4) void batch_project(
+    const BatchJob *__restrict__ jobs, const EvChunk *__restrict__ chunks, int nchunks,
+    unsigned *counter)
 {
     constexpr int kParts = kChunk / T, kRound = 64 / LPE;
     static_assert(kChunk % T == 0 && T % kRound == 0, "items split chunks into whole rounds");
