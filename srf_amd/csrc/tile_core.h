@@ -926,7 +926,7 @@ __global__ __launch_bounds__(256) void sort_chunks(TorParams p, TileGeom g,
     int key[kPer], ev[kPer];
 #pragma unroll
     for (int r = 0; r < kPer; r++) {
-        const int j = (int)threadIdx.x * kPer + r;
+        const int j = r * (int)blockDim.x + (int)threadIdx.x; // coalesced reads
         key[r] = -1;
         if (j < ch.count) {
             ev[r] = list[ch.start + j];
