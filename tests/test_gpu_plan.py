@@ -136,6 +136,46 @@ def test_plan_maplors(cuda, case):
     plan.close()
 
 
+def test_single_walk_modes(cuda, case):
+    """Events nearly parallel to the slice axis at kernel width 3.6 mm (a
+    tube radius of 1.8 mm, under one 2 mm row) all walk the 3 x 2 box, so a
+    plan of them (tile 12 x 32 x 10) scatters through the single-walk
+    kernel in every mode: the backprojection, its adjoint, Maplors and the
+    OSEM step, each against the FP64 oracle."""
+    from oracle import lib as O
+    from srf_amd.plan import TorPlan
+    lors, img = case
+    lors = lors.copy()
+    rng = np.random.default_rng(61)
+    n = lors.shape[1]
+    lors[3:5] = lors[0:2] + rng.uniform(-4.0, 4.0, (2, n)).astype(np.float32)
+    r = rng.uniform(0.2, 0.8, n).astype(np.float32)
+    lors[6:9] = lors[0:3] + r * (lors[3:6] - lors[0:3])
+    kw = 3.6
+    geo = (H.TOR_GRID, H.TOR_CENTER, H.TOR_SIZE)
+    args = (*geo, kw, False, H.TOF_BIN, H.TOF_SIGMA2)
+    L, X = _t(lors, cuda), _t(img, cuda)
+    plan = TorPlan(L, *geo, kw, False, H.TOF_BIN, H.TOF_SIGMA2, tile=(12, 32, 10), chunk=1024)
+    assert plan.walks() == (3, True)
+    v = rng.uniform(0.5, 2.0, n).astype(np.float32)
+    b = plan.backprojection(_t(v, cuda)).cpu().numpy()
+    assert H.rel_l2(b, O.tor_backproject(lors, v, *args, prec="f64")) <= TOL64
+    a = plan.backprojection_adjoint(_t(v - 1.0, cuda)).cpu().numpy()
+    assert H.rel_l2(a, O.tor_backproject_adjoint(lors, v - 1.0, *args)) <= TOL64
+    f = plan.osem_subset(X).cpu().numpy()
+    ref = O.tor_project(lors, img, *args, prec="f64")
+    keep = ref > 1e-7
+    inv = np.where(keep, 1.0 / np.where(keep, ref, 1.0), 0.0).astype(np.float32)
+    assert H.rel_l2(f, O.tor_backproject_adjoint(lors, inv, *args)) <= TOL64
+    plan.close()
+    l7 = np.ascontiguousarray(np.vstack([lors[0:6], lors[9:10]]))
+    plan = TorPlan(_t(l7, cuda), *geo, kw, False, ncols=7, tile=(12, 32, 10), chunk=1024)
+    assert plan.walks() == (3, True)
+    m = plan.maplors(_t(v, cuda)).cpu().numpy()
+    assert H.rel_l2(m, O.tor_map(l7, v, *geo, kw, False, prec="f64")) <= TOL64
+    plan.close()
+
+
 def test_plan_empty(cuda):
     import torch
     from srf_amd.plan import TorPlan
