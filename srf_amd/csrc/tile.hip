@@ -359,13 +359,17 @@ int plan_build_sets(const std::vector<PlanSet> &sets, hipStream_t st)
     }
     // temporaries: the sets' heads, their counters, one error count per set,
     // per plan the flag of an event that needs another walk than the
-    // NB x (NB - 1) box
+    // NB x (NB - 1) box, the chunk sort's jobs
     const size_t ctr_b = align256(sizeof(int) * std::max<size_t>(ctr_n, 1));
-    const size_t tmp_b = head_b + ctr_b + sizeof(int) * (S + nplans);
+    const size_t sjobs_at = align256(head_b + ctr_b + sizeof(int) * (S + nplans));
+    const size_t tmp_b = sjobs_at + sizeof(SortJob) * nplans;
     char *tmp = nullptr, *blk0 = nullptr, *blk1 = nullptr;
     std::vector<char> head(head_b);
     std::vector<int> ctr(ctr_n);
     std::vector<int> errs(S, 0), others(nplans, 0);
+    std::vector<SortJob> sjobs;
+    int64_t sblocks = 0;
+    int max_chunk = 0; // largest chunk of the sorted plans (the sort's LDS)
     std::vector<TileChunk> ch;
     int rc = SRFX_OK;
     int *ctr_d = nullptr, *err_d = nullptr, *other_d = nullptr;
@@ -538,18 +542,30 @@ int plan_build_sets(const std::vector<PlanSet> &sets, hipStream_t st)
             rc = SRFX_ELAUNCH; // count and fill passes disagreed: no valid plan
             goto done;
         }
-    // listings of each chunk ordered by slice count, longest first
+    // listings of each chunk ordered by slice count, longest first: one
+    // launch for the chunks of every plan
     for (const PlanSet &ps : sets) {
         if (ps.chunk > 16 * 256)
             continue;
         for (int k = 0; k < ps.K; k++) {
             srfx_tor_plan_s *pl = ps.pls[k];
-            if (pl->nchunks > 0)
-                hipLaunchKernelGGL(sort_chunks, dim3(pl->nchunks), dim3(256),
-                                   sizeof(int) * (33 + ps.chunk), st, pl->p, pl->g, pl->chunks,
-                                   pl->list, pl->rec);
+            if (pl->nchunks > 0) {
+                sjobs.push_back(SortJob{pl->chunks, pl->list, sblocks});
+                sblocks += pl->nchunks;
+                max_chunk = std::max(max_chunk, ps.chunk);
+            }
             pl->g.sorted = 1;
         }
+    }
+    if (!sjobs.empty()) {
+        if (hipMemcpyAsync(tmp + sjobs_at, sjobs.data(), sizeof(SortJob) * sjobs.size(),
+                           hipMemcpyHostToDevice, st) != hipSuccess) {
+            rc = SRFX_ELAUNCH;
+            goto done;
+        }
+        hipLaunchKernelGGL(sort_chunks_many, dim3((unsigned)sblocks), dim3(256),
+                           sizeof(int) * (33 + max_chunk), st, (const SortJob *)(tmp + sjobs_at),
+                           (int)sjobs.size());
     }
     rc = launch_status();
 done:

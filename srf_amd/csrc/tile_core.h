@@ -903,23 +903,20 @@ __device__ __forceinline__ void scatter_chunk(const TorParams &p, const TileGeom
 }
 
 // Plan build, after the fill pass: order each chunk's listings by the number
-// of slices they walk in the tile, longest first (a counting sort in LDS,
-// stable), so that the consecutive listings a wave of the scatter takes walk
-// about as many slices each (the order inside a bucket is the atomics').
-// With g.spans the slice counts come from the entries (the fill pass wrote
-// them) instead of the walk records.  One workgroup per chunk; LDS: 33
-// counters and `chunk` ints.  Needs ch.count <= 16 *
-// blockDim.x.
-__global__ __launch_bounds__(256) void sort_chunks(TorParams p, TileGeom g,
-                                                   const TileChunk *__restrict__ chunks,
-                                                   int *__restrict__ list,
-                                                   const float4 *__restrict__ rec)
+// of slices they walk in the tile, longest first (a counting sort in LDS;
+// inside a bucket the order of the LDS atomics, which with coalesced reads
+// keeps roughly the fill order), so that the consecutive listings a wave of
+// the scatter takes walk about as many slices each.  With spans the slice
+// counts come from the entries (the fill pass wrote them) instead of the walk
+// records.  One workgroup per chunk; LDS: 33 counters and `chunk` ints.
+// Needs ch.count <= 16 * blockDim.x.
+template <bool SPANS>
+__device__ __forceinline__ void sort_chunk(const TorParams &p, const TileGeom &g,
+                                           const TileChunk &ch, int *__restrict__ list,
+                                           const float4 *__restrict__ rec, int *sh)
 {
-    extern __shared__ int sh[];
     constexpr int kBuckets = 32, kPer = 16;
     int *cnt = sh, *ord = sh + kBuckets + 1;
-    const TileChunk ch = chunks[blockIdx.x];
-    const TileBox b = tile_box(p, g, ch.tile);
     for (int i = threadIdx.x; i <= kBuckets; i += blockDim.x)
         cnt[i] = 0;
     __syncthreads();
@@ -931,7 +928,7 @@ __global__ __launch_bounds__(256) void sort_chunks(TorParams p, TileGeom g,
         if (j < ch.count) {
             ev[r] = list[ch.start + j];
             int s0, s1;
-            if (g.spans) {
+            if (SPANS) {
                 // the fill pass put the listing's slices in the entry
                 s0 = (ev[r] >> kSpanEventBits) & 15;
                 s1 = (int)((unsigned)ev[r] >> (kSpanEventBits + 4));
@@ -939,7 +936,7 @@ __global__ __launch_bounds__(256) void sort_chunks(TorParams p, TileGeom g,
                 Event e;
                 Track t;
                 unpack_walk(p, rec, ev[r], e, t);
-                listing_slices(p, b, e, t, s0, s1);
+                listing_slices(p, tile_box(p, g, ch.tile), e, t, s0, s1);
             }
             // longest first: bucket 0 holds the longest walks
             key[r] = kBuckets - 1 - min(max(s1 - s0 + 1, 0), kBuckets - 1);
@@ -963,6 +960,45 @@ __global__ __launch_bounds__(256) void sort_chunks(TorParams p, TileGeom g,
     __syncthreads();
     for (int j = threadIdx.x; j < ch.count; j += blockDim.x)
         list[ch.start + j] = ord[j];
+}
+
+__global__ __launch_bounds__(256) void sort_chunks(TorParams p, TileGeom g,
+                                                   const TileChunk *__restrict__ chunks,
+                                                   int *__restrict__ list,
+                                                   const float4 *__restrict__ rec)
+{
+    extern __shared__ int sh[];
+    if (g.spans)
+        sort_chunk<true>(p, g, chunks[blockIdx.x], list, rec, sh);
+    else
+        sort_chunk<false>(p, g, chunks[blockIdx.x], list, rec, sh);
+}
+
+// The chunk sort of many plans with spans in one launch (a plan set build):
+// job j sorts the chunks of one plan, workgroups blk0 .. of the grid.
+struct SortJob {
+    const TileChunk *chunks;
+    int *list;
+    int64_t blk0;
+};
+
+__global__ __launch_bounds__(256) void sort_chunks_many(const SortJob *__restrict__ jobs,
+                                                        int njobs)
+{
+    extern __shared__ int sh[];
+    // the plan of this workgroup: the last job whose first workgroup <= b
+    const int64_t b = blockIdx.x;
+    int lo = 0, hi = njobs - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) / 2;
+        if (jobs[mid].blk0 <= b)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    const SortJob jb = jobs[lo];
+    const TileChunk ch = jb.chunks[b - jb.blk0];
+    sort_chunk<true>(TorParams{}, TileGeom{}, ch, jb.list, nullptr, sh);
 }
 
 // One workgroup per chunk.  The LDS tile is FP64: LDS FP64 atomic adds run
