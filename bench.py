@@ -228,19 +228,24 @@ def main():
         model.schedule["projection"] = (4, 256)
     osem = OSEM(model, data, emap, nb_subsets=args.subsets, fused=not args.unfused,
                 tiled=not args.untiled)
-    plan_s, plan_cold_s, listings = None, None, None
+    plan_s, plan_cold_s, listings, plan_runs_ms = None, None, None, None
     if osem.tiled:
         # the first build in a process also pays one-time costs (first
         # launches, first mapping of the plans' memory), as the first
-        # iteration does: like the iterations, the timed build follows a
-        # warm-up build, and the cold figure is reported beside it
-        for _ in range(2):
+        # iteration does: like the iterations, the timed builds follow a
+        # warm-up build, whose figure is reported beside them; the build
+        # charged is the median of three (a build's allocations and host
+        # round trips vary by ~1 ms from run to run)
+        builds = []
+        for _ in range(4):
             osem.close()
             torch.cuda.synchronize()
             t0 = time.perf_counter()
             osem.plan(image)
             torch.cuda.synchronize()
-            plan_cold_s, plan_s = plan_s, time.perf_counter() - t0
+            builds.append(time.perf_counter() - t0)
+        plan_cold_s, plan_s = builds[0], sorted(builds[1:])[1]
+        plan_runs_ms = [round(1e3 * b, 2) for b in builds[1:]]
         listings = sum(p.info()[0] for ps in osem.plans for p in ps.values() if p is not None)
         log(f"tile plans for {args.subsets} subsets in {plan_s:.2f}s: "
             f"{listings / args.events:.1f} (event, tile) listings per event")
@@ -404,6 +409,7 @@ def main():
         "value_excl_plan_build": round(value_kernels, 2),
         "plan_build_ms": round(plan_s * 1e3, 2) if plan_s is not None else None,
         "plan_build_cold_ms": round(plan_cold_s * 1e3, 2) if plan_cold_s is not None else None,
+        "plan_build_runs_ms": plan_runs_ms,
         "plan_build_amortised_over_iterations": C2_ITERATIONS if plan_s is not None else None,
         "config": {"workload": args.workload, "image": list(grid), "voxel_mm": list(meta["voxel"]),
                    "events": args.events, "subsets": args.subsets,

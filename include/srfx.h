@@ -269,6 +269,28 @@ int srfx_tor_plan_create_subsets(const float *lors, int64_t n, int64_t ld, int32
                                  int32_t nplans, const int64_t *bounds, srfx_tor_plan *plans,
                                  void *stream);
 
+/* One LOR array of srfx_tor_plan_create_subset_groups: the arguments
+ * srfx_tor_plan_create_subsets takes for it. */
+typedef struct srfx_subset_group {
+    const float *lors;
+    int64_t n, ld;
+    srfx_grid grid;
+    srfx_tile tile; /* projection must be 0 */
+    int32_t nplans;
+    const int64_t *bounds; /* nplans + 1 host values */
+} srfx_subset_group;
+
+/* srfx_tor_plan_create_subsets for several LOR arrays (e.g. the x, y and z
+ * groups of one event set) in one build: plans receives group 0's nplans
+ * plans, then group 1's, and so on, each as srfx_tor_plan_create_subsets
+ * would build it.  All groups share the count and fill phases, their two
+ * host synchronisations and one chunk sort launch.  Groups with
+ * tile.projection set are refused (SRFX_EINVAL).  No reference
+ * counterpart. */
+int srfx_tor_plan_create_subset_groups(const srfx_subset_group *groups, int32_t ngroups,
+                                       int32_t ncols, srfx_tor_model model,
+                                       srfx_tor_plan *plans, void *stream);
+
 /* Listings (event, tile pairs) and workgroup chunks of a plan. */
 int srfx_tor_plan_info(srfx_tor_plan plan, int64_t *nlistings, int32_t *nchunks);
 

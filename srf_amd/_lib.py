@@ -72,6 +72,14 @@ class SrfxTile(ctypes.Structure):
                 ("projection", ctypes.c_int32)]
 
 
+class SrfxSubsetGroup(ctypes.Structure):
+    """srfx_subset_group: one LOR array (SoA rows, n events, row stride ld) of
+    a subset-plan build, its frame, its tile and the bounds of its plans."""
+    _fields_ = [("lors", ctypes.c_void_p), ("n", ctypes.c_int64), ("ld", ctypes.c_int64),
+                ("grid", SrfxGrid), ("tile", SrfxTile), ("nplans", ctypes.c_int32),
+                ("bounds", ctypes.POINTER(ctypes.c_int64))]
+
+
 class SrfxSiddonTile(ctypes.Structure):
     """srfx_siddon_tile: image tiles of `tx` x `ty` x `tz` voxels, at most
     `chunk` runs per workgroup, `block` threads per workgroup; `sort` keeps a
@@ -116,6 +124,8 @@ _PROTOS = {
     "srfx_tor_plan_create_subsets": [_P, _I64, _I64, _I32, SrfxGrid, SrfxTorModel, SrfxTile,
                                      _I32, ctypes.POINTER(_I64), ctypes.POINTER(ctypes.c_void_p),
                                      _P],
+    "srfx_tor_plan_create_subset_groups": [ctypes.POINTER(SrfxSubsetGroup), _I32, _I32,
+                                           SrfxTorModel, ctypes.POINTER(ctypes.c_void_p), _P],
     "srfx_tor_plan_info": [_P, ctypes.POINTER(_I64), ctypes.POINTER(_I32)],
     "srfx_tor_plan_walks": [_P, ctypes.POINTER(_I32), ctypes.POINTER(_I32)],
     "srfx_tor_plan_destroy": [_P],

@@ -425,11 +425,11 @@ extern "C" int srfx_tor_batch_create(const srfx_recon *recons, int32_t nrecon,
             for (int k = 0; k <= nb_subsets; k++)
                 bd[k] = k * rc.n[a] / nb_subsets;
             requests.push_back(SubsetRequest{rc.lors[a], rc.n[a], rc.ld[a], group_grid(rc, a),
-                                             nb_subsets, bd.data()});
+                                             kBatchTile, nb_subsets, bd.data()});
         }
     std::vector<srfx_tor_plan> all_plans(requests.size() * nb_subsets, nullptr);
     rc_status = plan_create_subsets_many(requests.data(), (int)requests.size(), 10, m,
-                                         kBatchTile, all_plans.data(), nullptr);
+                                         all_plans.data(), nullptr);
     if (rc_status) {
         free_plan(b);
         return rc_status;

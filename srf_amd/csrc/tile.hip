@@ -531,19 +531,10 @@ int plan_build_sets(const std::vector<PlanSet> &sets, hipStream_t st)
     }
     if ((rc = launch(true)))
         goto done;
-    if (hipMemcpyAsync(errs.data(), err_d, sizeof(int) * S, hipMemcpyDeviceToHost, st) !=
-            hipSuccess ||
-        hipStreamSynchronize(st) != hipSuccess) {
-        rc = SRFX_ELAUNCH;
-        goto done;
-    }
-    for (int s = 0; s < S; s++)
-        if (errs[s]) {
-            rc = SRFX_ELAUNCH; // count and fill passes disagreed: no valid plan
-            goto done;
-        }
     // listings of each chunk ordered by slice count, longest first: one
-    // launch for the chunks of every plan
+    // launch for the chunks of every plan.  It runs before the fill's error
+    // count is read (one host synchronisation less): it only permutes the
+    // entries inside each chunk's range, whatever they hold.
     for (const PlanSet &ps : sets) {
         if (ps.chunk > 16 * 256)
             continue;
@@ -567,7 +558,19 @@ int plan_build_sets(const std::vector<PlanSet> &sets, hipStream_t st)
                            sizeof(int) * (33 + max_chunk), st, (const SortJob *)(tmp + sjobs_at),
                            (int)sjobs.size());
     }
-    rc = launch_status();
+    if ((rc = launch_status()))
+        goto done;
+    if (hipMemcpyAsync(errs.data(), err_d, sizeof(int) * S, hipMemcpyDeviceToHost, st) !=
+            hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess) {
+        rc = SRFX_ELAUNCH;
+        goto done;
+    }
+    for (int s = 0; s < S; s++)
+        if (errs[s]) {
+            rc = SRFX_ELAUNCH; // count and fill passes disagreed: no valid plan
+            goto done;
+        }
 done:
     // host buffers above are read by the copies: finish them before returning
     (void)hipStreamSynchronize(st);
@@ -771,8 +774,7 @@ extern "C" int srfx_tor_plan_create(const float *lors, int64_t n, int64_t ld, in
 namespace srfx {
 
 int plan_create_subsets_many(const SubsetRequest *req, int nreq, int32_t ncols,
-                             const srfx_tor_model &m, const srfx_tile &tile, srfx_tor_plan *plans,
-                             hipStream_t st)
+                             const srfx_tor_model &m, srfx_tor_plan *plans, hipStream_t st)
 {
     if (!plans || (nreq > 0 && !req) || nreq < 0)
         return SRFX_EINVAL;
@@ -796,14 +798,15 @@ int plan_create_subsets_many(const SubsetRequest *req, int nreq, int32_t ncols,
     std::vector<srfx_tor_plan_s *> pls(total, nullptr);
     std::vector<std::vector<int64_t>> first(nreq);
     int rc = SRFX_OK;
-    bool together = tile.projection == 0;
+    bool together = true;
     for (int r = 0, i = 0; r < nreq && !rc; r++) {
         const SubsetRequest &q = req[r];
+        together = together && q.tile.projection == 0;
         first[r].resize(q.nplans + 1);
         for (int k = 0; k < q.nplans && !rc; k++, i++) {
             const int64_t nk = q.bounds[k + 1] - q.bounds[k];
             rc = plan_new(q.lors ? q.lors + q.bounds[k] : nullptr, nk, q.ld, ncols, q.grid, m,
-                          tile, &pls[i]);
+                          q.tile, &pls[i]);
             if (!rc)
                 together = together && set_buildable(pls[i]->g, nk, false);
             first[r][k] = q.bounds[k] - q.bounds[0];
@@ -816,11 +819,12 @@ int plan_create_subsets_many(const SubsetRequest *req, int nreq, int32_t ncols,
             for (int r = 0, i = 0; r < nreq; i += req[r].nplans, r++)
                 sets.push_back(PlanSet{pls.data() + i, req[r].nplans, first[r].data(),
                                        req[r].lors ? req[r].lors + req[r].bounds[0] : nullptr,
-                                       tile.chunk});
+                                       req[r].tile.chunk});
             rc = plan_build_sets(sets, st);
         } else {
-            for (int i = 0; i < total && !rc; i++)
-                rc = plan_build(pls[i], st, tile.chunk, tile.projection != 0);
+            for (int r = 0, i = 0; r < nreq && !rc; r++)
+                for (int k = 0; k < req[r].nplans && !rc; k++, i++)
+                    rc = plan_build(pls[i], st, req[r].tile.chunk, req[r].tile.projection != 0);
         }
     }
     if (rc) {
@@ -842,8 +846,25 @@ extern "C" int srfx_tor_plan_create_subsets(const float *lors, int64_t n, int64_
 {
     if (!plans || !bounds || nplans < 1 || n < 0)
         return SRFX_EINVAL;
-    const SubsetRequest q{lors, n, ld, grid, nplans, bounds};
-    return plan_create_subsets_many(&q, 1, ncols, m, tile, plans, (hipStream_t)stream);
+    const SubsetRequest q{lors, n, ld, grid, tile, nplans, bounds};
+    return plan_create_subsets_many(&q, 1, ncols, m, plans, (hipStream_t)stream);
+}
+
+extern "C" int srfx_tor_plan_create_subset_groups(const srfx_subset_group *groups,
+                                                  int32_t ngroups, int32_t ncols,
+                                                  srfx_tor_model m, srfx_tor_plan *plans,
+                                                  void *stream)
+{
+    if (!plans || ngroups < 0 || (ngroups > 0 && !groups))
+        return SRFX_EINVAL;
+    std::vector<SubsetRequest> req;
+    for (int r = 0; r < ngroups; r++) {
+        const srfx_subset_group &q = groups[r];
+        if (!q.bounds || q.nplans < 1 || q.n < 0 || q.tile.projection != 0)
+            return SRFX_EINVAL;
+        req.push_back(SubsetRequest{q.lors, q.n, q.ld, q.grid, q.tile, q.nplans, q.bounds});
+    }
+    return plan_create_subsets_many(req.data(), ngroups, ncols, m, plans, (hipStream_t)stream);
 }
 
 extern "C" int srfx_tor_plan_info(srfx_tor_plan plan, int64_t *nlistings, int32_t *nchunks)

@@ -902,14 +902,31 @@ __device__ __forceinline__ void scatter_chunk(const TorParams &p, const TileGeom
     __syncthreads();
 }
 
+// The lanes of the wave whose v (0 <= v < 64) equals this lane's: one ballot
+// per bit of v.  Call with every lane of the wave active.
+__device__ __forceinline__ uint64_t lanes_with(int v)
+{
+    uint64_t m = ~0ull;
+#pragma unroll
+    for (int b = 0; b < 6; b++) {
+        const bool bit = (v >> b) & 1;
+        const uint64_t bal = __ballot(bit);
+        m &= bit ? bal : ~bal;
+    }
+    return m;
+}
+
 // Plan build, after the fill pass: order each chunk's listings by the number
-// of slices they walk in the tile, longest first (a counting sort in LDS;
-// inside a bucket the order of the LDS atomics, which with coalesced reads
-// keeps roughly the fill order), so that the consecutive listings a wave of
-// the scatter takes walk about as many slices each.  With spans the slice
-// counts come from the entries (the fill pass wrote them) instead of the walk
-// records.  One workgroup per chunk; LDS: 33 counters and `chunk` ints.
-// Needs ch.count <= 16 * blockDim.x.
+// of slices they walk in the tile, longest first (a counting sort in LDS), so
+// that the consecutive listings a wave of the scatter takes walk about as
+// many slices each.  Inside a bucket the listings keep their order within a
+// wave and the waves their order of arrival, which with coalesced reads
+// keeps roughly the fill order.  The lanes of a wave with the same key count
+// and take their positions together: one LDS atomic per key and wave, the
+// lowest such lane's, instead of one per lane on a few shared counters.
+// With spans the slice counts come from the entries (the fill pass wrote
+// them) instead of the walk records.  One workgroup per chunk, 256 threads;
+// LDS: 33 counters and `chunk` ints.  Needs ch.count <= 16 * blockDim.x.
 template <bool SPANS>
 __device__ __forceinline__ void sort_chunk(const TorParams &p, const TileGeom &g,
                                            const TileChunk &ch, int *__restrict__ list,
@@ -917,14 +934,21 @@ __device__ __forceinline__ void sort_chunk(const TorParams &p, const TileGeom &g
 {
     constexpr int kBuckets = 32, kPer = 16;
     int *cnt = sh, *ord = sh + kBuckets + 1;
+    const int lane = (int)threadIdx.x & 63;
     for (int i = threadIdx.x; i <= kBuckets; i += blockDim.x)
         cnt[i] = 0;
     __syncthreads();
-    int key[kPer], ev[kPer];
+    // per entry: its key (-1: no entry), event, and its rank among the wave's
+    // lanes with the same key | the first such lane << 6 | their number << 12
+    int key[kPer], ev[kPer], grp[kPer];
 #pragma unroll
     for (int r = 0; r < kPer; r++) {
-        const int j = r * (int)blockDim.x + (int)threadIdx.x; // coalesced reads
         key[r] = -1;
+        ev[r] = 0;
+        grp[r] = 0;
+        if (r * (int)blockDim.x >= ch.count)
+            continue; // the same for every lane of the workgroup
+        const int j = r * (int)blockDim.x + (int)threadIdx.x; // coalesced reads
         if (j < ch.count) {
             ev[r] = list[ch.start + j];
             int s0, s1;
@@ -940,23 +964,41 @@ __device__ __forceinline__ void sort_chunk(const TorParams &p, const TileGeom &g
             }
             // longest first: bucket 0 holds the longest walks
             key[r] = kBuckets - 1 - min(max(s1 - s0 + 1, 0), kBuckets - 1);
-            atomicAdd(&cnt[key[r]], 1);
         }
+        const uint64_t m = lanes_with(key[r] + 1);
+        const int rank = (int)__builtin_amdgcn_mbcnt_hi(
+            (unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+        const int n = __popcll(m);
+        grp[r] = rank | (__builtin_ctzll(m) << 6) | (n << 12);
+        if (key[r] >= 0 && rank == 0)
+            atomicAdd(&cnt[key[r]], n);
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
-        int acc = 0;
-        for (int k = 0; k < kBuckets; k++) {
-            const int c = cnt[k];
-            cnt[k] = acc;
-            acc += c;
+    // exclusive prefix of the bucket counts, by the first wave
+    if (threadIdx.x < 64) {
+        const int c = lane < kBuckets ? cnt[lane] : 0;
+        int v = c;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const int u = __shfl_up(v, d, 64);
+            v += lane >= d ? u : 0;
         }
+        if (lane < kBuckets)
+            cnt[lane] = v - c;
     }
     __syncthreads();
 #pragma unroll
-    for (int r = 0; r < kPer; r++)
+    for (int r = 0; r < kPer; r++) {
+        if (r * (int)blockDim.x >= ch.count)
+            continue;
+        const int rank = grp[r] & 63;
+        int base = 0;
+        if (key[r] >= 0 && rank == 0)
+            base = atomicAdd(&cnt[key[r]], grp[r] >> 12);
+        base = __shfl(base, (grp[r] >> 6) & 63, 64);
         if (key[r] >= 0)
-            ord[atomicAdd(&cnt[key[r]], 1)] = ev[r];
+            ord[base + rank] = ev[r];
+    }
     __syncthreads();
     for (int j = threadIdx.x; j < ch.count; j += blockDim.x)
         list[ch.start + j] = ord[j];
@@ -1071,21 +1113,21 @@ __global__ __launch_bounds__(256) void tile_reduce(const float *__restrict__ par
 
 } // namespace
 // The OSEM subset plans of several LOR arrays at once: for request r, the
-// plans srfx_tor_plan_create_subsets builds for (lors, n, ld, grid, nplans,
-// bounds), written in request order to plans (sum of nplans entries).  All
-// requests share the build's phases (one count pass per request, one host
-// synchronisation, one fill pass per request, one host synchronisation) and
-// two device blocks.
+// plans srfx_tor_plan_create_subsets builds for (lors, n, ld, grid, tile,
+// nplans, bounds), written in request order to plans (sum of nplans
+// entries).  All requests share the build's phases (one count pass per
+// request, one host synchronisation, one fill pass per request, one host
+// synchronisation, one chunk sort launch) and two device blocks.
 struct SubsetRequest {
     const float *lors;
     int64_t n, ld;
     srfx_grid grid;
+    srfx_tile tile;
     int32_t nplans;
     const int64_t *bounds;
 };
 int plan_create_subsets_many(const SubsetRequest *req, int nreq, int32_t ncols,
-                             const srfx_tor_model &m, const srfx_tile &tile, srfx_tor_plan *plans,
-                             hipStream_t st);
+                             const srfx_tor_model &m, srfx_tor_plan *plans, hipStream_t st);
 
 } // namespace srfx
 

@@ -148,28 +148,29 @@ class SplitLoRsModel:
 
     def make_subset_plans(self, image, data, nb_subsets, ncols=10):
         """make_plans for every OSEM subset of ``data`` (contiguous rows of
-        each group, recon.subset_bounds), each group's subsets built together
-        (srfx_tor_plan_create_subsets).  Returns a list over subsets of
-        {axis: TorPlan or None}."""
+        each group, recon.subset_bounds), the subsets of every group built
+        together (srfx_tor_plan_create_subset_groups).  Returns a list over
+        subsets of {axis: TorPlan or None}."""
         from .plan import DEFAULT_CHUNK, TorPlan, default_tile
         from .recon import subset_bounds
         x = _world(image)
         K = int(nb_subsets)
-        out = [dict() for _ in range(K)]
+        out = [{a: None for a in self.AXIS} for _ in range(K)]
+        groups, axes = [], []
         for a in self.AXIS:
             lm = data[a]
             n = len(lm)
-            bounds = [subset_bounds(n, k, K)[0] for k in range(K)] + [n]
             if n == 0:
-                for k in range(K):
-                    out[k][a] = None
                 continue
             g, c, sz, st = group_frame(x.shape, image.center, image.size, x.stride(), a)
-            plans = TorPlan.subsets(lm.soa, bounds, g, c, sz, self.kernel_width,
-                                    self.geo_sigma2_flag, self.tof_bin, self.tof_sigma2,
-                                    strides=st, ncols=ncols,
-                                    tile=self.tile or default_tile(g, sz, self.kernel_width),
-                                    chunk=self.chunk or DEFAULT_CHUNK, axis=a)
+            groups.append(dict(lors=lm.soa, bounds=[subset_bounds(n, k, K)[0] for k in range(K)]
+                               + [n], grid=g, center=c, size=sz, strides=st, axis=a,
+                               tile=self.tile or default_tile(g, sz, self.kernel_width)))
+            axes.append(a)
+        built = TorPlan.subset_groups(groups, self.kernel_width, self.geo_sigma2_flag,
+                                      self.tof_bin, self.tof_sigma2, ncols=ncols,
+                                      chunk=self.chunk or DEFAULT_CHUNK) if groups else []
+        for a, plans in zip(axes, built):
             for k, pl in enumerate(plans):
                 if pl.n == 0:
                     pl.close()

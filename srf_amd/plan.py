@@ -84,21 +84,68 @@ class TorPlan:
         bs = (ctypes.c_int64 * (K + 1))(*bounds)
         _lib.call("srfx_tor_plan_create_subsets", _ptr(full), n, ld, ncols, cgrid, model, t, K,
                   bs, hs, _stream())
-        out = []
-        for k in range(K):
-            pl = cls.__new__(cls)
-            pl.lors = full[:, bounds[k]:bounds[k + 1]]
-            pl.n = bounds[k + 1] - bounds[k]
-            pl.grid, pl._cgrid = g, cgrid
-            pl.center, pl.size = [float(c) for c in center], [float(s) for s in size]
-            pl.strides = tuple(int(s) for s in strides)
-            pl.axis = axis
-            pl.ncols = ncols
-            pl.tile = tuple(int(v) for v in tile)
-            pl.tiled_projection = False
-            pl._h = ctypes.c_void_p(hs[k])
-            out.append(pl)
+        return [cls._wrap(hs[k], full, bounds[k], bounds[k + 1], g, cgrid, center, size,
+                          strides, axis, ncols, tile) for k in range(K)]
+
+    @classmethod
+    def subset_groups(cls, groups, kernel_width, geo_sigma2_flag=False, tof_bin=1.0e4,
+                      tof_sigma2=1.0e4, ncols=10, chunk=DEFAULT_CHUNK, block=256):
+        """``subsets`` for several LOR tensors in one build
+        (srfx_tor_plan_create_subset_groups): ``groups`` is a list of dicts
+        with the keys lors, bounds, grid, center, size and optionally strides,
+        tile and axis, as ``subsets`` takes them.  Returns, per group, its list
+        of TorPlan."""
+        if ncols not in (7, 10):
+            raise ValueError("ncols must be 10 (TOF rows) or 7 (Maplors rows)")
+        model = _lib.make_tor_model(kernel_width, geo_sigma2_flag, tof_bin, tof_sigma2)
+        specs, keep = [], []
+        arr = (_lib.SrfxSubsetGroup * max(len(groups), 1))()
+        for i, gr in enumerate(groups):
+            full, n, ld = _soa(gr["lors"], ncols)
+            bounds = [int(b) for b in gr["bounds"]]
+            K = len(bounds) - 1
+            if K < 1 or bounds[0] < 0 or bounds[-1] > n or any(
+                    b1 < b0 for b0, b1 in zip(bounds, bounds[1:])):
+                raise ValueError("bounds must be non-decreasing event offsets within the LORs")
+            strides = gr.get("strides", (0, 0, 0))
+            tile = gr.get("tile", DEFAULT_TILE)
+            g, cgrid = _geom(gr["grid"], gr["center"], gr["size"], strides)
+            bs = (ctypes.c_int64 * (K + 1))(*bounds)
+            keep.append(bs)
+            arr[i] = _lib.SrfxSubsetGroup(
+                _ptr(full), n, ld, cgrid,
+                _lib.SrfxTile(*[int(v) for v in tile], int(chunk), int(block), 0), K, bs)
+            specs.append((full, bounds, g, cgrid, gr["center"], gr["size"], strides,
+                          gr.get("axis"), tile))
+        total = sum(len(sp[1]) - 1 for sp in specs)
+        hs = (ctypes.c_void_p * max(total, 1))()
+        _lib.call("srfx_tor_plan_create_subset_groups", arr, len(groups), ncols, model, hs,
+                  _stream())
+        out, j = [], 0
+        for full, bounds, g, cgrid, center, size, strides, axis, tile in specs:
+            plans = []
+            for k in range(len(bounds) - 1):
+                plans.append(cls._wrap(hs[j], full, bounds[k], bounds[k + 1], g, cgrid, center,
+                                       size, strides, axis, ncols, tile))
+                j += 1
+            out.append(plans)
         return out
+
+    @classmethod
+    def _wrap(cls, handle, full, b0, b1, g, cgrid, center, size, strides, axis, ncols, tile):
+        """The TorPlan of a handle from a subset build (events b0:b1 of full)."""
+        pl = cls.__new__(cls)
+        pl.lors = full[:, b0:b1]
+        pl.n = b1 - b0
+        pl.grid, pl._cgrid = g, cgrid
+        pl.center, pl.size = [float(c) for c in center], [float(s) for s in size]
+        pl.strides = tuple(int(s) for s in strides)
+        pl.axis = axis
+        pl.ncols = ncols
+        pl.tile = tuple(int(v) for v in tile)
+        pl.tiled_projection = False
+        pl._h = ctypes.c_void_p(handle)
+        return pl
 
     def info(self):
         """(number of (event, tile) listings, number of workgroup chunks)."""

@@ -91,6 +91,24 @@ def test_invalid_arguments_are_rejected():
     assert lib.srfx_frame_copy(fake, _grid(), None, None) == _lib.SRFX_EINVAL
     # plan queries: null plan
     assert lib.srfx_tor_plan_walks(None, None, None) == _lib.SRFX_EINVAL
+    # grouped subset plans: negative count, null groups, null output, a
+    # group with the tiled projection, decreasing bounds, bounds past n
+    hs = (ctypes.c_void_p * 4)()
+    bs = (ctypes.c_int64 * 3)(0, 6, 10)
+    tile = _lib.SrfxTile(10, 12, 32, 2048, 256, 0)
+
+    def group(bounds=bs, n=10, t=tile):
+        return (_lib.SrfxSubsetGroup * 1)(_lib.SrfxSubsetGroup(fake.value, n, n, _grid(), t, 2,
+                                                               bounds))
+    create = lib.srfx_tor_plan_create_subset_groups
+    assert create(group(), -1, 10, m, hs, None) == _lib.SRFX_EINVAL
+    assert create(None, 1, 10, m, hs, None) == _lib.SRFX_EINVAL
+    assert create(group(), 1, 10, m, None, None) == _lib.SRFX_EINVAL
+    assert create(group(t=_lib.SrfxTile(10, 12, 32, 2048, 256, 1)), 1, 10, m, hs,
+                  None) == _lib.SRFX_EINVAL
+    assert create(group(bounds=(ctypes.c_int64 * 3)(0, 6, 4)), 1, 10, m, hs,
+                  None) == _lib.SRFX_EINVAL
+    assert create(group(n=8), 1, 10, m, hs, None) == _lib.SRFX_EINVAL
     assert lib.srfx_strerror(_lib.SRFX_EINVAL) == b"invalid argument"
 
 

@@ -10,6 +10,7 @@ fixed order, so it is deterministic.
 """
 import numpy as np
 import pytest
+import torch
 
 from tests import helpers as H
 
@@ -349,3 +350,36 @@ def test_plan_subsets_equal_single_plans(cuda, case, tile, chunk):
         one.close()
     for pl in plans[::-1]:  # the shared blocks go with the last plan
         pl.close()
+
+
+def test_plan_subset_groups_equal_subsets(cuda, case):
+    """Subset plans of several LOR arrays built in one call
+    (srfx_tor_plan_create_subset_groups), each group with its own tile and
+    bounds, equal each group's plans built by srfx_tor_plan_create_subsets:
+    the same listings and walks, the same OSEM step (to FP32 rounding of the
+    flush order)."""
+    from srf_amd.plan import TorPlan
+    lors, img = case
+    L, X = _t(lors, cuda), _t(img, cuda)
+    n = lors.shape[1]
+    L2 = torch.flip(L, dims=[1]).contiguous()  # the same events in another order
+    args = (H.TOR_GRID, H.TOR_CENTER, H.TOR_SIZE)
+    model = (H.KW, True, H.TOF_BIN, H.TOF_SIGMA2)
+    specs = [dict(lors=L, bounds=[0, 1000, n], tile=(8, 32, 20)),
+             dict(lors=L2, bounds=[0, 3, 3, 2000, n], tile=(12, 32, 10), axis="y")]
+    groups = [dict(s, grid=args[0], center=args[1], size=args[2]) for s in specs]
+    built = TorPlan.subset_groups(groups, *model)
+    assert [len(b) for b in built] == [2, 4]
+    for spec, plans in zip(specs, built):
+        ref = TorPlan.subsets(spec["lors"], spec["bounds"], *args, *model, tile=spec["tile"])
+        for pl, rp in zip(plans, ref):
+            assert pl.n == rp.n and pl.tile == rp.tile and pl.axis == spec.get("axis")
+            assert pl.info() == rp.info() and pl.walks() == rp.walks()
+            if pl.n:
+                a, b = pl.osem_subset(X).cpu().numpy(), rp.osem_subset(X).cpu().numpy()
+                assert H.rel_l2(a, b) <= 1e-6
+        for rp in ref[::-1]:
+            rp.close()
+    for plans in built[::-1]:
+        for pl in plans[::-1]:
+            pl.close()
