@@ -384,6 +384,7 @@ int plan_build_sets(const std::vector<PlanSet> &sets, hipStream_t st)
         a.ntiles = l.nt;
         a.nplans = ps.K;
         a.lds = 3u * sizeof(int) * l.nt <= 65536u;
+        a.queue = kFillQueueLds + 3u * sizeof(int) * l.nt <= 65536u;
         a.first = (const int64_t *)(tmp + l.head);
         a.rec = (float4 *const *)(tmp + l.head + sizeof(int64_t) * (ps.K + 1));
         a.list = (int *const *)(tmp + l.head + sizeof(int64_t) * (ps.K + 1) +
@@ -404,7 +405,8 @@ int plan_build_sets(const std::vector<PlanSet> &sets, hipStream_t st)
             const PlanSetArgs a = args(s);
             TorParams p = ps.pls[0]->p;
             p.lors = ps.lors;
-            const unsigned lds = a.lds ? (fill ? 3u : 1u) * sizeof(int) * L[s].nt : 0u;
+            const unsigned lds = (a.lds ? (fill ? 3u : 1u) * sizeof(int) * L[s].nt : 0u) +
+                                 (fill && a.queue ? kFillQueueLds : 0u);
             if (fill)
                 hipLaunchKernelGGL(plan_set_pass<true>, dim3(blocks_for(n, 256)), dim3(256), lds,
                                    st, p, L[s].g, a);

@@ -391,6 +391,7 @@ __global__ __launch_bounds__(256) void plan_pass(TorParams p, TileGeom g, PlanAr
 struct PlanSetArgs {
     int cols7, ntiles, nplans;
     int lds;                // the per-workgroup counters fit in LDS
+    int queue;              // fill pass: LDS also holds the listing queues (kFillQueue)
     const int64_t *first;   // [nplans + 1]
     float4 *const *rec;     // [nplans] walk records
     int *cnt;               // count pass: listings per (plan, tile); fill pass: read
@@ -416,15 +417,59 @@ __device__ __forceinline__ int plan_of(const PlanSetArgs &a, int64_t e)
     return lo;
 }
 
+// The fill pass's listing queues: per wave, kFillQueue entries (tile | the
+// event's lane << 24) in LDS, and per event what its listings' slices need.
+constexpr int kFillQueue = 1280;
+struct FillEvent {
+    double ax, bx, ay, by, rbx, rby;
+    float hx, dymax;
+    int s_lo, s_hi;
+};
+// LDS bytes of the fill pass with queues, ahead of its three counter arrays
+constexpr unsigned kFillQueueLds =
+    256u * sizeof(FillEvent) + 4u * kFillQueue * sizeof(int) + 4u * sizeof(int);
+
+// listing_slices from a queued event's fields
+__device__ __forceinline__ void listing_slices(const TorParams &p, const TileBox &b,
+                                               const FillEvent &f, int &s0, int &s1)
+{
+    Track t;
+    t.ax = f.ax;
+    t.bx = f.bx;
+    t.ay = f.ay;
+    t.by = f.by;
+    t.hx = f.hx;
+    t.dymax = f.dymax;
+    s0 = max(f.s_lo, b.sa);
+    s1 = min(f.s_hi, b.sb);
+    window_slices(p, t, f.rbx, f.rby, b.x0, b.x1, b.y0, b.y1, s0, s1);
+}
+
 // plan_pass for a plan set, over all events of the set in one launch.  A
 // workgroup whose events all belong to one plan counts in LDS, as plan_pass
 // does for one plan; a workgroup across a plan boundary (one per boundary)
 // uses the global counters directly.  List entries carry the listing's
 // slices (the set's plans always have spans).
+//
+// The fill pass with queues (a.queue, workgroups of 256 counting in LDS):
+// each lane walks its event's tiles once, counting them in LDS and queueing
+// (tile, event) in its wave's queue; after the workgroup has reserved its
+// slots, the 64 lanes of a wave take the wave's queued listings in turn
+// (their slices and slots), so that a wave's work follows the listings of
+// its 64 events instead of its busiest event.  Listings past a full queue
+// are listed by their own lane afterwards, walking its tiles again.
 template <bool FILL>
 __global__ __launch_bounds__(256) void plan_set_pass(TorParams p, TileGeom g, PlanSetArgs a)
 {
-    extern __shared__ int sh[]; // [counts (FILL: + bases, ranks)][ntiles]
+    // LDS: with queues, [FillEvent x 256][4 queues of kFillQueue][4 queue
+    // lengths] first; then [counts (FILL: + bases, ranks)][ntiles]
+    extern __shared__ int4 sh4[];
+    const bool queued = FILL && a.queue;
+    char *const shb = reinterpret_cast<char *>(sh4);
+    int *const sh = reinterpret_cast<int *>(shb + (queued ? kFillQueueLds : 0u));
+    FillEvent *const fev = reinterpret_cast<FillEvent *>(shb);
+    int *const queue = reinterpret_cast<int *>(shb + 256u * sizeof(FillEvent));
+    int *const qlen = queue + 4 * kFillQueue;
     const int nt = a.ntiles;
     const int per = FILL ? 3 * nt : nt;
     const int64_t total = a.first[a.nplans];
@@ -433,9 +478,13 @@ __global__ __launch_bounds__(256) void plan_set_pass(TorParams p, TileGeom g, Pl
     const bool valid = e < total;
     const int kf = plan_of(a, e0), kl = plan_of(a, min(e0 + (int64_t)blockDim.x, total) - 1);
     const bool lds = a.lds && kl == kf;
+    const bool useq = queued && lds;
+    const int wv = (int)threadIdx.x >> 6, lane = (int)threadIdx.x & 63;
     if (lds) {
         for (int i = threadIdx.x; i < per; i += blockDim.x)
             sh[i] = 0;
+        if (useq && threadIdx.x < 4)
+            qlen[threadIdx.x] = 0;
         __syncthreads();
     }
     const int k = valid ? plan_of(a, e) : kf;
@@ -467,45 +516,106 @@ __global__ __launch_bounds__(256) void plan_set_pass(TorParams p, TileGeom g, Pl
                 flag_other(&a.other[k]);
         }
     }
-    if (!FILL || lds) {
+    int drop = -1; // with queues: this lane's first listing past its wave's full queue
+    if (useq) {
+        if (ok) {
+            fev[threadIdx.x] = FillEvent{t.ax, t.bx, t.ay, t.by, slope_rcp(t.bx),
+                                         slope_rcp(t.by), t.hx, t.dymax, ev.s_lo, ev.s_hi};
+            int kk = 0;
+            for_each_tile(q, g, ev, t, [&](int tile, int, int, int) {
+                atomicAdd(&lcnt[tile], 1);
+                // the active lanes take consecutive queue slots: one LDS
+                // atomic per wave, by the lowest of them
+                const uint64_t m = __ballot(1);
+                const int r = (int)__builtin_amdgcn_mbcnt_hi(
+                    (unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+                int base = 0;
+                if (r == 0)
+                    base = atomicAdd(&qlen[wv], __popcll(m));
+                base = __shfl(base, __builtin_ctzll(m), 64);
+                if (base + r < kFillQueue)
+                    queue[wv * kFillQueue + base + r] = tile | (lane << 24);
+                else if (drop < 0)
+                    drop = kk;
+                kk++;
+            });
+        }
+    } else if (!FILL || lds) {
         if (ok)
             for_each_tile(q, g, ev, t, [&](int tile, int, int, int) {
                 atomicAdd(lds ? &lcnt[tile] : &gcnt[tile], 1);
             });
-        if (lds) {
-            __syncthreads();
-            for (int tile = threadIdx.x; tile < nt; tile += blockDim.x) {
-                const int c = sh[tile];
-                if (c) {
-                    if (FILL)
-                        sh[nt + tile] = atomicAdd(&a.fill[(int64_t)kf * nt + tile], c);
-                    else
-                        atomicAdd(&a.cnt[(int64_t)kf * nt + tile], c);
-                }
+    }
+    if (lds) {
+        __syncthreads();
+        for (int tile = threadIdx.x; tile < nt; tile += blockDim.x) {
+            const int c = sh[tile];
+            if (c) {
+                if (FILL)
+                    sh[nt + tile] = atomicAdd(&a.fill[(int64_t)kf * nt + tile], c);
+                else
+                    atomicAdd(&a.cnt[(int64_t)kf * nt + tile], c);
             }
-            __syncthreads();
         }
+        __syncthreads();
     }
     if (!FILL)
         return;
-    const double rbx = slope_rcp(t.bx), rby = slope_rcp(t.by);
     const int *goff = a.off + (int64_t)k * nt;
     int *gfill = a.fill + (int64_t)k * nt;
     int *list = a.list[k];
+    // one listing of a workgroup counting in LDS: its slot, its entry
+    auto put = [&](int tile, const TileBox &b, int s0, int s1, int64_t lev) {
+        const int loc = atomicAdd(&lcnt[2 * nt + tile], 1);
+        const int rank = lcnt[nt + tile] + loc;
+        if (loc < lcnt[tile] && rank < gcnt[tile]) {
+            const int lo = s0 <= s1 ? s0 - b.sa : 1, hi = s0 <= s1 ? s1 - b.sa : 0;
+            list[goff[tile] + rank] =
+                (int)lev | (lo << kSpanEventBits) | (hi << (kSpanEventBits + 4));
+        } else {
+            atomicAdd(a.err, 1);
+        }
+    };
+    if (useq) {
+        // every valid lane of the workgroup is in plan kf
+        const int nq = min(qlen[wv], kFillQueue);
+        const int *wq = queue + wv * kFillQueue;
+        const int64_t le0 = e0 + 64 * wv - a.first[kf];
+        for (int i = lane; i < nq; i += 64) {
+            const int ent = wq[i];
+            const int tile = ent & 0xffffff, el = (int)((unsigned)ent >> 24);
+            const FillEvent f = fev[64 * wv + el];
+            const TileBox b = tile_box(q, g, tile);
+            int s0, s1;
+            listing_slices(q, b, f, s0, s1);
+            put(tile, b, s0, s1, le0 + el);
+        }
+        if (drop >= 0) {
+            const double rbx = slope_rcp(t.bx), rby = slope_rcp(t.by);
+            int kk = 0;
+            for_each_tile(q, g, ev, t, [&](int tile, int sl, int wx, int wy) {
+                if (kk++ >= drop) {
+                    const TileBox b = tile_box_at(q, g, sl, wx, wy);
+                    int s0, s1;
+                    listing_slices(q, b, ev, t, rbx, rby, s0, s1);
+                    put(tile, b, s0, s1, le);
+                }
+            });
+        }
+        return;
+    }
+    const double rbx = slope_rcp(t.bx), rby = slope_rcp(t.by);
     if (ok)
         for_each_tile(q, g, ev, t, [&](int tile, int sl, int wx, int wy) {
-            int rank;
-            bool fits;
+            const TileBox b = tile_box_at(q, g, sl, wx, wy);
             if (lds) {
-                const int loc = atomicAdd(&lcnt[2 * nt + tile], 1);
-                rank = lcnt[nt + tile] + loc;
-                fits = loc < lcnt[tile];
-            } else {
-                rank = atomicAdd(&gfill[tile], 1);
-                fits = true;
+                int s0, s1;
+                listing_slices(q, b, ev, t, rbx, rby, s0, s1);
+                put(tile, b, s0, s1, le);
+                return;
             }
-            if (fits && rank < gcnt[tile]) {
-                const TileBox b = tile_box_at(q, g, sl, wx, wy);
+            const int rank = atomicAdd(&gfill[tile], 1);
+            if (rank < gcnt[tile]) {
                 int s0, s1;
                 listing_slices(q, b, ev, t, rbx, rby, s0, s1);
                 const int lo = s0 <= s1 ? s0 - b.sa : 1, hi = s0 <= s1 ? s1 - b.sa : 0;
