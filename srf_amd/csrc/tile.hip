@@ -384,7 +384,16 @@ int plan_build_sets(const std::vector<PlanSet> &sets, hipStream_t st)
         a.ntiles = l.nt;
         a.nplans = ps.K;
         a.lds = 3u * sizeof(int) * l.nt <= 65536u;
-        a.queue = kFillQueueLds + 3u * sizeof(int) * l.nt <= 65536u;
+        // the longest listing queues that keep three workgroups of the fill
+        // pass per CU (with fewer, at C5's 2816 tiles, queues of 959 entries
+        // in 64 KB: 115 ms against 101 ms without queues)
+        a.queue = 0;
+        const unsigned qbudget = kLdsPerCU / 3u;
+        if (l.nt < (1 << 24) &&
+            fill_queue_lds(kFillQueueMin) + 3u * sizeof(int) * l.nt <= qbudget)
+            a.queue = std::min<int>(kFillQueue, (int)((qbudget - 3u * sizeof(int) * l.nt -
+                                                       fill_queue_lds(0)) /
+                                                      (4u * sizeof(int))));
         a.first = (const int64_t *)(tmp + l.head);
         a.rec = (float4 *const *)(tmp + l.head + sizeof(int64_t) * (ps.K + 1));
         a.list = (int *const *)(tmp + l.head + sizeof(int64_t) * (ps.K + 1) +
@@ -406,7 +415,7 @@ int plan_build_sets(const std::vector<PlanSet> &sets, hipStream_t st)
             TorParams p = ps.pls[0]->p;
             p.lors = ps.lors;
             const unsigned lds = (a.lds ? (fill ? 3u : 1u) * sizeof(int) * L[s].nt : 0u) +
-                                 (fill && a.queue ? kFillQueueLds : 0u);
+                                 (fill && a.queue ? fill_queue_lds(a.queue) : 0u);
             if (fill)
                 hipLaunchKernelGGL(plan_set_pass<true>, dim3(blocks_for(n, 256)), dim3(256), lds,
                                    st, p, L[s].g, a);

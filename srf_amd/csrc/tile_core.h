@@ -391,7 +391,8 @@ __global__ __launch_bounds__(256) void plan_pass(TorParams p, TileGeom g, PlanAr
 struct PlanSetArgs {
     int cols7, ntiles, nplans;
     int lds;                // the per-workgroup counters fit in LDS
-    int queue;              // fill pass: LDS also holds the listing queues (kFillQueue)
+    int queue;              // fill pass: LDS also holds listing queues of this many
+                            // entries per wave (0: no queues)
     const int64_t *first;   // [nplans + 1]
     float4 *const *rec;     // [nplans] walk records
     int *cnt;               // count pass: listings per (plan, tile); fill pass: read
@@ -417,17 +418,22 @@ __device__ __forceinline__ int plan_of(const PlanSetArgs &a, int64_t e)
     return lo;
 }
 
-// The fill pass's listing queues: per wave, kFillQueue entries (tile | the
-// event's lane << 24) in LDS, and per event what its listings' slices need.
-constexpr int kFillQueue = 1280;
+// The fill pass's listing queues: per wave, up to kFillQueue entries (tile |
+// the event's lane << 24) in LDS, and per event what its listings' slices
+// need.  A queue shorter than kFillQueueMin would overflow too often (a
+// wave lists ~700 (event, tile) pairs at 11 per event).
+constexpr int kFillQueue = 1280, kFillQueueMin = 768;
 struct FillEvent {
     double ax, bx, ay, by, rbx, rby;
     float hx, dymax;
     int s_lo, s_hi;
 };
-// LDS bytes of the fill pass with queues, ahead of its three counter arrays
-constexpr unsigned kFillQueueLds =
-    256u * sizeof(FillEvent) + 4u * kFillQueue * sizeof(int) + 4u * sizeof(int);
+// LDS bytes of the fill pass with queues of qcap entries, ahead of its
+// three counter arrays
+__host__ __device__ constexpr unsigned fill_queue_lds(int qcap)
+{
+    return 256u * sizeof(FillEvent) + 4u * (unsigned)qcap * sizeof(int) + 4u * sizeof(int);
+}
 
 // listing_slices from a queued event's fields
 __device__ __forceinline__ void listing_slices(const TorParams &p, const TileBox &b,
@@ -461,15 +467,16 @@ __device__ __forceinline__ void listing_slices(const TorParams &p, const TileBox
 template <bool FILL>
 __global__ __launch_bounds__(256) void plan_set_pass(TorParams p, TileGeom g, PlanSetArgs a)
 {
-    // LDS: with queues, [FillEvent x 256][4 queues of kFillQueue][4 queue
-    // lengths] first; then [counts (FILL: + bases, ranks)][ntiles]
+    // LDS: with queues, [FillEvent x 256][4 queues of a.queue entries][4
+    // queue lengths] first; then [counts (FILL: + bases, ranks)][ntiles]
     extern __shared__ int4 sh4[];
-    const bool queued = FILL && a.queue;
+    const bool queued = FILL && a.queue > 0;
+    const int qcap = a.queue;
     char *const shb = reinterpret_cast<char *>(sh4);
-    int *const sh = reinterpret_cast<int *>(shb + (queued ? kFillQueueLds : 0u));
+    int *const sh = reinterpret_cast<int *>(shb + (queued ? fill_queue_lds(qcap) : 0u));
     FillEvent *const fev = reinterpret_cast<FillEvent *>(shb);
     int *const queue = reinterpret_cast<int *>(shb + 256u * sizeof(FillEvent));
-    int *const qlen = queue + 4 * kFillQueue;
+    int *const qlen = queue + 4 * qcap;
     const int nt = a.ntiles;
     const int per = FILL ? 3 * nt : nt;
     const int64_t total = a.first[a.nplans];
@@ -533,8 +540,8 @@ __global__ __launch_bounds__(256) void plan_set_pass(TorParams p, TileGeom g, Pl
                 if (r == 0)
                     base = atomicAdd(&qlen[wv], __popcll(m));
                 base = __shfl(base, __builtin_ctzll(m), 64);
-                if (base + r < kFillQueue)
-                    queue[wv * kFillQueue + base + r] = tile | (lane << 24);
+                if (base + r < qcap)
+                    queue[wv * qcap + base + r] = tile | (lane << 24);
                 else if (drop < 0)
                     drop = kk;
                 kk++;
@@ -578,8 +585,8 @@ __global__ __launch_bounds__(256) void plan_set_pass(TorParams p, TileGeom g, Pl
     };
     if (useq) {
         // every valid lane of the workgroup is in plan kf
-        const int nq = min(qlen[wv], kFillQueue);
-        const int *wq = queue + wv * kFillQueue;
+        const int nq = min(qlen[wv], qcap);
+        const int *wq = queue + wv * qcap;
         const int64_t le0 = e0 + 64 * wv - a.first[kf];
         for (int i = lane; i < nq; i += 64) {
             const int ent = wq[i];

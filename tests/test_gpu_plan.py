@@ -308,9 +308,13 @@ def test_frame_gather_far_corner(cuda):
     assert H.rel_l2(p, pref) <= TOL64
 
 
-# (tile, chunk): the default tile (per-workgroup counters in LDS) and a tile
-# so small that the counters of two plans exceed the LDS (global counters)
-@pytest.mark.parametrize("tile,chunk", [((8, 32, 20), 2048), ((1, 2, 2), 50)])
+# (tile, chunk): the default tile (per-workgroup counters in LDS, the fill
+# pass with listing queues); small tiles whose dozens of listings per event
+# overflow the fill pass's queues (2 x 4 x 4, 1080 tiles) or shorten them
+# and overflow them (1 x 4 x 4, 2160 tiles: 768-entry queues); and a tile so
+# small that the counters of two plans exceed the LDS (global counters)
+@pytest.mark.parametrize("tile,chunk", [((8, 32, 20), 2048), ((2, 4, 4), 256),
+                                        ((1, 4, 4), 512), ((1, 2, 2), 50)])
 def test_plan_subsets_equal_single_plans(cuda, case, tile, chunk):
     """Plans of consecutive event ranges built together
     (srfx_tor_plan_create_subsets) equal plans built one by one: the same
