@@ -5,7 +5,8 @@ of 2.0x2.0x2.2 mm, 20M synthetic events (cylinder + 6 spheres, 530 ps FWHM,
 kernel_width 4 mm), 10 subsets.  A step is one OSEM iteration: every event is
 projected and backprojected once.  Per subset and axis group: the projection
 kernel, then the tiled backprojection of 1/p (events binned into image tiles
-once per subset, accumulated in LDS); then the image update.  Events are
+once per subset, accumulated in LDS), the axis groups' kernels on their own
+streams at once; then the image update.  Events are
 generated, preprocessed (axis groups, locality order inside each subset), the
 tile plans built and the efficiency map computed on the GPU before the timed
 region; inputs are resident in HBM when it starts.  --untiled times the
@@ -16,7 +17,8 @@ time being the timed iterations plus steps/3 of the tile-plan build (the
 workload's reconstruction runs 3 iterations over one event set, and the plans
 are built once per set; value_excl_plan_build leaves the build out);
 roofline = the dominant kernel's algorithmic bytes per launch over its mean
-launch time (HIP events on the launch stream); cpu_baseline = the FP32 CPU
+launch time (HIP events on the launch stream, in as many iterations again with
+each group's kernels launched alone); cpu_baseline = the FP32 CPU
 port of the reference kernels on a bounded sample of the same events.
 """
 import argparse
@@ -88,7 +90,8 @@ def _path_label(args, tiled):
         return ("per group: tor_kernel<kProj> projection, then tor_kernel<kBack> ratio "
                 "backprojection (FP32 global atomics); mlem_update per subset")
     if tiled:
-        return ("per group: frame_copy_kernel (op-frame copy of the image), "
+        return ("per group, the groups of a subset on their own streams at once: "
+                "frame_copy_kernel (op-frame copy of the image), "
                 "record_proj_kernel (projection into the plan's event records), "
                 "tile_scatter<kBack> (tiled backprojection of 1/p); mlem_update per subset")
     return ("per group: frame_copy_kernel, tor_kernel<kOsem> (fused projection + ratio "
@@ -299,14 +302,9 @@ def main():
             timings.setdefault(a, []).append((e0, e1, sub[a].soa.data_ptr()))
         return Image(out, img.center, img.size)
 
-    if osem.tiled and not args.unfused:
-        model.osem_subset = timed_tiled
-    elif not args.unfused:
-        model.osem_subset = timed_osem
     for _ in range(args.warmup):
         osem.iterate(image, 1)
     torch.cuda.synchronize()
-    timings.clear()
     t0 = time.perf_counter()
     s0, s1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     s0.record()
@@ -315,6 +313,17 @@ def main():
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     gpu_ms = s0.elapsed_time(s1)
+    # per-launch kernel times: as many iterations again, each group's kernels
+    # launched alone between HIP events (the timed iterations above run the
+    # groups of a subset at once)
+    if osem.tiled and not args.unfused:
+        model.osem_subset = timed_tiled
+    elif not args.unfused:
+        model.osem_subset = timed_osem
+    timings.clear()
+    osem.iterate(image, args.steps)
+    torch.cuda.synchronize()
+    model.osem_subset = orig
     # The tile plans are built once per event set, and this workload's
     # reconstruction runs C2_ITERATIONS iterations over one set: each timed
     # iteration carries 1/C2_ITERATIONS of the build (the reference builds

@@ -32,6 +32,43 @@ def group_frame(image_shape, center, size, strides, axis):
     return grid, cen, siz, st
 
 
+# Side streams per device for the concurrent group steps (default priority).
+_SIDE_STREAMS = {}
+
+
+def _concurrent(calls, tensors):
+    """Run calls[0] on the current stream and every other call on a side
+    stream of its own, each after the work queued so far on the current
+    stream; the current stream then waits for all of them (a fork and a join
+    through events, so the whole can be captured in a HIP graph).  The
+    tensors the calls use are marked as used on the side streams.  Groups of
+    one subset run together this way: a group's kernels leave the GPU part
+    idle while their last workgroups finish, which the other group's fill."""
+    if len(calls) <= 1:
+        for c in calls:
+            c()
+        return
+    main = torch.cuda.current_stream()
+    side = _SIDE_STREAMS.setdefault(main.device.index, [])
+    while len(side) < len(calls) - 1:
+        side.append(torch.cuda.Stream(device=main.device))
+    fork = torch.cuda.Event()
+    fork.record(main)
+    joins = []
+    for st, c in zip(side, calls[1:]):
+        st.wait_event(fork)
+        with torch.cuda.stream(st):
+            for t in tensors:
+                t.record_stream(st)
+            c()
+        e = torch.cuda.Event()
+        e.record(st)
+        joins.append(e)
+    calls[0]()
+    for e in joins:
+        main.wait_event(e)
+
+
 def _world(image):
     data = image.data
     _check_dev(data, "image")
@@ -268,13 +305,18 @@ class SplitLoRsModel:
         x = _world(image)
         if out is None:
             out = torch.zeros_like(x)
+        if plans is not None:
+            # the groups' steps run concurrently (their scatters add into out
+            # with atomics)
+            steps = [(a, self._planned(plans, a, len(data[a]), out)) for a in self.AXIS
+                     if len(data[a])]
+            _concurrent([lambda a=a, pl=pl: self.plan_osem_group(pl, x, out, a)
+                         for a, pl in steps], (x, out))
+            return Image(out, image.center, image.size)
         for a in self.AXIS:
             lm = data[a]
             lors, n, ld = _soa(lm.soa, 10)
             if n == 0:
-                continue
-            if plans is not None:
-                self.plan_osem_group(self._planned(plans, a, n, out), x, out, a)
                 continue
             src, g_in = self._source(image, a)
             g_out = self._grid(Image(out, image.center, image.size), a)
