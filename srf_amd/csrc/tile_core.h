@@ -79,29 +79,22 @@ __device__ __forceinline__ double row_extent(const TorParams &p, const Track &t)
 }
 
 // Windows wx0..wx1, wy0..wy1 holding every voxel the event can visit in the
-// slices s0..s1.  The patch origin of slice s is (int)u(s) - half with u
-// affine in s (cross_at), or the reference's exact origin, which differs from
-// it by at most one voxel; the patch spans P voxels from its origin.
+// slices s0..s1: the image's columns and rows that the padded ellipse covers
+// over the slab (u and w are affine in s).  Every walk adds only to voxels
+// whose centre lies inside that ellipse, whatever the patch origin (the
+// walks also stay inside the patch, which at these pitches always contains
+// the ellipse's columns and rows, so intersecting with it as well lists no
+// fewer tiles).
 __device__ __forceinline__ bool footprint(const TorParams &p, const TileGeom &g, const Track &t,
                                           int s0, int s1, int &wx0, int &wx1, int &wy0, int &wy1)
 {
-    int a0 = trunc_clamped(t.ax + (double)s0 * t.bx) - p.half;
-    int a1 = trunc_clamped(t.ax + (double)s1 * t.bx) - p.half;
-    int b0 = trunc_clamped(t.ay + (double)s0 * t.by) - p.half;
-    int b1 = trunc_clamped(t.ay + (double)s1 * t.by) - p.half;
-    int x0 = max(min(a0, a1) - 1, 0), x1 = min(max(a0, a1) + p.patch, p.gx - 1);
-    int y0 = max(min(b0, b1) - 1, 0), y1 = min(max(b0, b1) + p.patch, p.gy - 1);
-    // and the columns / rows the padded ellipse covers over the slab (the
-    // only ones the walks add to; u and w are affine in s)
-    {
-        const double u0 = t.ax + (double)s0 * t.bx, u1 = t.ax + (double)s1 * t.bx;
-        const double w0 = t.ay + (double)s0 * t.by, w1 = t.ay + (double)s1 * t.by;
-        const double ex = col_extent(p, t), ey = row_extent(p, t);
-        x0 = max(x0, trunc_clamped(ceil(fmin(u0, u1) - 0.5 - ex)));
-        x1 = min(x1, trunc_clamped(floor(fmax(u0, u1) - 0.5 + ex)));
-        y0 = max(y0, trunc_clamped(ceil(fmin(w0, w1) - 0.5 - ey)));
-        y1 = min(y1, trunc_clamped(floor(fmax(w0, w1) - 0.5 + ey)));
-    }
+    const double u0 = t.ax + (double)s0 * t.bx, u1 = t.ax + (double)s1 * t.bx;
+    const double w0 = t.ay + (double)s0 * t.by, w1 = t.ay + (double)s1 * t.by;
+    const double ex = col_extent(p, t), ey = row_extent(p, t);
+    const int x0 = max(0, trunc_clamped(ceil(fmin(u0, u1) - 0.5 - ex)));
+    const int x1 = min(p.gx - 1, trunc_clamped(floor(fmax(u0, u1) - 0.5 + ex)));
+    const int y0 = max(0, trunc_clamped(ceil(fmin(w0, w1) - 0.5 - ey)));
+    const int y1 = min(p.gy - 1, trunc_clamped(floor(fmax(w0, w1) - 0.5 + ey)));
     if (x0 > x1 || y0 > y1)
         return false;
     wx0 = x0 / g.Wx;
