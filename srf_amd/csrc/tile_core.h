@@ -191,9 +191,12 @@ __device__ __forceinline__ double slope_rcp(double b)
 // Narrow [s0, s1] to a superset of the slices at which the walk can add to
 // a voxel of columns x0..x1 and rows y0..y1: u(s) within the column extent
 // of the window, [x0 + 0.5 - hx', x1 + 0.5 + hx'], and w(s) within the row
-// extent; and the patch [trunc(u) - half, trunc(u) - half + P) meeting it
-// (2 voxels of slack for the exact origin).  rbx, rby: slope_rcp of the
-// track's slopes.
+// extent (every walk adds only to voxels inside the padded ellipse).  The
+// walks also stay inside the patch, but its condition on u, [x0 - P + half
+// - 2, x1 + half + 2] with 2 voxels of slack for the exact origin, follows
+// from the column extent's whenever hx' < min(half, P - half) + 1.5 voxels,
+// as for the candidate-box events; it is not tested.  rbx, rby: slope_rcp
+// of the track's slopes.
 __device__ __forceinline__ void window_slices(const TorParams &p, const Track &t, double rbx,
                                               double rby, int x0, int x1, int y0, int y1, int &s0,
                                               int &s1)
@@ -201,9 +204,6 @@ __device__ __forceinline__ void window_slices(const TorParams &p, const Track &t
     const double ex = col_extent(p, t), ey = row_extent(p, t);
     axis_slices(t.ax, t.bx, rbx, x0 + 0.5 - ex, x1 + 0.5 + ex, s0, s1);
     axis_slices(t.ay, t.by, rby, y0 + 0.5 - ey, y1 + 0.5 + ey, s0, s1);
-    const int P = p.patch, half = p.half;
-    axis_slices(t.ax, t.bx, rbx, (double)(x0 - P + half - 2), (double)(x1 + half + 2), s0, s1);
-    axis_slices(t.ay, t.by, rby, (double)(y0 - P + half - 2), (double)(y1 + half + 2), s0, s1);
 }
 
 // The slices s0..s1 a listing walks: the event's crossed slices in the
