@@ -534,14 +534,19 @@ int plan_build_sets(const std::vector<PlanSet> &sets, hipStream_t st)
     }
     if ((ctr_n > 0 && hipMemcpyAsync(ctr_d, ctr.data(), sizeof(int) * ctr_n,
                                      hipMemcpyHostToDevice, st) != hipSuccess) ||
-        hipMemcpyAsync(tmp, head.data(), head_b, hipMemcpyHostToDevice, st) != hipSuccess ||
-        (!ch.empty() && hipMemcpyAsync(blk1, ch.data(), sizeof(TileChunk) * ch.size(),
-                                       hipMemcpyHostToDevice, st) != hipSuccess)) {
+        hipMemcpyAsync(tmp, head.data(), head_b, hipMemcpyHostToDevice, st) != hipSuccess) {
         rc = SRFX_ELAUNCH;
         goto done;
     }
     if ((rc = launch(true)))
         goto done;
+    // the chunk table, read by the sort only: its copy from pageable memory
+    // is staged while the fill pass runs
+    if (!ch.empty() && hipMemcpyAsync(blk1, ch.data(), sizeof(TileChunk) * ch.size(),
+                                      hipMemcpyHostToDevice, st) != hipSuccess) {
+        rc = SRFX_ELAUNCH;
+        goto done;
+    }
     // listings of each chunk ordered by slice count, longest first: one
     // launch for the chunks of every plan.  It runs before the fill's error
     // count is read (one host synchronisation less): it only permutes the
