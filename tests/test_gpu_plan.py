@@ -370,10 +370,12 @@ def test_plan_subset_groups_equal_subsets(cuda, case):
     args = (H.TOR_GRID, H.TOR_CENTER, H.TOR_SIZE)
     model = (H.KW, True, H.TOF_BIN, H.TOF_SIGMA2)
     specs = [dict(lors=L, bounds=[0, 1000, n], tile=(8, 32, 20)),
+             dict(lors=L[:, :0], bounds=[0, 0, 0], tile=(8, 32, 20)),  # a group without events
              dict(lors=L2, bounds=[0, 3, 3, 2000, n], tile=(12, 32, 10), axis="y")]
     groups = [dict(s, grid=args[0], center=args[1], size=args[2]) for s in specs]
     built = TorPlan.subset_groups(groups, *model)
-    assert [len(b) for b in built] == [2, 4]
+    assert [len(b) for b in built] == [2, 2, 4]
+    assert all(pl.n == 0 and pl.info() == (0, 0) for pl in built[1])
     for spec, plans in zip(specs, built):
         ref = TorPlan.subsets(spec["lors"], spec["bounds"], *args, *model, tile=spec["tile"])
         for pl, rp in zip(plans, ref):
